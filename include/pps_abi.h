@@ -1,0 +1,170 @@
+/*
+ * pps_abi.h -- C ABI of the MI355X Poisson Krylov library (libpps.so).
+ *
+ * The library replaces the device side of the reference's solver entry call
+ *     T_Solver solver(blockGrid, exactSolutionAndBCs, communicator, alpakaHelper);
+ *     solver(fieldX, fieldB);
+ * (solverPoissonMPI_alpaka/src/main.cpp:90,104; BiCGstabAlpaka.hpp:68-475) and
+ * of the CPU form solver.setProblem(x,b); solver(x,b,operatorA)
+ * (solverPoissonMPI_CPU/src/main.cpp:83,94,99).  The reference has no C ABI:
+ * these entry points are what an FFI (ctypes, or the C++ facade in
+ * include/pps/) binds for that path.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *  - every call returns 0 on success or a negative PPS_ERR_* code; the text of
+ *    the last error is pps_last_error(ctx) (ctx may be NULL for pps_create);
+ *  - host fields are dense guarded arrays of (n0+2)(n1+2)(n2+2) doubles, x
+ *    fastest, the reference host layout (iterativeSolverBase.hpp:34-35);
+ *  - the context owns its device memory and one HIP stream; calls on one
+ *    context are not thread safe; contexts are independent;
+ *  - one GPU: the rank grid must be 1x1x1 (the MPI halo exchange of
+ *    communicationMPI.hpp becomes the on-device ghost fill).
+ */
+#ifndef PPS_ABI_H
+#define PPS_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PPS_ABI_VERSION 1
+
+enum {
+    PPS_OK = 0,
+    PPS_ERR_ARG = -1,          /* invalid argument                                 */
+    PPS_ERR_HIP = -2,          /* HIP runtime error                                */
+    PPS_ERR_NOMEM = -3,        /* device allocation failed                         */
+    PPS_ERR_UNSUPPORTED = -4,  /* configuration outside what this build implements */
+    PPS_ERR_STATE = -5,        /* call out of order (e.g. NN not trained)          */
+    PPS_ERR_NOGPU = -6         /* no HIP device                                    */
+};
+
+/* Grid description: the arguments of BlockGrid's constructor
+ * (solverPoissonMPI_CPU/include/blockGrid.hpp:15). */
+typedef struct pps_grid {
+    int dim;            /* DIM; this build implements 3                            */
+    int nranks[3];      /* rank grid; must be {1,1,1} on one GPU                   */
+    int my_rank;        /* 0                                                       */
+    int npglobal[3];    /* data points per direction, incl. Dirichlet layers       */
+    double ds[3];       /* mesh spacing                                            */
+    double origin[3];   /* coordinates of the first data point                     */
+    int guards[3];      /* ghost width; the 7-point stencil needs {1,1,1}          */
+    int bcs_type[6];    /* 0 Dirichlet, 1 Neumann; order x- x+ y- y+ z- z+          */
+    double bcs_value[6];/* carried for signature parity; values come from fields   */
+} pps_grid;
+
+/* fields addressable through the ABI (FP64, device resident) */
+enum {
+    PPS_FIELD_X = 0,    /* solution / initial guess with Dirichlet values          */
+    PPS_FIELD_B = 1,    /* right-hand side                                          */
+    PPS_FIELD_T0 = 2,   /* scratch fields for the kernel-level entry points         */
+    PPS_FIELD_T1 = 3,
+    PPS_FIELD_T2 = 4,
+    PPS_FIELD_T3 = 5,
+    PPS_NUM_FIELDS = 6
+};
+
+/* solver-chain stage kinds: the template chain of inputParam.hpp:22-31 */
+enum {
+    PPS_NONE = 0,        /* NoneSolver (noneSolver.hpp:24-27)                      */
+    PPS_CG = 1,          /* BaseCG (baseCG.hpp:44-260)                             */
+    PPS_BICGSTAB = 2,    /* BiCGSTAB (BiCGSTAB.hpp:55-322; BiCGstabAlpaka.hpp)     */
+    PPS_CHEBYSHEV = 3,   /* ChebyshevIteration (chebyshevIteration.hpp:48-140)     */
+    PPS_JACOBI = 4,      /* diagonal preconditioner (no reference counterpart)     */
+    PPS_FCG = 5,         /* flexible CG, truncation 1 (no reference counterpart)   */
+    PPS_IR = 6,          /* FP64 outer refinement of an FP32 inner stage (new)     */
+    PPS_NN = 7           /* learned multilevel preconditioner (new)                */
+};
+
+/* One stage of the solver chain.  Stage 0 is the main solver
+ * (isMainLoop = true), stage s+1 preconditions stage s. */
+typedef struct pps_stage {
+    int kind;            /* PPS_*                                                   */
+    int precision;       /* 64 or 32: vector precision (T_data / T_data_chebyshev)  */
+    double tol;          /* tolerance * tollScalingFactor (baseCG.hpp:22)           */
+    int max_iter;        /* maxIteration; Chebyshev degree chebyshevMax             */
+    double cheb_eps;     /* epsilon (solverSetup.hpp)                               */
+    double cheb_rescale_min; /* rescaleEigMin                                       */
+    double cheb_rescale_max; /* rescaleEigMax                                       */
+    int cheb_local;      /* alpaka `local` eigenvalue variant                       */
+    int stall_window;    /* PPS_NN: iterations per stall test (0 = never fall back) */
+    double stall_ratio;  /* PPS_NN: fall back if residual drop over the window > this */
+} pps_stage;
+
+typedef struct pps_result {
+    int iterations;                 /* getNumIterationFinal                         */
+    int outer_iterations;           /* PPS_IR outer refinement steps                */
+    long long precond_iterations;   /* getNumIterationPreconditionerFinal           */
+    double error_from_iteration;    /* getErrorFromIteration (recursive residual)   */
+    double error_compute_operator;  /* getErrorComputeOperator (true residual)      */
+    double norm_b;                  /* normalisation factor ||b_adj||               */
+    double seconds;                 /* device time of the solve (HIP events)        */
+    int fallback_switches;          /* learned preconditioner -> fallback switches   */
+    int history_len;                /* entries available from pps_get_history       */
+} pps_result;
+
+/* ---- context ----------------------------------------------------------- */
+int pps_create(const pps_grid* grid, void** ctx);
+void pps_destroy(void* ctx);
+const char* pps_last_error(void* ctx);
+int pps_abi_version(void);
+/* guarded extent (n+2 per direction) and the device row pitch in elements */
+int pps_get_extent(void* ctx, int ng[3]);
+
+/* ---- fields (replaces the host<->device copies of BiCGstabAlpaka.hpp:95-112,473-474) */
+int pps_upload(void* ctx, int field, const double* host_guarded);
+int pps_download(void* ctx, int field, double* host_guarded);
+int pps_fill(void* ctx, int field, double value);
+/* du/dn data of a Neumann face (trueSolutionDdir on the face's data points);
+ * plane of the two other directions' guarded extents, lower direction fastest */
+int pps_set_neumann_flux(void* ctx, int face, const double* host_plane);
+
+/* ---- the solve (BaseCG / BiCGSTAB / ChebyshevIteration operator()) ------ */
+int pps_solve(void* ctx, const pps_stage* chain, int nstages, pps_result* res);
+int pps_get_history(void* ctx, double* out, int cap);
+/* wait for the context's stream */
+int pps_synchronize(void* ctx);
+
+/* ---- kernel-level entry points (parity tests, halo/ghost semantics) ----- */
+/* y = A x on the solver region (matrixFreeOperatorA.hpp:22-39) */
+int pps_apply_A(void* ctx, int y, int x);
+/* ghost fill: Neumann mirror (iterativeSolverBase.hpp:62-169); with_flux adds
+ * the -+2 ds flux / norm term of data fields.  Replaces CommunicatorMPI on one GPU. */
+int pps_ghost_fill(void* ctx, int field, int with_flux, double norm);
+/* r = b - A x on the solver region, returns ||r|| (iterativeSolverBase.hpp:236-280) */
+int pps_residual(void* ctx, int r, int x, int b, double* norm);
+/* sum a.b over the solver region, deterministic single-pass grid reduction */
+int pps_dot(void* ctx, int a, int b, double* out);
+/* ||b_adj|| of the boundary-adjusted right-hand side (iterativeSolverBase.hpp:171-234) */
+int pps_norm_b_adjusted(void* ctx, double* out);
+
+/* ---- seeded right-hand sides, written into PPS_FIELD_B as b = -f -------- */
+int pps_rhs_normal(void* ctx, uint64_t seed);
+int pps_rhs_gauss(void* ctx, uint64_t seed, int nsrc, double wmin, double wmax);
+/* accumulate: 0 overwrite, 1 add to the current b */
+int pps_rhs_spectrum(void* ctx, uint64_t seed, int kmax, int accumulate);
+
+/* ---- measurement ------------------------------------------------------- */
+/* kernel ids for pps_time_kernel */
+enum {
+    PPS_K_CG_STENCIL = 0,   /* fused p = z + beta p, Ap = A p, p.Ap       */
+    PPS_K_CG_UPDATE = 1,    /* x += alpha p, r -= alpha Ap, r.r, r.z      */
+    PPS_K_RESIDUAL = 2,     /* r = b - A x, r.r                           */
+    PPS_K_COPY = 3,         /* plain copy of the solver region (reference) */
+    PPS_K_CHEB_SWEEP = 4,   /* Chebyshev sweep                            */
+    PPS_K_CG_STENCIL_F32 = 5,
+    PPS_K_CG_UPDATE_F32 = 6
+};
+/* Average device time (ms) of `reps` back-to-back launches of one kernel on
+ * the context stream, measured with HIP events; *bytes = algorithmic bytes
+ * moved per launch. */
+int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes);
+/* tiling knobs of the stencil kernels (0 keeps the default) */
+int pps_set_tiling(void* ctx, int kz, int ty);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

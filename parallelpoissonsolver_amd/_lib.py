@@ -1,0 +1,247 @@
+"""ctypes binding of include/pps_abi.h (libpps.so, built in-tree for gfx950).
+
+There is no fallback: if the HIP library is missing or no GPU is present the
+calls raise, so a silent CPU path can never stand in for the device kernels.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpps.so")
+
+PPS_OK = 0
+ERRORS = {-1: "PPS_ERR_ARG", -2: "PPS_ERR_HIP", -3: "PPS_ERR_NOMEM", -4: "PPS_ERR_UNSUPPORTED", -5: "PPS_ERR_STATE",
+          -6: "PPS_ERR_NOGPU"}
+
+FIELD_X, FIELD_B, FIELD_T0, FIELD_T1, FIELD_T2, FIELD_T3 = range(6)
+NONE, CG, BICGSTAB, CHEBYSHEV, JACOBI, FCG, IR, NN = range(8)
+K_CG_STENCIL, K_CG_UPDATE, K_RESIDUAL, K_COPY, K_CHEB_SWEEP, K_CG_STENCIL_F32, K_CG_UPDATE_F32 = range(7)
+
+
+class PPSError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class CGrid(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("nranks", ctypes.c_int * 3),
+        ("my_rank", ctypes.c_int),
+        ("npglobal", ctypes.c_int * 3),
+        ("ds", ctypes.c_double * 3),
+        ("origin", ctypes.c_double * 3),
+        ("guards", ctypes.c_int * 3),
+        ("bcs_type", ctypes.c_int * 6),
+        ("bcs_value", ctypes.c_double * 6),
+    ]
+
+
+class CStage(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("tol", ctypes.c_double),
+        ("max_iter", ctypes.c_int),
+        ("cheb_eps", ctypes.c_double),
+        ("cheb_rescale_min", ctypes.c_double),
+        ("cheb_rescale_max", ctypes.c_double),
+        ("cheb_local", ctypes.c_int),
+        ("stall_window", ctypes.c_int),
+        ("stall_ratio", ctypes.c_double),
+    ]
+
+
+class CResult(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int),
+        ("outer_iterations", ctypes.c_int),
+        ("precond_iterations", ctypes.c_longlong),
+        ("error_from_iteration", ctypes.c_double),
+        ("error_compute_operator", ctypes.c_double),
+        ("norm_b", ctypes.c_double),
+        ("seconds", ctypes.c_double),
+        ("fallback_switches", ctypes.c_int),
+        ("history_len", ctypes.c_int),
+    ]
+
+
+# every symbol declared in include/pps_abi.h, with its ctypes signature
+_VP = ctypes.c_void_p
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+SIGNATURES = {
+    "pps_abi_version": ([], ctypes.c_int),
+    "pps_create": ([ctypes.POINTER(CGrid), ctypes.POINTER(_VP)], ctypes.c_int),
+    "pps_destroy": ([_VP], None),
+    "pps_last_error": ([_VP], ctypes.c_char_p),
+    "pps_get_extent": ([_VP, _IP], ctypes.c_int),
+    "pps_upload": ([_VP, ctypes.c_int, _DP], ctypes.c_int),
+    "pps_download": ([_VP, ctypes.c_int, _DP], ctypes.c_int),
+    "pps_fill": ([_VP, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+    "pps_set_neumann_flux": ([_VP, ctypes.c_int, _DP], ctypes.c_int),
+    "pps_solve": ([_VP, ctypes.POINTER(CStage), ctypes.c_int, ctypes.POINTER(CResult)], ctypes.c_int),
+    "pps_get_history": ([_VP, _DP, ctypes.c_int], ctypes.c_int),
+    "pps_synchronize": ([_VP], ctypes.c_int),
+    "pps_apply_A": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "pps_ghost_fill": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+    "pps_residual": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _DP], ctypes.c_int),
+    "pps_dot": ([_VP, ctypes.c_int, ctypes.c_int, _DP], ctypes.c_int),
+    "pps_norm_b_adjusted": ([_VP, _DP], ctypes.c_int),
+    "pps_rhs_normal": ([_VP, ctypes.c_uint64], ctypes.c_int),
+    "pps_rhs_gauss": ([_VP, ctypes.c_uint64, ctypes.c_int, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+    "pps_rhs_spectrum": ([_VP, ctypes.c_uint64, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "pps_time_kernel": ([_VP, ctypes.c_int, ctypes.c_int, _DP, _DP], ctypes.c_int),
+    "pps_set_tiling": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libpps.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libpps.so not built at {LIB_PATH}: run `python3 -m parallelpoissonsolver_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+class Context:
+    """One device context (pps_create): fields X, B (+ scratch T0..T3) on the GPU."""
+
+    def __init__(self, npglobal, ds, origin=(0.0, 0.0, 0.0), bcs=(0, 0, 0, 0, 0, 0), nranks=(1, 1, 1), my_rank=0,
+                 guards=(1, 1, 1)):
+        g = CGrid()
+        g.dim = 3
+        g.nranks[:] = list(nranks)
+        g.my_rank = my_rank
+        g.npglobal[:] = [int(v) for v in npglobal]
+        g.ds[:] = [float(v) for v in ds]
+        g.origin[:] = [float(v) for v in origin]
+        g.guards[:] = list(guards)
+        g.bcs_type[:] = [int(v) for v in bcs]
+        self.grid = g
+        self.h = _VP()
+        rc = lib().pps_create(ctypes.byref(g), ctypes.byref(self.h))
+        if rc != 0:
+            raise PPSError(rc, lib().pps_last_error(None).decode())
+        self.npglobal = tuple(int(v) for v in npglobal)
+        self.ng = tuple(n + 2 for n in self.npglobal)
+        self.shape = (self.ng[2], self.ng[1], self.ng[0])
+        self.bcs = tuple(int(v) for v in bcs)
+
+    def close(self):
+        if self.h:
+            lib().pps_destroy(self.h)
+            self.h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc < 0:
+            raise PPSError(rc, lib().pps_last_error(self.h).decode())
+        return rc
+
+    def _arr(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != self.shape[0] * self.shape[1] * self.shape[2]:
+            raise ValueError(f"field must have {self.shape} guarded points")
+        return a
+
+    def upload(self, field, a):
+        a = self._arr(a)
+        self._chk(lib().pps_upload(self.h, field, _dp(a)))
+
+    def download(self, field):
+        a = np.empty(self.shape)
+        self._chk(lib().pps_download(self.h, field, _dp(a)))
+        return a
+
+    def fill(self, field, v):
+        self._chk(lib().pps_fill(self.h, field, float(v)))
+
+    def set_flux(self, face, plane):
+        plane = np.ascontiguousarray(plane, dtype=np.float64)
+        self._chk(lib().pps_set_neumann_flux(self.h, face, _dp(plane)))
+
+    def solve(self, chain):
+        st = (CStage * len(chain))()
+        for i, c in enumerate(chain):
+            st[i].kind = c["kind"]
+            st[i].precision = c.get("precision", 64)
+            st[i].tol = c.get("tol", 1e-10)
+            st[i].max_iter = c.get("maxit", 100000)
+            st[i].cheb_eps = c.get("cheb_eps", 1e-4)
+            st[i].cheb_rescale_min = c.get("cheb_rmin", 500.0)
+            st[i].cheb_rescale_max = c.get("cheb_rmax", 1 - 1e-4)
+            st[i].cheb_local = c.get("cheb_local", 0)
+            st[i].stall_window = c.get("stall_window", 0)
+            st[i].stall_ratio = c.get("stall_ratio", 0.0)
+        res = CResult()
+        self._chk(lib().pps_solve(self.h, st, len(chain), ctypes.byref(res)))
+        out = {k: getattr(res, k) for k, _ in CResult._fields_}
+        n = res.history_len
+        hist = np.zeros(max(n, 1))
+        if n > 0:
+            self._chk(lib().pps_get_history(self.h, _dp(hist), n))
+        out["history"] = hist[:n]
+        return out
+
+    def apply_A(self, y, x):
+        self._chk(lib().pps_apply_A(self.h, y, x))
+
+    def ghost_fill(self, field, with_flux=False, norm=1.0):
+        self._chk(lib().pps_ghost_fill(self.h, field, int(with_flux), float(norm)))
+
+    def residual(self, r, x, b):
+        v = ctypes.c_double()
+        self._chk(lib().pps_residual(self.h, r, x, b, ctypes.byref(v)))
+        return v.value
+
+    def dot(self, a, b):
+        v = ctypes.c_double()
+        self._chk(lib().pps_dot(self.h, a, b, ctypes.byref(v)))
+        return v.value
+
+    def norm_b_adjusted(self):
+        v = ctypes.c_double()
+        self._chk(lib().pps_norm_b_adjusted(self.h, ctypes.byref(v)))
+        return v.value
+
+    def rhs_normal(self, seed):
+        self._chk(lib().pps_rhs_normal(self.h, seed))
+
+    def rhs_gauss(self, seed, nsrc=64, wmin=0.01, wmax=0.1):
+        self._chk(lib().pps_rhs_gauss(self.h, seed, nsrc, wmin, wmax))
+
+    def rhs_spectrum(self, seed, kmax=16, accumulate=False):
+        self._chk(lib().pps_rhs_spectrum(self.h, seed, kmax, int(accumulate)))
+
+    def time_kernel(self, kernel, reps=20):
+        ms, nbytes = ctypes.c_double(), ctypes.c_double()
+        self._chk(lib().pps_time_kernel(self.h, kernel, reps, ctypes.byref(ms), ctypes.byref(nbytes)))
+        return ms.value, nbytes.value
+
+    def set_tiling(self, kz=0, ty=0):
+        self._chk(lib().pps_set_tiling(self.h, kz, ty))
+
+    def synchronize(self):
+        self._chk(lib().pps_synchronize(self.h))

@@ -1,0 +1,1440 @@
+// pps_host.hip -- host side of libpps: the context behind the C ABI, the
+// solver chain (main solver + nested preconditioners, the template chain of
+// the reference's inputParam.hpp), and the launch control of the Krylov loops.
+//
+// Launch control.  All Krylov scalars live on the device, so an iteration is a
+// fixed sequence of kernel launches with no host round trip.  The host queues
+// iterations in batches and keeps two batches in flight: after queuing batch
+// k+1 it waits for the event of batch k and reads the solver's convergence
+// flag that batch k copied to pinned memory.  Every loop kernel is gated on
+// that flag, so iterations queued past convergence retire as empty launches
+// and the iterate sequence, iteration count and stopping rule are exactly
+// those of the reference loop (baseCG.hpp:115-228, BiCGSTAB.hpp:131-292).
+#include "../../include/pps_abi.h"
+#include "pps_kernels.hpp"
+#include "pps_nn.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace pps {
+
+static thread_local std::string g_create_error;
+
+struct Field {
+    void* p = nullptr;
+    int prec = 64;
+    double* d() const { return static_cast<double*>(p); }
+    float* f() const { return static_cast<float*>(p); }
+};
+
+struct Ctx;
+
+// -------------------------------------------------------------- solvers
+
+struct Stage {
+    Ctx& C;
+    pps_stage cfg;
+    int level;
+    Stage* pre = nullptr;     // preconditioner stage (null = identity)
+    Scal* S = nullptr;        // device scalars
+    long long prec_iters = 0; // iterations spent in this stage as a preconditioner
+    long long calls = 0;
+    Stage(Ctx& c, const pps_stage& s, int l) : C(c), cfg(s), level(l) {}
+    virtual ~Stage() = default;
+    int prec() const { return cfg.precision == 32 ? 32 : 64; }
+    // pointwise preconditioner: out = c * in
+    virtual bool pointwise(double* c) const { return false; }
+    // needs host control per application (Krylov preconditioners)
+    virtual bool host_driven() const { return false; }
+    // operator()(out, in) as a preconditioner; kernels gated on *gate
+    virtual int apply(Field out, Field in, const int* gate) = 0;
+    // operator()(x, b) as the main solver on the context's X and B
+    virtual int solve_main() { return PPS_ERR_UNSUPPORTED; }
+};
+
+struct Ctx {
+    pps_grid grid{};
+    int np[3]{}, ng[3]{};
+    Layout L64{}, L32{};
+    Region solver{}, data{}, box{};
+    int lim_data[6]{}, lim_solver[6]{};
+    double coef[3]{};
+    double eig_local[2]{}, eig_global[2]{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_batch[2]{};
+    hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
+    RedBuf rb{};
+    Faces faces{};
+    double* flux_d[6]{};
+    int fng[6][2]{};
+    Field fields[PPS_NUM_FIELDS];
+    std::map<std::string, Field> work;
+    Scal* scal_d = nullptr;  // pool of device scalar blocks
+    int nscal = 16;
+    int* flag_h = nullptr;   // pinned: [2][4] (iter, done, outer, pad) per in-flight batch
+    Scal* scal_h = nullptr;  // pinned copy of one Scal
+    double* hist_d = nullptr;
+    int hist_cap = 0;
+    int hist_len = 0;
+    int kz = 32;
+    std::string err;
+    std::unique_ptr<NNPrecond> nn;
+
+    int fail(int code, const char* fmt, ...)
+    {
+        char buf[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+    int hip(hipError_t e, const char* what)
+    {
+        if (e == hipSuccess) return PPS_OK;
+        return fail(e == hipErrorOutOfMemory ? PPS_ERR_NOMEM : PPS_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    }
+    const Layout& lay(int prec) const { return prec == 32 ? L32 : L64; }
+    Geo geo(const Region& R, int prec) const { return make_geo(lay(prec), R, coef, kz); }
+    Geo gsolver(int prec) const { return geo(solver, prec); }
+    Geo gbox(int prec) const { return geo(box, prec); }
+    Geo gdata(int prec) const { return geo(data, prec); }
+
+    int alloc_field(Field& f, int prec)
+    {
+        const size_t bytes = (size_t)lay(prec).total * (prec == 32 ? 4 : 8);
+        f.prec = prec;
+        int rc = hip(hipMalloc(&f.p, bytes), "hipMalloc field");
+        if (rc) return rc;
+        return hip(hipMemsetAsync(f.p, 0, bytes, stream), "hipMemsetAsync field");
+    }
+    int get_work(const std::string& key, int prec, Field& out)
+    {
+        const std::string k = key + (prec == 32 ? "/32" : "/64");
+        auto it = work.find(k);
+        if (it != work.end()) {
+            out = it->second;
+            return PPS_OK;
+        }
+        Field f;
+        int rc = alloc_field(f, prec);
+        if (rc) return rc;
+        work[k] = f;
+        out = f;
+        return PPS_OK;
+    }
+    void zero(Field f)
+    {
+        if (f.prec == 32) launch_fill<float>(gbox(32), stream, f.f(), 0.0f);
+        else launch_fill<double>(gbox(64), stream, f.d(), 0.0);
+    }
+    void ghost(Field f, bool with_flux, const Scal* S, const int* gate)
+    {
+        if (faces.n == 0) return;
+        if (f.prec == 32) launch_ghost<float>(stream, L32, faces, f.f(), 0, nullptr, gate);
+        else launch_ghost<double>(stream, L64, faces, f.d(), with_flux ? 1 : 0, S, gate);
+    }
+    Scal* scal(int i) const { return scal_d + i; }
+};
+
+// tiny scalar-update kernel: keeps scalar bookkeeping on the stream, no
+// host/device round trip and no memset nodes
+enum { SC_SET = 0, SC_RESET_ITER = 1, SC_OUTER_INC = 2, SC_SET_DONE = 3 };
+__global__ void k_scal_op(Scal* S, int op, int idx, double v)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (op == SC_SET) S->v[idx] = v;
+    else if (op == SC_RESET_ITER) {
+        S->iter = 0;
+        S->done = 0;
+        S->outer = 0;
+    } else if (op == SC_OUTER_INC) S->outer += 1;
+    else if (op == SC_SET_DONE) S->done = (int)v;
+}
+
+static void scal_op(Ctx& C, Scal* S, int op, int idx = 0, double v = 0.0)
+{
+    hipLaunchKernelGGL(k_scal_op, dim3(1), dim3(64), 0, C.stream, S, op, idx, v);
+}
+
+// initialise a scalar block from the host (stream ordered through pinned staging)
+static int scal_init(Ctx& C, Scal* S, double tol, int maxit, double jac, double* hist, int hist_cap)
+{
+    int rc = C.hip(hipStreamSynchronize(C.stream), "sync");
+    if (rc) return rc;
+    Scal h{};
+    h.v[S_TOL] = tol;
+    h.v[S_JAC] = jac;
+    h.v[S_NORM] = 1.0;
+    h.v[S_BETA] = 0.0;
+    h.v[S_RHO0] = 1.0;
+    h.v[S_PAP_OLD] = 1.0;
+    h.maxit = maxit;
+    h.hist = hist;
+    h.hist_cap = hist_cap;
+    *C.scal_h = h;
+    rc = C.hip(hipMemcpyAsync(S, C.scal_h, sizeof(Scal), hipMemcpyHostToDevice, C.stream), "scal init");
+    if (rc) return rc;
+    return C.hip(hipStreamSynchronize(C.stream), "sync");
+}
+
+static int scal_read(Ctx& C, const Scal* S, Scal& out)
+{
+    int rc = C.hip(hipMemcpyAsync(C.scal_h, S, sizeof(Scal), hipMemcpyDeviceToHost, C.stream), "scal read");
+    if (rc) return rc;
+    rc = C.hip(hipStreamSynchronize(C.stream), "sync");
+    if (rc) return rc;
+    out = *C.scal_h;
+    return PPS_OK;
+}
+
+// Runs `body` (one iteration's launches) until S->done, two batches in flight.
+template <class Body>
+static int run_loop(Ctx& C, Scal* S, int maxit, bool host_each, Body body)
+{
+    if (maxit <= 0) return PPS_OK;  // while (iter < maxIteration) never runs
+    int batch = host_each ? 1 : 4;
+    int queued = 0;
+    int slot = 0;
+    bool have_prev = false;
+    for (;;) {
+        const int nb = batch;
+        for (int b = 0; b < nb; ++b) {
+            int rc = body();
+            if (rc) return rc;
+        }
+        queued += nb;
+        int rc = C.hip(hipMemcpyAsync(C.flag_h + 4 * slot, &S->iter, 2 * sizeof(int), hipMemcpyDeviceToHost, C.stream),
+                       "flag copy");
+        if (rc) return rc;
+        rc = C.hip(hipEventRecord(C.ev_batch[slot], C.stream), "event record");
+        if (rc) return rc;
+        rc = C.hip(hipGetLastError(), "kernel launch");
+        if (rc) return rc;
+        const int waitslot = host_each ? slot : slot ^ 1;
+        if (host_each || have_prev) {
+            rc = C.hip(hipEventSynchronize(C.ev_batch[waitslot]), "event sync");
+            if (rc) return rc;
+            if (C.flag_h[4 * waitslot + 1] != 0) break;
+        }
+        have_prev = true;
+        slot ^= 1;
+        if (!host_each) batch = std::min(batch * 2, 64);
+        if (queued > maxit + 2 * 64) break;  // every gate is closed by now
+    }
+    return C.hip(hipStreamSynchronize(C.stream), "sync");
+}
+
+// normalizeProblemToFieldBNorm (iterativeSolverBase.hpp:171-234) on device
+static void normalize(Ctx& C, Scal* S, bool main, Field x, Field b)
+{
+    if (main) {
+        launch_bnorm(C.gsolver(64), C.stream, C.rb, x.d(), b.d(), C.grid.bcs_type, C.lim_data, C.lim_solver,
+                     C.flux_d, C.fng, C.grid.ds, S);
+    } else {
+        // plain ||b|| of the preconditioner input; FP64 accumulation
+        if (b.prec == 32) launch_dot<float>(C.gsolver(32), C.stream, C.rb, b.f(), b.f(), &S->v[S_AUX0], nullptr);
+        else launch_dot<double>(C.gsolver(64), C.stream, C.rb, b.d(), b.d(), &S->v[S_AUX0], nullptr);
+    }
+}
+
+__global__ void k_sqrt_norm(Scal* S)
+{
+    if (threadIdx.x == 0) S->v[S_NORM] = sqrt(S->v[S_AUX0]);
+}
+
+static void scale_box(Ctx& C, Field f, const Scal* S, int mode)
+{
+    if (f.prec == 32) launch_scale_box<float>(C.gbox(32), C.stream, f.f(), S, mode);
+    else launch_scale_box<double>(C.gbox(64), C.stream, f.d(), S, mode);
+}
+
+// ------------------------------------------------------------------ None / Jacobi
+
+struct NoneStage : Stage {
+    using Stage::Stage;
+    bool pointwise(double* c) const override
+    {
+        *c = 1.0;
+        return true;
+    }
+    int apply(Field out, Field in, const int* gate) override
+    {
+        // NoneSolver copies the whole guarded array (noneSolver.hpp:24-27)
+        if (in.prec == 32) launch_pointwise<float>(C.gbox(32), C.stream, in.f(), out.f(), 1.0, gate);
+        else launch_pointwise<double>(C.gbox(64), C.stream, in.d(), out.d(), 1.0, gate);
+        return PPS_OK;
+    }
+};
+
+struct JacobiStage : Stage {
+    using Stage::Stage;
+    double scale() const
+    {
+        const double* ds = C.grid.ds;
+        return 1.0 / (-2.0 * (1.0 / (ds[0] * ds[0]) + 1.0 / (ds[1] * ds[1]) + 1.0 / (ds[2] * ds[2])));
+    }
+    bool pointwise(double* c) const override
+    {
+        *c = scale();
+        return true;
+    }
+    int apply(Field out, Field in, const int* gate) override
+    {
+        if (in.prec == 32) launch_pointwise<float>(C.gbox(32), C.stream, in.f(), out.f(), scale(), gate);
+        else launch_pointwise<double>(C.gbox(64), C.stream, in.d(), out.d(), scale(), gate);
+        return PPS_OK;
+    }
+};
+
+// ---------------------------------------------------------------- Chebyshev
+
+struct ChebStage : Stage {
+    double theta = 0, delta = 0, sigma = 0;
+    Field Y, Z, W, Bc;  // Bc: FP32 copy of the input when the sweeps run in FP32
+    ChebStage(Ctx& c, const pps_stage& s, int l) : Stage(c, s, l)
+    {
+        // chebyshevIteration.hpp:22-26 (global, rescaled); alpaka local form
+        if (cfg.cheb_local) {
+            theta = (C.eig_local[0] + C.eig_local[1]) * 0.5;
+            delta = (C.eig_local[0] - C.eig_local[1]) * 0.5;
+        } else {
+            theta = (C.eig_global[0] * cfg.cheb_rescale_min + C.eig_global[1] * cfg.cheb_rescale_max) * 0.5 *
+                    (1.0 + cfg.cheb_eps);
+            delta = (C.eig_global[0] * cfg.cheb_rescale_min - C.eig_global[1] * cfg.cheb_rescale_max) * 0.5;
+        }
+        sigma = theta / delta;
+    }
+    int buffers()
+    {
+        const int P = prec();
+        const std::string k = "cheb" + std::to_string(level);
+        int rc = C.get_work(k + ".Y", P, Y);
+        if (!rc) rc = C.get_work(k + ".Z", P, Z);
+        if (!rc) rc = C.get_work(k + ".W", P, W);
+        if (!rc && P == 32) rc = C.get_work(k + ".B", 32, Bc);
+        return rc;
+    }
+    template <typename T, typename TO>
+    void sweeps(const T* B, TO* X, const int* gate, Field Yf, Field Zf, Field Wf)
+    {
+        // y1 = B/theta, y2 = 2 rho1/delta (2B + A B/theta), y_{c+1} from the
+        // three-term recurrence; the reference returns -W after its final
+        // rotation, which is -y_{m-1} (chebyshevIteration.hpp:93-128): the two
+        // sweeps computing y_m and y_{m+1} do not reach the output and are skipped.
+        const int m = cfg.max_iter;
+        const Geo g = C.gsolver(sizeof(T) == 4 ? 32 : 64);
+        double rhoOld = 1 / sigma;
+        double rhoCurr = 1 / (2 * sigma - rhoOld);
+        const double c0 = 1 / theta, c1 = 2 * rhoCurr / delta;
+        if (m <= 1) {
+            launch_fill<TO>(C.gsolver(sizeof(TO) == 4 ? 32 : 64), C.stream, X, TO(0));
+            return;
+        }
+        if (m == 2 || m == 3) {
+            launch_cheb_first_out<T, TO>(g, C.stream, B, X, c0, c1, m == 2 ? 0 : 1, -1.0, gate);
+            return;
+        }
+        T* Yp = static_cast<T*>(Yf.p);
+        T* Zp = static_cast<T*>(Zf.p);
+        T* Wp = static_cast<T*>(Wf.p);
+        launch_cheb_first<T>(g, C.stream, B, Zp, Yp, c0, c1, gate);
+        for (int c = 2; c <= m - 2; ++c) {
+            rhoOld = rhoCurr;
+            rhoCurr = 1 / (2 * sigma - rhoOld);
+            Field yf;
+            yf.p = Yp;
+            yf.prec = sizeof(T) == 4 ? 32 : 64;
+            C.ghost(yf, false, nullptr, gate);
+            const double a = rhoCurr * 2 * sigma, b = rhoCurr * 2 / delta, cz = rhoCurr * rhoOld;
+            if (c < m - 2) {
+                launch_cheb_sweep<T, T>(g, C.stream, Yp, B, Zp, Wp, a, b, cz, 1.0, gate);
+                T* t = Zp;  // swap(Z,Y); swap(W,Y)
+                Zp = Yp;
+                Yp = Wp;
+                Wp = t;
+            } else {
+                launch_cheb_sweep<T, TO>(g, C.stream, Yp, B, Zp, X, a, b, cz, -1.0, gate);
+            }
+        }
+    }
+    int apply(Field out, Field in, const int* gate) override
+    {
+        int rc = buffers();
+        if (rc) return rc;
+        calls++;
+        prec_iters += cfg.max_iter;
+        // resetNeumanBCs on the input (chebyshevIteration.hpp:74), in place
+        if (prec() == 64 && in.prec == 64) {
+            C.ghost(in, false, nullptr, gate);
+            if (out.prec == 64) sweeps<double, double>(in.d(), out.d(), gate, Y, Z, W);
+            else return C.fail(PPS_ERR_UNSUPPORTED, "FP64 Chebyshev into an FP32 field");
+        } else {
+            // CastPrecisionFieldKernel (kernelsAlpakaChebyshev.hpp:7-18) then FP32 sweeps
+            const float* B = nullptr;
+            if (in.prec == 64) {
+                C.ghost(in, false, nullptr, gate);
+                launch_cast<double, float>(C.gbox(32), C.stream, in.d(), Bc.f(), gate);
+                B = Bc.f();
+            } else {
+                C.ghost(in, false, nullptr, gate);
+                B = in.f();
+            }
+            if (out.prec == 64) sweeps<float, double>(B, out.d(), gate, Y, Z, W);
+            else sweeps<float, float>(B, out.f(), gate, Y, Z, W);
+        }
+        return PPS_OK;
+    }
+    int solve_main() override;
+};
+
+// ------------------------------------------------------------------ Krylov base
+
+struct KrylovStage : Stage {
+    using Stage::Stage;
+    bool host_driven() const override { return true; }
+    bool pre_pointwise(double* c) const
+    {
+        if (!pre) {
+            *c = 1.0;
+            return true;
+        }
+        return pre->pointwise(c);
+    }
+    bool pre_host() const { return pre && pre->host_driven(); }
+    int pre_apply(Field out, Field in, const int* gate)
+    {
+        if (!pre) {
+            NoneStage none(C, pps_stage{}, level + 1);
+            return none.apply(out, in, gate);
+        }
+        return pre->apply(out, in, gate);
+    }
+    // common prologue of the main/inner loop: returns 1 if the loop must not run
+    int prologue(bool main, Field x, Field b, Field r, int* early)
+    {
+        if (main) C.ghost(x, true, S, nullptr);
+        else C.zero(x);
+        normalize(C, S, main, x, b);
+        if (!main) hipLaunchKernelGGL(k_sqrt_norm, dim3(1), dim3(64), 0, C.stream, S);
+        scale_box(C, x, S, 0);
+        scale_box(C, b, S, 0);
+        C.ghost(x, main, S, nullptr);
+        launch_residual(C.gsolver(64), C.stream, C.rb, x.d(), b.d(), r.d(), S, 0);
+        Scal h;
+        int rc = scal_read(C, S, h);
+        if (rc) return rc;
+        *early = (h.done == 2);
+        return PPS_OK;
+    }
+    void epilogue(bool main, Field x, Field b, Field r)
+    {
+        if (main) {
+            C.ghost(x, true, S, nullptr);
+            launch_residual(C.gsolver(64), C.stream, C.rb, x.d(), b.d(), r.d(), S, 1);
+        }
+        scale_box(C, x, S, 1);
+        scale_box(C, b, S, 1);
+    }
+};
+
+// ------------------------------------------------------------------------ CG
+
+struct CGStage : KrylovStage {
+    using KrylovStage::KrylovStage;
+    // BaseCG::operator() (baseCG.hpp:44-260), FP64 vectors
+    int run(bool main, Field x, Field b)
+    {
+        if (prec() != 64 || x.prec != 64) return C.fail(PPS_ERR_UNSUPPORTED, "CG stage %d: FP32 vectors only inside PPS_IR", level);
+        const std::string k = "cg" + std::to_string(level);
+        Field r, p0, p1, Ap, z;
+        int rc = C.get_work(k + ".r", 64, r);
+        if (!rc) rc = C.get_work(k + ".p0", 64, p0);
+        if (!rc) rc = C.get_work(k + ".p1", 64, p1);
+        if (!rc) rc = C.get_work(k + ".Ap", 64, Ap);
+        double c = 1.0;
+        const bool pw = pre_pointwise(&c);
+        if (!rc && !pw) rc = C.get_work(k + ".z", 64, z);
+        if (rc) return rc;
+        C.zero(r);
+        C.zero(p0);
+        C.zero(p1);
+        C.zero(Ap);
+        if (!pw) C.zero(z);
+        rc = scal_init(C, S, cfg.tol, cfg.max_iter, pw ? c : 1.0, main ? C.hist_d : nullptr, main ? C.hist_cap : 0);
+        if (rc) return rc;
+        int early = 0;
+        rc = prologue(main, x, b, r, &early);
+        if (rc) return rc;
+        if (early) return PPS_OK;  // baseCG.hpp:103-107: x, b stay normalised
+        const Geo g = C.gsolver(64);
+        if (!pw) {
+            rc = pre_apply(z, r, &S->done);
+            if (rc) return rc;
+            launch_cg_rz<double>(g, C.stream, C.rb, r.d(), z.d(), S);
+        }
+        Field p[2] = {p0, p1};
+        int cur = 0;
+        rc = run_loop(C, S, cfg.max_iter, pre_host(), [&]() -> int {
+            launch_cg_A2<double>(g, C.stream, C.rb, pw ? r.d() : z.d(), p[cur].d(), p[cur ^ 1].d(), Ap.d(), S, pw);
+            cur ^= 1;
+            launch_cg_B<double>(g, C.stream, C.rb, x.d(), p[cur].d(), r.d(), Ap.d(), S, pw, main);
+            if (!pw) {
+                int e = pre_apply(z, r, &S->done);
+                if (e) return e;
+                launch_cg_rz<double>(g, C.stream, C.rb, r.d(), z.d(), S);
+            }
+            return PPS_OK;
+        });
+        if (rc) return rc;
+        epilogue(main, x, b, r);
+        return PPS_OK;
+    }
+    int solve_main() override { return run(true, C.fields[PPS_FIELD_X], C.fields[PPS_FIELD_B]); }
+    int apply(Field out, Field in, const int*) override
+    {
+        calls++;
+        int rc = run(false, out, in);
+        Scal h;
+        if (!rc) rc = scal_read(C, S, h);
+        if (!rc) prec_iters += h.iter;
+        return rc;
+    }
+};
+
+// ------------------------------------------------------------------ BiCGSTAB
+
+struct BiCGStage : KrylovStage {
+    using KrylovStage::KrylovStage;
+    // BiCGSTAB::operator() (BiCGSTAB.hpp:55-322) as the K1..K6 kernels of
+    // BiCGstabAlpaka.hpp:210-450 with K4/K5 fused
+    int run(bool main, Field x, Field b)
+    {
+        if (prec() != 64 || x.prec != 64) return C.fail(PPS_ERR_UNSUPPORTED, "BiCGSTAB stage %d: FP64 only", level);
+        const std::string k = "bicg" + std::to_string(level);
+        Field r, r0, p, AMp, Az, Mp, z;
+        int rc = C.get_work(k + ".r", 64, r);
+        if (!rc) rc = C.get_work(k + ".r0", 64, r0);
+        if (!rc) rc = C.get_work(k + ".p", 64, p);
+        if (!rc) rc = C.get_work(k + ".AMp", 64, AMp);
+        if (!rc) rc = C.get_work(k + ".Az", 64, Az);
+        double c = 1.0;
+        const bool pw = pre_pointwise(&c);
+        if (!rc && !pw) rc = C.get_work(k + ".Mp", 64, Mp);
+        if (!rc && !pw) rc = C.get_work(k + ".z", 64, z);
+        if (rc) return rc;
+        for (Field f : {r, r0, p, AMp, Az}) C.zero(f);
+        if (!pw) {
+            C.zero(Mp);
+            C.zero(z);
+        }
+        rc = scal_init(C, S, cfg.tol, cfg.max_iter, pw ? c : 1.0, main ? C.hist_d : nullptr, main ? C.hist_cap : 0);
+        if (rc) return rc;
+        int early = 0;
+        rc = prologue(main, x, b, r, &early);
+        if (rc) return rc;
+        if (early) return PPS_OK;
+        const Geo g = C.gsolver(64);
+        launch_copy<double>(g, C.stream, r.d(), p.d());
+        launch_copy<double>(g, C.stream, r.d(), r0.d());
+        const int* gate = &S->done;
+        rc = run_loop(C, S, cfg.max_iter, pre_host(), [&]() -> int {
+            if (pw) {
+                C.ghost(p, false, nullptr, gate);
+                launch_bicg_k1<double>(g, C.stream, C.rb, p.d(), c, AMp.d(), r0.d(), S);
+            } else {
+                int e = pre_apply(Mp, p, gate);
+                if (e) return e;
+                C.ghost(Mp, false, nullptr, gate);
+                launch_bicg_k1<double>(g, C.stream, C.rb, Mp.d(), 1.0, AMp.d(), r0.d(), S);
+            }
+            launch_bicg_k2<double>(g, C.stream, r.d(), AMp.d(), S);
+            if (pw) {
+                C.ghost(r, false, nullptr, gate);
+                launch_bicg_k3<double>(g, C.stream, C.rb, r.d(), c, Az.d(), r.d(), S);
+                launch_bicg_k45<double>(g, C.stream, C.rb, x.d(), p.d(), c, r.d(), c, r.d(), Az.d(), r0.d(), S, main);
+            } else {
+                int e = pre_apply(z, r, gate);
+                if (e) return e;
+                C.ghost(z, false, nullptr, gate);
+                launch_bicg_k3<double>(g, C.stream, C.rb, z.d(), 1.0, Az.d(), r.d(), S);
+                launch_bicg_k45<double>(g, C.stream, C.rb, x.d(), Mp.d(), 1.0, z.d(), 1.0, r.d(), Az.d(), r0.d(), S, main);
+            }
+            launch_bicg_k6<double>(g, C.stream, p.d(), r.d(), AMp.d(), S);
+            return PPS_OK;
+        });
+        if (rc) return rc;
+        // resetNeumanBCs<isMainLoop,true>(fieldX) (BiCGSTAB.hpp:301) before the true residual
+        C.ghost(x, main, S, nullptr);
+        epilogue(main, x, b, r);
+        return PPS_OK;
+    }
+    int solve_main() override { return run(true, C.fields[PPS_FIELD_X], C.fields[PPS_FIELD_B]); }
+    int apply(Field out, Field in, const int*) override
+    {
+        calls++;
+        int rc = run(false, out, in);
+        Scal h;
+        if (!rc) rc = scal_read(C, S, h);
+        if (!rc) prec_iters += h.iter;
+        return rc;
+    }
+};
+
+// ------------------------------------------------------------------------ FCG
+
+struct FCGStage : KrylovStage {
+    using KrylovStage::KrylovStage;
+    // Flexible CG, truncation 1 (Notay 2000); prologue/epilogue as BaseCG.
+    // T = double (main) or float (inner solve of PPS_IR, see IRStage).
+    int run(bool main, Field x, Field b)
+    {
+        if (prec() != 64 || x.prec != 64) return C.fail(PPS_ERR_UNSUPPORTED, "FCG stage %d: FP32 vectors only inside PPS_IR", level);
+        const std::string k = "fcg" + std::to_string(level);
+        Field r, z, p0, p1, A0, A1;
+        int rc = C.get_work(k + ".r", 64, r);
+        if (!rc) rc = C.get_work(k + ".z", 64, z);
+        if (!rc) rc = C.get_work(k + ".p0", 64, p0);
+        if (!rc) rc = C.get_work(k + ".p1", 64, p1);
+        if (!rc) rc = C.get_work(k + ".A0", 64, A0);
+        if (!rc) rc = C.get_work(k + ".A1", 64, A1);
+        if (rc) return rc;
+        for (Field f : {r, z, p0, p1, A0, A1}) C.zero(f);
+        rc = scal_init(C, S, cfg.tol, cfg.max_iter, 1.0, main ? C.hist_d : nullptr, main ? C.hist_cap : 0);
+        if (rc) return rc;
+        int early = 0;
+        rc = prologue(main, x, b, r, &early);
+        if (rc) return rc;
+        if (early) return PPS_OK;
+        const Geo g = C.gsolver(64);
+        Field p[2] = {p0, p1}, Ap[2] = {A0, A1};
+        int cur = 0;  // p[cur] = p_new, p[cur^1] = p_old
+        rc = run_loop(C, S, cfg.max_iter, pre_host(), [&]() -> int {
+            int e = pre_apply(z, r, &S->done);
+            if (e) return e;
+            launch_fcg_gamma<double>(g, C.stream, C.rb, z.d(), Ap[cur ^ 1].d(), S);
+            launch_fcg_p<double>(g, C.stream, C.rb, z.d(), p[cur ^ 1].d(), p[cur].d(), Ap[cur].d(), r.d(), S);
+            launch_fcg_update<double>(g, C.stream, C.rb, x.d(), p[cur].d(), r.d(), Ap[cur].d(), S, main);
+            cur ^= 1;
+            return PPS_OK;
+        });
+        if (rc) return rc;
+        epilogue(main, x, b, r);
+        return PPS_OK;
+    }
+    int solve_main() override { return run(true, C.fields[PPS_FIELD_X], C.fields[PPS_FIELD_B]); }
+    int apply(Field out, Field in, const int*) override
+    {
+        calls++;
+        int rc = run(false, out, in);
+        Scal h;
+        if (!rc) rc = scal_read(C, S, h);
+        if (!rc) prec_iters += h.iter;
+        return rc;
+    }
+};
+
+// --------------------------------------------------------- learned preconditioner
+
+struct NNStage : Stage {
+    using Stage::Stage;
+    int apply(Field out, Field in, const int* gate) override
+    {
+        if (!C.nn) return C.fail(PPS_ERR_STATE, "PPS_NN stage without a trained network");
+        calls++;
+        prec_iters++;
+        return C.nn->apply(C, out.p, out.prec, in.p, in.prec, gate);
+    }
+};
+
+// ------------------------------------------------- mixed-precision refinement
+
+struct IRStage : KrylovStage {
+    using KrylovStage::KrylovStage;
+    long long inner_total = 0;
+    int outer_steps = 0;
+    // FP32 inner solve of A d = r32 (CG or FCG, non-main semantics of BaseCG:
+    // x = 0, b normalised by its norm, relative tolerance), gated on SI->done
+    int inner(Stage* in, Field d, Field r32)
+    {
+        Scal* SI = in->S;
+        const std::string k = "ir" + std::to_string(level);
+        Field p0, p1, A0, A1, z;
+        int rc = C.get_work(k + ".p0", 32, p0);
+        if (!rc) rc = C.get_work(k + ".p1", 32, p1);
+        if (!rc) rc = C.get_work(k + ".A0", 32, A0);
+        if (!rc) rc = C.get_work(k + ".A1", 32, A1);
+        if (rc) return rc;
+        Stage* M = in->pre;
+        double c = 1.0;
+        const bool pw = (M == nullptr) || M->pointwise(&c);
+        const bool flexible = in->cfg.kind == PPS_FCG;
+        if ((!pw || flexible) && (rc = C.get_work(k + ".z", 32, z))) return rc;
+        rc = scal_init(C, SI, in->cfg.tol, in->cfg.max_iter, pw ? c : 1.0, nullptr, 0);
+        if (rc) return rc;
+        const Geo g = C.gsolver(32);
+        Field p[2] = {p0, p1}, Ap[2] = {A0, A1};
+        launch_ir_inner_init(g, C.stream, C.rb, r32.f(), d.f(), p0.f(), SI, pw);
+        // pin p1 and the old Ap to zero as well: iteration 0 multiplies them by beta = 0
+        launch_fill<float>(g, C.stream, p1.f(), 0.0f);
+        launch_fill<float>(g, C.stream, A0.f(), 0.0f);
+        launch_fill<float>(g, C.stream, A1.f(), 0.0f);
+        int cur = 0;
+        const bool host_each = M && M->host_driven();
+        auto applyM = [&](const int* gate) -> int {
+            if (M) return M->apply(z, r32, gate);
+            launch_pointwise<float>(C.gbox(32), C.stream, r32.f(), z.f(), 1.0, gate);
+            return PPS_OK;
+        };
+        if (!flexible && !pw) {
+            rc = applyM(&SI->done);
+            if (rc) return rc;
+            launch_cg_rz<float>(g, C.stream, C.rb, r32.f(), z.f(), SI);
+        }
+        rc = run_loop(C, SI, in->cfg.max_iter, host_each, [&]() -> int {
+            if (flexible) {
+                int e = applyM(&SI->done);
+                if (e) return e;
+                launch_fcg_gamma<float>(g, C.stream, C.rb, z.f(), Ap[cur ^ 1].f(), SI);
+                launch_fcg_p<float>(g, C.stream, C.rb, z.f(), p[cur ^ 1].f(), p[cur].f(), Ap[cur].f(), r32.f(), SI);
+                launch_fcg_update<float>(g, C.stream, C.rb, d.f(), p[cur].f(), r32.f(), Ap[cur].f(), SI, false);
+                cur ^= 1;
+            } else {
+                launch_cg_A2<float>(g, C.stream, C.rb, pw ? r32.f() : z.f(), p[cur].f(), p[cur ^ 1].f(), Ap[0].f(), SI, pw);
+                cur ^= 1;
+                launch_cg_B<float>(g, C.stream, C.rb, d.f(), p[cur].f(), r32.f(), Ap[0].f(), SI, pw, false);
+                if (!pw) {
+                    int e = applyM(&SI->done);
+                    if (e) return e;
+                    launch_cg_rz<float>(g, C.stream, C.rb, r32.f(), z.f(), SI);
+                }
+            }
+            return PPS_OK;
+        });
+        if (rc) return rc;
+        Scal h;
+        rc = scal_read(C, SI, h);
+        if (rc) return rc;
+        inner_total += h.iter;
+        in->prec_iters += h.iter;
+        in->calls++;
+        return PPS_OK;
+    }
+    int solve_main() override
+    {
+        if (!pre || (pre->cfg.kind != PPS_CG && pre->cfg.kind != PPS_FCG) || pre->prec() != 32)
+            return C.fail(PPS_ERR_UNSUPPORTED, "PPS_IR needs an FP32 CG or FCG inner stage");
+        Field x = C.fields[PPS_FIELD_X], b = C.fields[PPS_FIELD_B];
+        Field r32, d;
+        int rc = C.get_work("ir.r32", 32, r32);
+        if (!rc) rc = C.get_work("ir.d", 32, d);
+        if (rc) return rc;
+        C.zero(r32);
+        C.zero(d);
+        inner_total = 0;
+        outer_steps = 0;
+        rc = scal_init(C, S, cfg.tol, cfg.max_iter, 1.0, C.hist_d, C.hist_cap);
+        if (rc) return rc;
+        Scal* SI = pre->S;
+        C.ghost(x, true, S, nullptr);
+        normalize(C, S, true, x, b);
+        scale_box(C, x, S, 0);
+        scale_box(C, b, S, 0);
+        C.ghost(x, true, S, nullptr);
+        const Geo g64 = C.gsolver(64);
+        launch_ir_residual(g64, C.stream, C.rb, x.d(), b.d(), r32.f(), S, SI);
+        Scal h;
+        rc = scal_read(C, S, h);
+        if (rc) return rc;
+        if (h.done) return PPS_OK;  // early exit as in BaseCG
+        while (outer_steps < cfg.max_iter) {
+            rc = inner(pre, d, r32);
+            if (rc) return rc;
+            launch_ir_update(g64, C.stream, x.d(), d.f(), SI);
+            C.ghost(x, true, S, nullptr);
+            scal_op(C, S, SC_OUTER_INC);
+            launch_ir_residual(g64, C.stream, C.rb, x.d(), b.d(), r32.f(), S, SI);
+            outer_steps++;
+            rc = scal_read(C, S, h);
+            if (rc) return rc;
+            if (h.done) break;
+        }
+        scale_box(C, x, S, 1);
+        scale_box(C, b, S, 1);
+        return PPS_OK;
+    }
+    int apply(Field, Field, const int*) override { return C.fail(PPS_ERR_UNSUPPORTED, "PPS_IR is a main solver only"); }
+};
+
+// Chebyshev as the main solver (isMainLoop = true, chebyshevIteration.hpp:61-67,132-139)
+__global__ void k_adjust_copy(double* dst, const double* src, long long n)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t];
+}
+
+int ChebStage::solve_main()
+{
+    int rc = buffers();
+    if (rc) return rc;
+    if (prec() != 64) return C.fail(PPS_ERR_UNSUPPORTED, "main Chebyshev runs in FP64");
+    Field x = C.fields[PPS_FIELD_X], b = C.fields[PPS_FIELD_B];
+    Field Bt, r;
+    rc = C.get_work("chebmain.B", 64, Bt);
+    if (!rc) rc = C.get_work("chebmain.r", 64, r);
+    if (rc) return rc;
+    rc = scal_init(C, S, cfg.tol, cfg.max_iter, 1.0, nullptr, 0);
+    if (rc) return rc;
+    const long long n = C.L64.total;
+    hipLaunchKernelGGL(k_adjust_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, C.stream, Bt.d(), b.d(), n);
+    adjust_b_field(C.gdata(64), C.stream, x.d(), Bt.d(), C.grid.bcs_type, C.lim_data, C.flux_d, C.fng, C.grid.ds);
+    C.ghost(Bt, false, nullptr, nullptr);
+    sweeps<double, double>(Bt.d(), x.d(), nullptr, Y, Z, W);
+    C.ghost(x, true, S, nullptr);
+    launch_residual(C.gsolver(64), C.stream, C.rb, x.d(), b.d(), r.d(), S, 1);
+    scal_op(C, S, SC_SET, S_ERR, 0.0);
+    return PPS_OK;
+}
+
+}  // namespace pps
+
+// ====================================================================== C ABI
+
+using namespace pps;
+
+static int build_chain(Ctx& C, const pps_stage* chain, int n, std::vector<std::unique_ptr<Stage>>& st)
+{
+    if (!chain || n < 1 || n > 8) return C.fail(PPS_ERR_ARG, "solver chain needs 1..8 stages");
+    for (int i = 0; i < n; ++i) {
+        pps_stage s = chain[i];
+        if (s.precision != 32) s.precision = 64;
+        std::unique_ptr<Stage> p;
+        switch (s.kind) {
+        case PPS_NONE: p.reset(new NoneStage(C, s, i)); break;
+        case PPS_JACOBI: p.reset(new JacobiStage(C, s, i)); break;
+        case PPS_CG: p.reset(new CGStage(C, s, i)); break;
+        case PPS_BICGSTAB: p.reset(new BiCGStage(C, s, i)); break;
+        case PPS_CHEBYSHEV: p.reset(new ChebStage(C, s, i)); break;
+        case PPS_FCG: p.reset(new FCGStage(C, s, i)); break;
+        case PPS_IR: p.reset(new IRStage(C, s, i)); break;
+        case PPS_NN: p.reset(new NNStage(C, s, i)); break;
+        default: return C.fail(PPS_ERR_ARG, "unknown stage kind %d", s.kind);
+        }
+        if (i >= C.nscal) return C.fail(PPS_ERR_ARG, "too many stages");
+        p->S = C.scal(i);
+        if (s.max_iter < 0) return C.fail(PPS_ERR_ARG, "stage %d: max_iter < 0", i);
+        st.push_back(std::move(p));
+    }
+    for (int i = 0; i + 1 < n; ++i) st[i]->pre = st[i + 1].get();
+    const int k0 = chain[0].kind;
+    if (k0 == PPS_NONE || k0 == PPS_JACOBI || k0 == PPS_NN)
+        return C.fail(PPS_ERR_ARG, "stage 0 must be a solver (CG, BiCGSTAB, Chebyshev, FCG, IR)");
+    return PPS_OK;
+}
+
+static Ctx* as_ctx(void* p) { return static_cast<Ctx*>(p); }
+
+extern "C" {
+
+int pps_abi_version(void) { return PPS_ABI_VERSION; }
+
+const char* pps_last_error(void* ctx)
+{
+    if (!ctx) return g_create_error.c_str();
+    return as_ctx(ctx)->err.c_str();
+}
+
+int pps_create(const pps_grid* grid, void** out)
+{
+    if (!out || !grid) {
+        g_create_error = "null argument";
+        return PPS_ERR_ARG;
+    }
+    *out = nullptr;
+    if (grid->dim != 3) {
+        g_create_error = "DIM must be 3 in this build";
+        return PPS_ERR_UNSUPPORTED;
+    }
+    if (grid->nranks[0] * grid->nranks[1] * grid->nranks[2] != 1 || grid->my_rank != 0) {
+        g_create_error = "one GPU: the rank grid must be 1x1x1";
+        return PPS_ERR_UNSUPPORTED;
+    }
+    for (int d = 0; d < 3; ++d) {
+        if (grid->guards[d] != 1) {
+            g_create_error = "guards must be {1,1,1} for the 7-point stencil";
+            return PPS_ERR_ARG;
+        }
+        if (grid->npglobal[d] < 3 || !(grid->ds[d] > 0)) {
+            g_create_error = "npglobal must be >= 3 and ds > 0";
+            return PPS_ERR_ARG;
+        }
+    }
+    for (int f = 0; f < 6; ++f)
+        if (grid->bcs_type[f] != 0 && grid->bcs_type[f] != 1) {
+            g_create_error = "bcsType must be 0 (Dirichlet) or 1 (Neumann)";
+            return PPS_ERR_ARG;
+        }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_create_error = "no HIP device";
+        return PPS_ERR_NOGPU;
+    }
+    std::unique_ptr<Ctx> C(new Ctx());
+    C->grid = *grid;
+    for (int d = 0; d < 3; ++d) {
+        C->np[d] = grid->npglobal[d];
+        C->ng[d] = grid->npglobal[d] + 2;
+        C->coef[d] = 1.0 / (grid->ds[d] * grid->ds[d]);
+        C->lim_data[2 * d] = 1;
+        C->lim_data[2 * d + 1] = grid->npglobal[d] + 1;
+    }
+    // blockGrid.hpp:208-222 and :301-340 for one rank
+    std::memcpy(C->lim_solver, C->lim_data, sizeof C->lim_solver);
+    for (int d = 0; d < 3; ++d) {
+        if (grid->bcs_type[2 * d] == 0) C->lim_solver[2 * d] += 1;
+        if (grid->bcs_type[2 * d + 1] == 0) C->lim_solver[2 * d + 1] -= 1;
+    }
+    const double PI = 3.141592653589793;
+    double emin = 0, emax = 0, gmin = 0, gmax = 0;
+    for (int d = 0; d < 3; ++d) {
+        const int cnt = C->lim_solver[2 * d + 1] - C->lim_solver[2 * d];
+        const double s1 = std::sin(1 * PI / 2 / (cnt + 1)), s2 = std::sin(cnt * PI / 2 / (cnt + 1));
+        const double dd = grid->ds[d] * grid->ds[d];
+        emin += 4 * s1 * s1 / dd;
+        emax += 4 * s2 * s2 / dd;
+        int ni = grid->npglobal[d];
+        if (grid->bcs_type[2 * d] == 0) ni -= 1;
+        if (grid->bcs_type[2 * d + 1] == 0) ni -= 1;
+        const double g1 = std::sin(1 * PI / 2 / (ni + 1)), g2 = std::sin(ni * PI / 2 / (ni + 1));
+        gmin += 4 * g1 * g1 / dd;
+        gmax += 4 * g2 * g2 / dd;
+    }
+    C->eig_local[0] = emin;
+    C->eig_local[1] = emax;
+    C->eig_global[0] = gmin;
+    C->eig_global[1] = gmax;
+    for (int d = 0; d < 3; ++d) {
+        C->solver.lo[d] = C->lim_solver[2 * d];
+        C->solver.hi[d] = C->lim_solver[2 * d + 1];
+        C->data.lo[d] = C->lim_data[2 * d];
+        C->data.hi[d] = C->lim_data[2 * d + 1];
+        C->box.lo[d] = 0;
+        C->box.hi[d] = C->ng[d];
+    }
+    for (int prec : {64, 32}) {
+        Layout& L = prec == 32 ? C->L32 : C->L64;
+        // One element layout for every precision: FP32 and FP64 fields share
+        // offsets (mixed-precision kernels index both with one offset); rows are
+        // 32 elements aligned = 128 B (FP32) / 256 B (FP64).
+        const int A = 32;
+        for (int d = 0; d < 3; ++d) L.ng[d] = C->ng[d];
+        L.ox = A - C->lim_solver[0];
+        L.px = ((L.ox + C->ng[0] + A - 1) / A) * A;
+        L.plane = (long long)L.px * C->ng[1];
+        L.total = L.plane * C->ng[2] + A;
+    }
+    Ctx& X = *C;
+    if (X.hip(hipStreamCreateWithFlags(&X.stream, hipStreamNonBlocking), "stream")) {
+        g_create_error = X.err;
+        return PPS_ERR_HIP;
+    }
+    int rc = PPS_OK;
+    rc = rc ? rc : X.hip(hipEventCreateWithFlags(&X.ev_batch[0], hipEventDisableTiming), "event");
+    rc = rc ? rc : X.hip(hipEventCreateWithFlags(&X.ev_batch[1], hipEventDisableTiming), "event");
+    rc = rc ? rc : X.hip(hipEventCreate(&X.ev_t0), "event");
+    rc = rc ? rc : X.hip(hipEventCreate(&X.ev_t1), "event");
+    // reduction workspace: enough partials for the largest launch
+    // partials for the largest launch: the whole box at the smallest kz allowed
+    const Geo gbig = make_geo(X.L64, X.box, X.coef, kMinKz);
+    X.rb.cap = gbig.nblocks + 8;
+    rc = rc ? rc : X.hip(hipMalloc(&X.rb.partials, sizeof(double) * kMaxRedQ * X.rb.cap), "partials");
+    rc = rc ? rc : X.hip(hipMalloc(&X.rb.counter, 256), "counter");
+    rc = rc ? rc : X.hip(hipMemsetAsync(X.rb.counter, 0, 256, X.stream), "counter");
+    rc = rc ? rc : X.hip(hipMalloc(&X.scal_d, sizeof(Scal) * X.nscal), "scalars");
+    rc = rc ? rc : X.hip(hipMemsetAsync(X.scal_d, 0, sizeof(Scal) * X.nscal, X.stream), "scalars");
+    rc = rc ? rc : X.hip(hipHostMalloc(&X.flag_h, 64, hipHostMallocDefault), "pinned flags");
+    rc = rc ? rc : X.hip(hipHostMalloc(&X.scal_h, sizeof(Scal), hipHostMallocDefault), "pinned scalars");
+    X.hist_cap = 1 << 20;
+    rc = rc ? rc : X.hip(hipMalloc(&X.hist_d, sizeof(double) * X.hist_cap), "history");
+    rc = rc ? rc : X.alloc_field(X.fields[PPS_FIELD_X], 64);
+    rc = rc ? rc : X.alloc_field(X.fields[PPS_FIELD_B], 64);
+    // Neumann faces
+    X.faces.n = 0;
+    for (int f = 0; f < 6; ++f) {
+        const int d = f / 2;
+        const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+        X.fng[f][0] = X.ng[d1];
+        X.fng[f][1] = X.ng[d2];
+        if (grid->bcs_type[f] == 1 && !rc) {
+            const size_t bytes = sizeof(double) * X.ng[d1] * X.ng[d2];
+            rc = X.hip(hipMalloc(&X.flux_d[f], bytes), "flux");
+            if (!rc) rc = X.hip(hipMemsetAsync(X.flux_d[f], 0, bytes, X.stream), "flux");
+            const int q = X.faces.n++;
+            X.faces.face[q] = f;
+            X.faces.flux[q] = X.flux_d[f];
+        }
+    }
+    for (int f = 0; f < 6; ++f) {
+        X.faces.lim_data[f] = X.lim_data[f];
+        X.faces.fng[f][0] = X.fng[f][0];
+        X.faces.fng[f][1] = X.fng[f][1];
+    }
+    for (int d = 0; d < 3; ++d) X.faces.ds[d] = grid->ds[d];
+    if (!rc) rc = X.hip(hipStreamSynchronize(X.stream), "sync");
+    if (rc) {
+        g_create_error = X.err;
+        pps_destroy(C.release());
+        return rc;
+    }
+    *out = C.release();
+    return PPS_OK;
+}
+
+void pps_destroy(void* ctx)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return;
+    if (C->stream) (void)hipStreamSynchronize(C->stream);
+    C->nn.reset();
+    for (auto& kv : C->work) (void)hipFree(kv.second.p);
+    for (auto& f : C->fields)
+        if (f.p) (void)hipFree(f.p);
+    for (auto* p : C->flux_d)
+        if (p) (void)hipFree(p);
+    if (C->rb.partials) (void)hipFree(C->rb.partials);
+    if (C->rb.counter) (void)hipFree(C->rb.counter);
+    if (C->scal_d) (void)hipFree(C->scal_d);
+    if (C->hist_d) (void)hipFree(C->hist_d);
+    if (C->flag_h) (void)hipHostFree(C->flag_h);
+    if (C->scal_h) (void)hipHostFree(C->scal_h);
+    for (auto e : C->ev_batch)
+        if (e) (void)hipEventDestroy(e);
+    if (C->ev_t0) (void)hipEventDestroy(C->ev_t0);
+    if (C->ev_t1) (void)hipEventDestroy(C->ev_t1);
+    if (C->stream) (void)hipStreamDestroy(C->stream);
+    delete C;
+}
+
+int pps_get_extent(void* ctx, int ng[3])
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C || !ng) return PPS_ERR_ARG;
+    for (int d = 0; d < 3; ++d) ng[d] = C->ng[d];
+    return PPS_OK;
+}
+
+static int field_of(Ctx& C, int id, Field& f)
+{
+    if (id < 0 || id >= PPS_NUM_FIELDS) return C.fail(PPS_ERR_ARG, "bad field id %d", id);
+    if (!C.fields[id].p) {
+        int rc = C.alloc_field(C.fields[id], 64);
+        if (rc) return rc;
+    }
+    f = C.fields[id];
+    return PPS_OK;
+}
+
+int pps_upload(void* ctx, int field, const double* host)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C || !host) return PPS_ERR_ARG;
+    Field f;
+    int rc = field_of(*C, field, f);
+    if (rc) return rc;
+    hipMemcpy3DParms p{};
+    p.srcPtr = make_hipPitchedPtr(const_cast<double*>(host), sizeof(double) * C->ng[0], C->ng[0], C->ng[1]);
+    p.dstPtr = make_hipPitchedPtr(f.d() + C->L64.ox, sizeof(double) * C->L64.px, C->ng[0], C->ng[1]);
+    p.extent = make_hipExtent(sizeof(double) * C->ng[0], C->ng[1], C->ng[2]);
+    p.kind = hipMemcpyHostToDevice;
+    rc = C->hip(hipMemcpy3DAsync(&p, C->stream), "upload");
+    if (rc) return rc;
+    return C->hip(hipStreamSynchronize(C->stream), "sync");
+}
+
+int pps_download(void* ctx, int field, double* host)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C || !host) return PPS_ERR_ARG;
+    Field f;
+    int rc = field_of(*C, field, f);
+    if (rc) return rc;
+    hipMemcpy3DParms p{};
+    p.srcPtr = make_hipPitchedPtr(f.d() + C->L64.ox, sizeof(double) * C->L64.px, C->ng[0], C->ng[1]);
+    p.dstPtr = make_hipPitchedPtr(host, sizeof(double) * C->ng[0], C->ng[0], C->ng[1]);
+    p.extent = make_hipExtent(sizeof(double) * C->ng[0], C->ng[1], C->ng[2]);
+    p.kind = hipMemcpyDeviceToHost;
+    rc = C->hip(hipMemcpy3DAsync(&p, C->stream), "download");
+    if (rc) return rc;
+    return C->hip(hipStreamSynchronize(C->stream), "sync");
+}
+
+int pps_fill(void* ctx, int field, double value)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    Field f;
+    int rc = field_of(*C, field, f);
+    if (rc) return rc;
+    launch_fill<double>(C->gbox(64), C->stream, f.d(), value);
+    return C->hip(hipStreamSynchronize(C->stream), "fill");
+}
+
+int pps_set_neumann_flux(void* ctx, int face, const double* host)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C || !host || face < 0 || face > 5) return PPS_ERR_ARG;
+    if (!C->flux_d[face]) return C->fail(PPS_ERR_ARG, "face %d is not a Neumann face", face);
+    const size_t bytes = sizeof(double) * C->fng[face][0] * C->fng[face][1];
+    int rc = C->hip(hipMemcpyAsync(C->flux_d[face], host, bytes, hipMemcpyHostToDevice, C->stream), "flux upload");
+    if (rc) return rc;
+    return C->hip(hipStreamSynchronize(C->stream), "sync");
+}
+
+int pps_solve(void* ctx, const pps_stage* chain, int nstages, pps_result* res)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    std::vector<std::unique_ptr<Stage>> st;
+    int rc = build_chain(*C, chain, nstages, st);
+    if (rc) return rc;
+    C->err.clear();
+    rc = C->hip(hipEventRecord(C->ev_t0, C->stream), "event");
+    if (rc) return rc;
+    rc = st[0]->solve_main();
+    if (rc) return rc;
+    rc = C->hip(hipEventRecord(C->ev_t1, C->stream), "event");
+    if (!rc) rc = C->hip(hipEventSynchronize(C->ev_t1), "event sync");
+    if (!rc) rc = C->hip(hipGetLastError(), "kernel");
+    if (rc) return rc;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, C->ev_t0, C->ev_t1);
+    Scal h;
+    rc = scal_read(*C, st[0]->S, h);
+    if (rc) return rc;
+    if (res) {
+        std::memset(res, 0, sizeof *res);
+        const int kind = chain[0].kind;
+        res->seconds = ms * 1e-3;
+        res->norm_b = h.v[S_NORM];
+        if (kind == PPS_CHEBYSHEV) {
+            res->iterations = chain[0].max_iter;
+            res->error_from_iteration = h.v[S_ERR_OP];
+            res->error_compute_operator = h.v[S_ERR_OP];
+            res->history_len = 0;
+        } else if (kind == PPS_IR) {
+            auto* ir = static_cast<IRStage*>(st[0].get());
+            res->iterations = (int)ir->inner_total;
+            res->outer_iterations = ir->outer_steps;
+            res->error_from_iteration = h.v[S_ERR];
+            res->error_compute_operator = h.v[S_ERR_OP];
+            res->history_len = std::min(h.outer + 1, C->hist_cap);
+        } else {
+            res->iterations = h.iter;
+            res->error_from_iteration = h.done == 2 ? h.v[S_ERR0] : h.v[S_ERR];
+            res->error_compute_operator = h.v[S_ERR_OP];
+            res->history_len = std::min(h.iter + 1, C->hist_cap);
+        }
+        long long pi = 0;
+        for (size_t i = 1; i < st.size(); ++i) pi += st[i]->prec_iters;
+        res->precond_iterations = pi;
+        res->fallback_switches = C->nn ? C->nn->fallback_switches() : 0;
+        C->hist_len = res->history_len;
+    }
+    return PPS_OK;
+}
+
+int pps_get_history(void* ctx, double* out, int cap)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C || !out || cap < 0) return PPS_ERR_ARG;
+    const int n = std::min(cap, C->hist_len);
+    if (n == 0) return 0;
+    int rc = C->hip(hipMemcpy(out, C->hist_d, sizeof(double) * n, hipMemcpyDeviceToHost), "history");
+    return rc ? rc : n;
+}
+
+int pps_synchronize(void* ctx)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    return C->hip(hipStreamSynchronize(C->stream), "sync");
+}
+
+int pps_apply_A(void* ctx, int y, int x)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    Field fy, fx;
+    int rc = field_of(*C, y, fy);
+    if (!rc) rc = field_of(*C, x, fx);
+    if (rc) return rc;
+    launch_apply_A<double>(C->gsolver(64), C->stream, fx.d(), fy.d());
+    return C->hip(hipStreamSynchronize(C->stream), "apply_A");
+}
+
+int pps_ghost_fill(void* ctx, int field, int with_flux, double norm)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    Field f;
+    int rc = field_of(*C, field, f);
+    if (rc) return rc;
+    Scal* S = C->scal(C->nscal - 1);
+    rc = scal_init(*C, S, 0.0, 0, 1.0, nullptr, 0);
+    if (rc) return rc;
+    scal_op(*C, S, SC_SET, S_NORM, norm);
+    C->ghost(f, with_flux != 0, S, nullptr);
+    return C->hip(hipStreamSynchronize(C->stream), "ghost");
+}
+
+int pps_residual(void* ctx, int r, int x, int b, double* norm)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    Field fr, fx, fb;
+    int rc = field_of(*C, r, fr);
+    if (!rc) rc = field_of(*C, x, fx);
+    if (!rc) rc = field_of(*C, b, fb);
+    if (rc) return rc;
+    Scal* S = C->scal(C->nscal - 1);
+    rc = scal_init(*C, S, 0.0, 0, 1.0, nullptr, 0);
+    if (rc) return rc;
+    launch_residual(C->gsolver(64), C->stream, C->rb, fx.d(), fb.d(), fr.d(), S, 1);
+    Scal h;
+    rc = scal_read(*C, S, h);
+    if (rc) return rc;
+    if (norm) *norm = h.v[S_ERR_OP];
+    return PPS_OK;
+}
+
+int pps_dot(void* ctx, int a, int b, double* out)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C || !out) return PPS_ERR_ARG;
+    Field fa, fb;
+    int rc = field_of(*C, a, fa);
+    if (!rc) rc = field_of(*C, b, fb);
+    if (rc) return rc;
+    Scal* S = C->scal(C->nscal - 1);
+    launch_dot<double>(C->gsolver(64), C->stream, C->rb, fa.d(), fb.d(), &S->v[S_AUX0], nullptr);
+    Scal h;
+    rc = scal_read(*C, S, h);
+    if (rc) return rc;
+    *out = h.v[S_AUX0];
+    return PPS_OK;
+}
+
+int pps_norm_b_adjusted(void* ctx, double* out)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C || !out) return PPS_ERR_ARG;
+    Scal* S = C->scal(C->nscal - 1);
+    launch_bnorm(C->gsolver(64), C->stream, C->rb, C->fields[PPS_FIELD_X].d(), C->fields[PPS_FIELD_B].d(),
+                 C->grid.bcs_type, C->lim_data, C->lim_solver, C->flux_d, C->fng, C->grid.ds, S);
+    Scal h;
+    int rc = scal_read(*C, S, h);
+    if (rc) return rc;
+    *out = h.v[S_NORM];
+    return PPS_OK;
+}
+
+int pps_set_tiling(void* ctx, int kz, int ty)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    if (ty != 0 && ty != kTY) return C->fail(PPS_ERR_UNSUPPORTED, "tile height is fixed at %d rows", kTY);
+    if (kz != 0 && (kz < kMinKz || kz > 4096)) return C->fail(PPS_ERR_ARG, "kz must be in [%d, 4096]", kMinKz);
+    if (kz > 0) C->kz = kz;
+    return PPS_OK;
+}
+
+int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C || reps < 1 || !ms || !bytes) return PPS_ERR_ARG;
+    const int P = (kernel == PPS_K_CG_STENCIL_F32 || kernel == PPS_K_CG_UPDATE_F32) ? 32 : 64;
+    const double w = P == 32 ? 4.0 : 8.0;
+    const double n = (double)C->solver.count();
+    Field a, b, c, d, e;
+    int rc = C->get_work("time.a", P, a);
+    if (!rc) rc = C->get_work("time.b", P, b);
+    if (!rc) rc = C->get_work("time.c", P, c);
+    if (!rc) rc = C->get_work("time.d", P, d);
+    if (!rc) rc = C->get_work("time.e", P, e);
+    if (rc) return rc;
+    Scal* S = C->scal(C->nscal - 2);
+    rc = scal_init(*C, S, -1.0, 1 << 30, 1.0, nullptr, 0);
+    if (rc) return rc;
+    scal_op(*C, S, SC_SET, S_RZ, 1.0);
+    scal_op(*C, S, SC_SET, S_BETA, 0.5);
+    scal_op(*C, S, SC_SET, S_ALPHA, 1e-3);
+    if (P == 64) {
+        launch_fill<double>(C->gsolver(64), C->stream, a.d(), 1.0);
+        launch_fill<double>(C->gsolver(64), C->stream, b.d(), 0.5);
+        launch_fill<double>(C->gsolver(64), C->stream, c.d(), 0.25);
+    } else {
+        launch_fill<float>(C->gsolver(32), C->stream, a.f(), 1.0f);
+        launch_fill<float>(C->gsolver(32), C->stream, b.f(), 0.5f);
+        launch_fill<float>(C->gsolver(32), C->stream, c.f(), 0.25f);
+    }
+    const Geo g = C->gsolver(P);
+    auto launch = [&](int it) {
+        switch (kernel) {
+        case PPS_K_CG_STENCIL:
+            launch_cg_A2<double>(g, C->stream, C->rb, a.d(), (it & 1) ? c.d() : b.d(), (it & 1) ? b.d() : c.d(), d.d(), S, true);
+            break;
+        case PPS_K_CG_STENCIL_F32:
+            launch_cg_A2<float>(g, C->stream, C->rb, a.f(), (it & 1) ? c.f() : b.f(), (it & 1) ? b.f() : c.f(), d.f(), S, true);
+            break;
+        case PPS_K_CG_UPDATE: launch_cg_B<double>(g, C->stream, C->rb, e.d(), b.d(), a.d(), c.d(), S, true, false); break;
+        case PPS_K_CG_UPDATE_F32: launch_cg_B<float>(g, C->stream, C->rb, e.f(), b.f(), a.f(), c.f(), S, true, false); break;
+        case PPS_K_RESIDUAL: launch_residual(g, C->stream, C->rb, a.d(), b.d(), c.d(), S, 1); break;
+        case PPS_K_COPY: launch_copy<double>(g, C->stream, a.d(), e.d()); break;
+        case PPS_K_CHEB_SWEEP: launch_cheb_sweep<double, double>(g, C->stream, a.d(), b.d(), c.d(), e.d(), 0.1, 0.2, 0.3, 1.0, nullptr); break;
+        }
+    };
+    double per = 0;
+    switch (kernel) {
+    case PPS_K_CG_STENCIL: case PPS_K_CG_STENCIL_F32: per = 4; break;   // read r, p ; write p, Ap
+    case PPS_K_CG_UPDATE: case PPS_K_CG_UPDATE_F32: per = 6; break;     // read x, p, r, Ap ; write x, r
+    case PPS_K_RESIDUAL: per = 3; break;                                // read x, b ; write r
+    case PPS_K_COPY: per = 2; break;
+    case PPS_K_CHEB_SWEEP: per = 4; break;                              // read Y, B, Z ; write W
+    default: return C->fail(PPS_ERR_ARG, "unknown kernel id %d", kernel);
+    }
+    for (int i = 0; i < 3; ++i) launch(i);  // warm-up
+    rc = C->hip(hipEventRecord(C->ev_t0, C->stream), "event");
+    for (int i = 0; i < reps; ++i) launch(i);
+    if (!rc) rc = C->hip(hipEventRecord(C->ev_t1, C->stream), "event");
+    if (!rc) rc = C->hip(hipEventSynchronize(C->ev_t1), "event sync");
+    if (!rc) rc = C->hip(hipGetLastError(), "kernel");
+    if (rc) return rc;
+    float t = 0;
+    (void)hipEventElapsedTime(&t, C->ev_t0, C->ev_t1);
+    *ms = t / reps;
+    *bytes = per * n * w;
+    return PPS_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ right-hand sides
+
+namespace {
+
+uint64_t mix64h(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double u01h(uint64_t seed, uint64_t stream, uint64_t ctr)
+{
+    const uint64_t r = mix64h(mix64h(mix64h(seed) ^ stream) ^ ctr);
+    return ((double)(r >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+double normalh(uint64_t seed, uint64_t stream, uint64_t ctr)
+{
+    const double a = u01h(seed, stream, 2u * ctr), b = u01h(seed, stream, 2u * ctr + 1u);
+    return std::sqrt(-2.0 * std::log(a)) * std::cos(6.283185307179586 * b);
+}
+
+// temporary device buffer freed at scope exit (after a stream sync)
+struct TmpDev {
+    void* p = nullptr;
+    ~TmpDev()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int pps_rhs_normal(void* ctx, uint64_t seed)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    TmpDev t;
+    int rc = C->hip(hipMalloc(&t.p, sizeof(double) * 4096), "rhs scratch");
+    if (rc) return rc;
+    launch_rhs_normal(C->stream, C->L64, C->solver, C->np, seed, C->fields[PPS_FIELD_B].d(), static_cast<double*>(t.p));
+    return C->hip(hipStreamSynchronize(C->stream), "rhs_normal");
+}
+
+int pps_rhs_gauss(void* ctx, uint64_t seed, int nsrc, double wmin, double wmax)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    if (nsrc < 1 || nsrc > 256) return C->fail(PPS_ERR_ARG, "nsrc must be in [1,256]");
+    // centre uniform in [0,1]^3, width uniform in [wmin,wmax], amplitude N(0,1)
+    std::vector<double> tab(5 * (size_t)nsrc);
+    for (int s = 0; s < nsrc; ++s) {
+        tab[s] = u01h(seed, 2u, 4u * (uint64_t)s + 0u);
+        tab[nsrc + s] = u01h(seed, 2u, 4u * (uint64_t)s + 1u);
+        tab[2 * nsrc + s] = u01h(seed, 2u, 4u * (uint64_t)s + 2u);
+        tab[3 * nsrc + s] = wmin + (wmax - wmin) * u01h(seed, 2u, 4u * (uint64_t)s + 3u);
+        tab[4 * nsrc + s] = normalh(seed, 4u, (uint64_t)s);
+    }
+    TmpDev t;
+    int rc = C->hip(hipMalloc(&t.p, sizeof(double) * tab.size()), "gauss table");
+    if (!rc) rc = C->hip(hipMemcpy(t.p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice), "gauss table");
+    if (rc) return rc;
+    launch_rhs_gauss(C->stream, C->L64, C->solver, static_cast<double*>(t.p), nsrc, C->grid.origin, C->grid.ds,
+                     C->fields[PPS_FIELD_B].d());
+    return C->hip(hipStreamSynchronize(C->stream), "rhs_gauss");
+}
+
+int pps_rhs_spectrum(void* ctx, uint64_t seed, int K, int accumulate)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    if (K < 1 || K > 256) return C->fail(PPS_ERR_ARG, "kmax must be in [1,256]");
+    const double PI = 3.141592653589793;
+    std::vector<double> coef((size_t)K * K * K);
+    for (int kz = 1; kz <= K; ++kz)
+        for (int ky = 1; ky <= K; ++ky)
+            for (int kx = 1; kx <= K; ++kx) {
+                const uint64_t idx = (uint64_t)(kx - 1) + (uint64_t)K * ((uint64_t)(ky - 1) + (uint64_t)K * (uint64_t)(kz - 1));
+                const double k2 = (double)kx * kx + (double)ky * ky + (double)kz * kz;
+                coef[idx] = normalh(seed, 3u, idx) / k2;
+            }
+    std::vector<double> S[3];
+    for (int d = 0; d < 3; ++d) {
+        const int n = C->np[d];
+        S[d].resize((size_t)K * n);
+        for (int k = 1; k <= K; ++k)
+            for (int i = 0; i < n; ++i) S[d][(k - 1) + (size_t)K * i] = std::sin(k * PI * (double)i / (n - 1));
+    }
+    TmpDev dc, dsx, dsy, dsz, t1, t2;
+    int rc = C->hip(hipMalloc(&dc.p, sizeof(double) * coef.size()), "spec");
+    if (!rc) rc = C->hip(hipMalloc(&dsx.p, sizeof(double) * S[0].size()), "spec");
+    if (!rc) rc = C->hip(hipMalloc(&dsy.p, sizeof(double) * S[1].size()), "spec");
+    if (!rc) rc = C->hip(hipMalloc(&dsz.p, sizeof(double) * S[2].size()), "spec");
+    if (!rc) rc = C->hip(hipMalloc(&t1.p, sizeof(double) * (size_t)K * K * C->np[2]), "spec");
+    if (!rc) rc = C->hip(hipMalloc(&t2.p, sizeof(double) * (size_t)K * C->np[1] * C->np[2]), "spec");
+    if (!rc) rc = C->hip(hipMemcpy(dc.p, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice), "spec");
+    if (!rc) rc = C->hip(hipMemcpy(dsx.p, S[0].data(), sizeof(double) * S[0].size(), hipMemcpyHostToDevice), "spec");
+    if (!rc) rc = C->hip(hipMemcpy(dsy.p, S[1].data(), sizeof(double) * S[1].size(), hipMemcpyHostToDevice), "spec");
+    if (!rc) rc = C->hip(hipMemcpy(dsz.p, S[2].data(), sizeof(double) * S[2].size(), hipMemcpyHostToDevice), "spec");
+    if (rc) return rc;
+    launch_rhs_spectrum(C->stream, C->L64, C->solver, C->np, K, static_cast<double*>(dc.p), static_cast<double*>(dsx.p),
+                        static_cast<double*>(dsy.p), static_cast<double*>(dsz.p), static_cast<double*>(t1.p),
+                        static_cast<double*>(t2.p), accumulate, C->fields[PPS_FIELD_B].d());
+    return C->hip(hipStreamSynchronize(C->stream), "rhs_spectrum");
+}
+
+}  // extern "C"

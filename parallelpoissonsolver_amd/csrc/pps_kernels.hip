@@ -1,0 +1,1283 @@
+// pps_kernels.hip -- hand-written gfx950 kernels of the Krylov hot path.
+//
+// Two kernel skeletons carry every operation:
+//  * k_stencil: 2.5-D z-marching tile.  A block of 64x4 threads owns a 64x4
+//    column tile of one x-y slab and marches through kz planes.  Each thread
+//    keeps its column's (k-1, k, k+1) values in registers; the plane k tile and
+//    its one-point halo are staged in LDS (double-buffered, one barrier per
+//    plane), so x- and y-neighbours come from LDS and every value of the input
+//    field is produced once per plane (this is what lets the CG search
+//    direction p = z + beta*p be formed on the fly inside the stencil).
+//    The 7-point operator is the reference's +Laplacian
+//    (matrixFreeOperatorA.hpp:33-38) written in difference form,
+//    cx((x[i-1]-x)+(x[i+1]-x)) + cy(..) + cz(..), which keeps FP32 accurate
+//    for smooth fields.
+//  * k_vector: the same tiling without halo, for fused axpy/dot updates.
+//
+// Reductions are FP64 in every precision: wavefront shuffles, LDS across the
+// four wavefronts, one partial per block, and the last block to arrive sums
+// the partials in a fixed order (deterministic, no host round trip) and runs
+// the solver's scalar epilogue (alpha, beta, omega, convergence flag).
+//
+// Blocks are remapped so that consecutive tiles land on the same XCD (the
+// hardware deals blocks round-robin over the 8 XCDs, each with its own L2):
+// the y-halo rows a tile reads from its neighbour are then L2 hits.
+#include "pps_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+
+namespace pps {
+
+Geo make_geo(const Layout& L, const Region& R, const double coef[3], int kz)
+{
+    Geo g{};
+    g.plane = L.plane;
+    g.px = L.px;
+    g.ox = L.ox;
+    g.lo0 = R.lo[0]; g.hi0 = R.hi[0];
+    g.lo1 = R.lo[1]; g.hi1 = R.hi[1];
+    g.lo2 = R.lo[2]; g.hi2 = R.hi[2];
+    g.kz = kz > 0 ? kz : 1;
+    g.ntx = (R.hi[0] - R.lo[0] + kTX - 1) / kTX;
+    g.nty = (R.hi[1] - R.lo[1] + kTY - 1) / kTY;
+    const int ntz = (R.hi[2] - R.lo[2] + g.kz - 1) / g.kz;
+    g.nblocks = g.ntx * g.nty * ntz;
+    g.cx = coef[0];
+    g.cy = coef[1];
+    g.cz = coef[2];
+    return g;
+}
+
+// ------------------------------------------------------------ common pieces
+
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+    const int full = nb & ~7;
+    if (b >= full) return b;
+    return (b & 7) * (full >> 3) + (b >> 3);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int NQ, class Fin>
+__device__ __forceinline__ void grid_reduce(double (&v)[NQ], const RedBuf& rb, Fin& fin)
+{
+    constexpr int NW = kThreads / 64;
+    __shared__ double sm[NQ][NW];
+    __shared__ int amLast;
+    const int tid = threadIdx.x + kTX * threadIdx.y;
+    const int lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double t = wave_sum(v[q]);
+        if (lane == 0) sm[q][w] = t;
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double t = 0.0;
+#pragma unroll
+            for (int ww = 0; ww < NW; ++ww) t += sm[q][ww];
+            __hip_atomic_store(&rb.partials[(size_t)q * rb.cap + blockIdx.x], t, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __threadfence();
+        const unsigned ticket = atomicAdd(rb.counter, 1u);
+        amLast = (ticket == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!amLast) return;
+    __threadfence();
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    for (int b = tid; b < (int)gridDim.x; b += kThreads) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+            acc[q] += __hip_atomic_load(&rb.partials[(size_t)q * rb.cap + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const double t = wave_sum(acc[q]);
+        if (lane == 0) sm[q][w] = t;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double tot[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            tot[q] = 0.0;
+#pragma unroll
+            for (int ww = 0; ww < NW; ++ww) tot[q] += sm[q][ww];
+        }
+        fin.finish(tot);
+        __threadfence();
+        __hip_atomic_store(rb.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+struct Gate {
+    const int* gate;
+    __device__ __forceinline__ bool gated() const { return gate != nullptr && *gate != 0; }
+};
+
+// record the end of one iteration of a main/inner Krylov loop
+__device__ __forceinline__ void finish_iteration(Scal* S, double err, bool history)
+{
+    S->v[S_ERR] = err;
+    const int it = S->iter + 1;
+    S->iter = it;
+    if (history && S->hist != nullptr && it < S->hist_cap) S->hist[it] = err;
+    if (err < S->v[S_TOL] || it >= S->maxit) S->done = 1;
+}
+
+// --------------------------------------------------------- kernel skeletons
+
+template <typename T, int NQ, class Op>
+__global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
+{
+    if (op.gated()) return;
+    op.load();
+    __shared__ T lds[2][kTY + 2][kTX + 2];
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx0 = b % g.ntx;
+    const int rr = b / g.ntx;
+    const int ty0 = rr % g.nty;
+    const int tz0 = rr / g.nty;
+    const int lx = threadIdx.x, ly = threadIdx.y, tid = lx + kTX * ly;
+    const int i = g.lo0 + tx0 * kTX + lx;
+    const int j = g.lo1 + ty0 * kTY + ly;
+    const int k0 = g.lo2 + tz0 * g.kz;
+    const int k1 = min(k0 + g.kz, g.hi2);
+    const bool act = (i < g.hi0) && (j < g.hi1);
+    const bool inb = (i <= g.hi0) && (j <= g.hi1);
+
+    // halo producers: 128 threads for the two y-halo rows, 8 for the x-halo columns
+    int hr = 0, hc = 0, hi = 0, hj = 0;
+    bool hv = false;
+    if (tid < 2 * kTX) {
+        const int col = tid % kTX;
+        const bool top = tid >= kTX;
+        hr = top ? kTY + 1 : 0;
+        hc = col + 1;
+        hi = g.lo0 + tx0 * kTX + col;
+        hj = g.lo1 + ty0 * kTY + (top ? kTY : -1);
+        hv = true;
+    } else if (tid < 2 * kTX + 2 * kTY) {
+        const int t = tid - 2 * kTX;
+        const int row = t % kTY;
+        const bool right = t >= kTY;
+        hr = row + 1;
+        hc = right ? kTX + 1 : 0;
+        hi = g.lo0 + tx0 * kTX + (right ? kTX : -1);
+        hj = g.lo1 + ty0 * kTY + row;
+        hv = true;
+    }
+    const bool hin = hv && hi <= g.hi0 && hj <= g.hi1;
+    const long long hbase = (long long)g.ox + hi + (long long)g.px * hj;
+    const long long base = (long long)g.ox + i + (long long)g.px * j;
+    const T cx = (T)g.cx, cy = (T)g.cy, cz = (T)g.cz;
+
+    double acc[NQ > 0 ? NQ : 1];
+#pragma unroll
+    for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) acc[q] = 0.0;
+
+    T vm = inb ? op.in(base + g.plane * (k0 - 1)) : T(0);
+    T vc = inb ? op.in(base + g.plane * k0) : T(0);
+    int buf = 0;
+    for (int k = k0; k < k1; ++k) {
+        const long long c = base + g.plane * k;
+        lds[buf][ly + 1][lx + 1] = vc;
+        if (hv) lds[buf][hr][hc] = hin ? op.in(hbase + g.plane * k) : T(0);
+        const T vp = inb ? op.in(c + g.plane) : T(0);
+        __syncthreads();
+        if (act) {
+            const T xm = lds[buf][ly + 1][lx], xp = lds[buf][ly + 1][lx + 2];
+            const T ym = lds[buf][ly][lx + 1], yp = lds[buf][ly + 2][lx + 1];
+            const T Av = cx * ((xm - vc) + (xp - vc)) + cy * ((ym - vc) + (yp - vc)) + cz * ((vm - vc) + (vp - vc));
+            op.out(c, vc, Av, acc);
+        }
+        vm = vc;
+        vc = vp;
+        buf ^= 1;
+    }
+    if constexpr (NQ > 0) grid_reduce<NQ>(acc, rb, op);
+}
+
+template <int NQ, class Op>
+__global__ __launch_bounds__(kThreads) void k_vector(Geo g, Op op, RedBuf rb)
+{
+    if (op.gated()) return;
+    op.load();
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx0 = b % g.ntx;
+    const int rr = b / g.ntx;
+    const int ty0 = rr % g.nty;
+    const int tz0 = rr / g.nty;
+    const int i = g.lo0 + tx0 * kTX + threadIdx.x;
+    const int j = g.lo1 + ty0 * kTY + threadIdx.y;
+    const int k0 = g.lo2 + tz0 * g.kz;
+    const int k1 = min(k0 + g.kz, g.hi2);
+    double acc[NQ > 0 ? NQ : 1];
+#pragma unroll
+    for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) acc[q] = 0.0;
+    if (i < g.hi0 && j < g.hi1) {
+        long long c = (long long)g.ox + i + (long long)g.px * j + g.plane * k0;
+#pragma unroll 4
+        for (int k = k0; k < k1; ++k, c += g.plane) op.point(c, i, j, k, acc);
+    }
+    if constexpr (NQ > 0) grid_reduce<NQ>(acc, rb, op);
+}
+
+template <typename T, int NQ, class Op>
+static void run_stencil(const Geo& g, hipStream_t s, const Op& op, RedBuf rb)
+{
+    if (g.nblocks <= 0) return;
+    if (NQ > 0 && g.nblocks > rb.cap) {
+        std::fprintf(stderr, "pps: %d blocks exceed the reduction workspace (%d); launch skipped\n", g.nblocks, rb.cap);
+        return;
+    }
+    hipLaunchKernelGGL((k_stencil<T, NQ, Op>), dim3(g.nblocks), dim3(kTX, kTY), 0, s, g, op, rb);
+}
+
+template <int NQ, class Op>
+static void run_vector(const Geo& g, hipStream_t s, const Op& op, RedBuf rb)
+{
+    if (g.nblocks <= 0) return;
+    if (NQ > 0 && g.nblocks > rb.cap) {
+        std::fprintf(stderr, "pps: %d blocks exceed the reduction workspace (%d); launch skipped\n", g.nblocks, rb.cap);
+        return;
+    }
+    hipLaunchKernelGGL((k_vector<NQ, Op>), dim3(g.nblocks), dim3(kTX, kTY), 0, s, g, op, rb);
+}
+
+static const RedBuf kNoRed{nullptr, nullptr, 0};
+
+// ------------------------------------------------------------ basic kernels
+
+template <typename T>
+struct OpApply : Gate {
+    const T* __restrict__ x;
+    T* __restrict__ y;
+    __device__ void load() {}
+    __device__ T in(long long c) const { return x[c]; }
+    __device__ void out(long long c, T, T Av, double*) const { y[c] = Av; }
+    __device__ void finish(const double*) {}
+};
+
+template <typename T>
+void launch_apply_A(const Geo& g, hipStream_t s, const T* x, T* y)
+{
+    OpApply<T> op;
+    op.gate = nullptr;
+    op.x = x;
+    op.y = y;
+    run_stencil<T, 0>(g, s, op, kNoRed);
+}
+
+template <typename T>
+struct OpCopy : Gate {
+    const T* __restrict__ x;
+    T* __restrict__ y;
+    __device__ void load() {}
+    __device__ void point(long long c, int, int, int, double*) const { y[c] = x[c]; }
+    __device__ void finish(const double*) {}
+};
+
+template <typename T>
+void launch_copy(const Geo& g, hipStream_t s, const T* x, T* y)
+{
+    OpCopy<T> op;
+    op.gate = nullptr;
+    op.x = x;
+    op.y = y;
+    run_vector<0>(g, s, op, kNoRed);
+}
+
+template <typename T>
+struct OpDot : Gate {
+    const T* __restrict__ a;
+    const T* __restrict__ b;
+    double* out;
+    __device__ void load() {}
+    __device__ void point(long long c, int, int, int, double* acc) const { acc[0] += (double)a[c] * (double)b[c]; }
+    __device__ void finish(const double* t) { *out = t[0]; }
+};
+
+template <typename T>
+void launch_dot(const Geo& g, hipStream_t s, RedBuf rb, const T* a, const T* b, double* out, const int* gate)
+{
+    OpDot<T> op;
+    op.gate = gate;
+    op.a = a;
+    op.b = b;
+    op.out = out;
+    run_vector<1>(g, s, op, rb);
+}
+
+template <typename T>
+struct OpFill : Gate {
+    T* __restrict__ a;
+    T v;
+    __device__ void load() {}
+    __device__ void point(long long c, int, int, int, double*) const { a[c] = v; }
+    __device__ void finish(const double*) {}
+};
+
+template <typename T>
+void launch_fill(const Geo& g, hipStream_t s, T* a, T v)
+{
+    OpFill<T> op;
+    op.gate = nullptr;
+    op.a = a;
+    op.v = v;
+    run_vector<0>(g, s, op, kNoRed);
+}
+
+template <typename T>
+struct OpScaleBox : Gate {
+    T* __restrict__ a;
+    const Scal* S;
+    int mode;
+    double nrm;
+    __device__ void load() { nrm = S->v[S_NORM]; }
+    __device__ void point(long long c, int, int, int, double*) const
+    {
+        // normalizeProblemToFieldBNorm divides (iterativeSolverBase.hpp:227-231);
+        // the solvers' epilogues multiply (baseCG.hpp:248-252)
+        if (mode == 0) a[c] = (T)((double)a[c] / nrm);
+        else a[c] = (T)((double)a[c] * nrm);
+    }
+    __device__ void finish(const double*) {}
+};
+
+template <typename T>
+void launch_scale_box(const Geo& gbox, hipStream_t s, T* a, const Scal* S, int mode)
+{
+    OpScaleBox<T> op;
+    op.gate = nullptr;
+    op.a = a;
+    op.S = S;
+    op.mode = mode;
+    run_vector<0>(gbox, s, op, kNoRed);
+}
+
+template <typename T>
+struct OpPointwise : Gate {
+    const T* __restrict__ in;
+    T* __restrict__ out;
+    T c;
+    __device__ void load() {}
+    __device__ void point(long long p, int, int, int, double*) const { out[p] = c * in[p]; }
+    __device__ void finish(const double*) {}
+};
+
+template <typename T>
+void launch_pointwise(const Geo& gbox, hipStream_t s, const T* in, T* out, double c, const int* gate)
+{
+    OpPointwise<T> op;
+    op.gate = gate;
+    op.in = in;
+    op.out = out;
+    op.c = (T)c;
+    run_vector<0>(gbox, s, op, kNoRed);
+}
+
+template <typename TS, typename TD>
+struct OpCast : Gate {
+    const TS* __restrict__ src;
+    TD* __restrict__ dst;
+    __device__ void load() {}
+    __device__ void point(long long c, int, int, int, double*) const { dst[c] = (TD)src[c]; }
+    __device__ void finish(const double*) {}
+};
+
+template <typename TS, typename TD>
+void launch_cast(const Geo& gbox, hipStream_t s, const TS* src, TD* dst, const int* gate)
+{
+    OpCast<TS, TD> op;
+    op.gate = gate;
+    op.src = src;
+    op.dst = dst;
+    run_vector<0>(gbox, s, op, kNoRed);
+}
+
+// ---------------------------------------------------- residual and b-norm
+
+// r = b - A x (iterativeSolverBase.hpp:257-268)
+struct OpResidual : Gate {
+    const double* __restrict__ x;
+    const double* __restrict__ b;
+    double* __restrict__ r;
+    Scal* S;
+    int mode;
+    double jac;
+    __device__ void load() { jac = S->v[S_JAC]; }
+    __device__ double in(long long c) const { return x[c]; }
+    __device__ void out(long long c, double, double Av, double* acc) const
+    {
+        const double rv = b[c] - Av;
+        r[c] = rv;
+        acc[0] += rv * rv;
+        acc[1] += rv * (jac * rv);
+    }
+    __device__ void finish(const double* t)
+    {
+        const double e = sqrt(t[0]);
+        if (mode == 0) {
+            S->v[S_ERR0] = e;
+            S->v[S_ERR] = e;
+            S->v[S_ERR_OP] = e;
+            S->v[S_RZ] = t[1];
+            S->v[S_BETA] = 0.0;
+            S->v[S_RHO0] = 1.0;
+            S->iter = 0;
+            if (S->hist != nullptr && S->hist_cap > 0) S->hist[0] = e;
+            S->done = (e < S->v[S_TOL]) ? 2 : 0;
+        } else {
+            S->v[S_ERR_OP] = e;
+        }
+    }
+};
+
+void launch_residual(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, double* r, Scal* S, int mode)
+{
+    OpResidual op;
+    op.gate = nullptr;
+    op.x = x;
+    op.b = b;
+    op.r = r;
+    op.S = S;
+    op.mode = mode;
+    run_stencil<double, 2>(g, s, op, rb);
+}
+
+// ||b_adj||^2 (iterativeSolverBase.hpp:171-201 with adjustFieldBForDirichletNeumanBCs :429-534)
+struct OpBNorm : Gate {
+    const double* __restrict__ x;
+    const double* __restrict__ b;
+    const double* flux[6];
+    int bc[6], ld[6], fng[6][2];
+    double ds[3];
+    long long stride[3];
+    Scal* S;
+    __device__ void load() {}
+    __device__ long long fpos(int d, int i, int j, int k) const
+    {
+        const int f = 2 * d;
+        if (d == 0) return j + (long long)fng[f][0] * k;
+        if (d == 1) return i + (long long)fng[f][0] * k;
+        return i + (long long)fng[f][0] * j;
+    }
+    __device__ void point(long long c, int i, int j, int k, double* acc) const
+    {
+        double v = b[c];
+        const int idx[3] = {i, j, k};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double dd = ds[d] * ds[d];
+            // low face
+            if (bc[2 * d] == 0) {
+                if (idx[d] == ld[2 * d] + 1) v -= x[c - stride[d]] / dd;
+            } else if (bc[2 * d] == 1 && flux[2 * d] != nullptr) {
+                if (idx[d] == ld[2 * d]) v += 2 * flux[2 * d][fpos(d, i, j, k)] / ds[d];
+            }
+            // high face
+            if (bc[2 * d + 1] == 0) {
+                if (idx[d] == ld[2 * d + 1] - 2) v -= x[c + stride[d]] / dd;
+            } else if (bc[2 * d + 1] == 1 && flux[2 * d + 1] != nullptr) {
+                if (idx[d] == ld[2 * d + 1] - 1) v -= 2 * flux[2 * d + 1][fpos(d, i, j, k)] / ds[d];
+            }
+        }
+        acc[0] += v * v;
+    }
+    __device__ void finish(const double* t) { S->v[S_NORM] = sqrt(t[0]); }
+};
+
+void launch_bnorm(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, const int bc[6],
+                  const int lim_data[6], const int lim_solver[6], const double* const flux[6], const int fng[6][2],
+                  const double ds[3], Scal* S)
+{
+    (void)lim_solver;
+    OpBNorm op;
+    op.gate = nullptr;
+    op.x = x;
+    op.b = b;
+    for (int f = 0; f < 6; ++f) {
+        op.flux[f] = flux[f];
+        op.bc[f] = bc[f];
+        op.ld[f] = lim_data[f];
+        op.fng[f][0] = fng[f][0];
+        op.fng[f][1] = fng[f][1];
+    }
+    for (int d = 0; d < 3; ++d) op.ds[d] = ds[d];
+    op.stride[0] = 1;
+    op.stride[1] = g.px;
+    op.stride[2] = g.plane;
+    op.S = S;
+    run_vector<1>(g, s, op, rb);
+}
+
+// adjustFieldBForDirichletNeumanBCs (iterativeSolverBase.hpp:429-534) written
+// into dst over the data region (the main-loop Chebyshev works on this copy)
+struct OpAdjustB : OpBNorm {
+    double* __restrict__ dst;
+    __device__ void point(long long c, int i, int j, int k, double*) const
+    {
+        double v = b[c];
+        const int idx[3] = {i, j, k};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double dd = ds[d] * ds[d];
+            if (bc[2 * d] == 0) {
+                if (idx[d] == ld[2 * d] + 1) v -= x[c - stride[d]] / dd;
+            } else if (bc[2 * d] == 1 && flux[2 * d] != nullptr) {
+                if (idx[d] == ld[2 * d]) v += 2 * flux[2 * d][fpos(d, i, j, k)] / ds[d];
+            }
+            if (bc[2 * d + 1] == 0) {
+                if (idx[d] == ld[2 * d + 1] - 2) v -= x[c + stride[d]] / dd;
+            } else if (bc[2 * d + 1] == 1 && flux[2 * d + 1] != nullptr) {
+                if (idx[d] == ld[2 * d + 1] - 1) v -= 2 * flux[2 * d + 1][fpos(d, i, j, k)] / ds[d];
+            }
+        }
+        dst[c] = v;
+    }
+};
+
+void adjust_b_field(const Geo& gdata, hipStream_t s, const double* x, double* b, const int bc[6], const int lim_data[6],
+                    const double* const flux[6], const int fng[6][2], const double ds[3])
+{
+    OpAdjustB op;
+    op.gate = nullptr;
+    op.x = x;
+    op.b = b;
+    op.dst = b;
+    for (int f = 0; f < 6; ++f) {
+        op.flux[f] = flux[f];
+        op.bc[f] = bc[f];
+        op.ld[f] = lim_data[f];
+        op.fng[f][0] = fng[f][0];
+        op.fng[f][1] = fng[f][1];
+    }
+    for (int d = 0; d < 3; ++d) op.ds[d] = ds[d];
+    op.stride[0] = 1;
+    op.stride[1] = gdata.px;
+    op.stride[2] = gdata.plane;
+    op.S = nullptr;
+    run_vector<0>(gdata, s, op, kNoRed);
+}
+
+// ------------------------------------------------------- Neumann ghost fill
+
+// iterativeSolverBase.hpp:62-169 (orderNeumanBcs == 2): the guard point of a
+// Neumann face takes the value of the data point mirrored through the face
+// point; data fields add -+ 2 ds du/dn / normFieldB_.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ghost(Layout L, Faces F, T* field, int with_flux, const Scal* S, const int* gate)
+{
+    if (gate != nullptr && *gate != 0) return;
+    const int slot = blockIdx.y;
+    const int f = F.face[slot];
+    const int d = f >> 1, side = f & 1;
+    const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+    const int n1 = F.lim_data[2 * d1 + 1] - F.lim_data[2 * d1];
+    const int n2 = F.lim_data[2 * d2 + 1] - F.lim_data[2 * d2];
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n1 * n2) return;
+    int idx[3];
+    idx[d1] = F.lim_data[2 * d1] + (int)(t % n1);
+    idx[d2] = F.lim_data[2 * d2] + (int)(t / n1);
+    idx[d] = side == 0 ? F.lim_data[2 * d] : F.lim_data[2 * d + 1] - 1;
+    const long long e = d == 0 ? 1 : (d == 1 ? (long long)L.px : L.plane);
+    const long long p = L.at(idx[0], idx[1], idx[2]);
+    const long long guard = side == 0 ? p - e : p + e;
+    const long long data = side == 0 ? p + e : p - e;
+    if (with_flux && F.flux[slot] != nullptr) {
+        const double nrm = S != nullptr ? S->v[S_NORM] : 1.0;
+        const double q = F.flux[slot][idx[d1] + (long long)F.fng[f][0] * idx[d2]];
+        const double v = (double)field[data];
+        field[guard] = (T)(side == 0 ? v - 2 * F.ds[d] * q / nrm : v + 2 * F.ds[d] * q / nrm);
+    } else {
+        field[guard] = field[data];
+    }
+}
+
+template <typename T>
+void launch_ghost(hipStream_t s, const Layout& L, const Faces& F, T* field, int with_flux, const Scal* S, const int* gate)
+{
+    if (F.n == 0) return;
+    int maxpts = 0;
+    for (int q = 0; q < F.n; ++q) {
+        const int f = F.face[q], d = f >> 1;
+        const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+        const int pts = (F.lim_data[2 * d1 + 1] - F.lim_data[2 * d1]) * (F.lim_data[2 * d2 + 1] - F.lim_data[2 * d2]);
+        if (pts > maxpts) maxpts = pts;
+    }
+    dim3 grid((maxpts + 255) / 256, F.n);
+    hipLaunchKernelGGL(k_ghost<T>, grid, dim3(256), 0, s, L, F, field, with_flux, S, gate);
+}
+
+// ------------------------------------------------------------------- CG
+
+// p_new = a*src + beta*p_old, Ap = A p_new, p.Ap ; alpha = rz / pAp  (baseCG.hpp:126-150,197-207)
+template <typename T>
+struct OpCgA : Gate {
+    const T* __restrict__ src;
+    const T* __restrict__ pin;
+    T* __restrict__ pout;
+    T* __restrict__ Ap;
+    Scal* S;
+    bool pointwise;
+    T a, beta;
+    __device__ void load()
+    {
+        a = pointwise ? (T)S->v[S_JAC] : T(1);
+        beta = (T)S->v[S_BETA];
+    }
+    __device__ T in(long long c) const { return a * src[c] + beta * pin[c]; }
+    __device__ void out(long long c, T vc, T Av, double* acc) const
+    {
+        pout[c] = vc;
+        Ap[c] = Av;
+        acc[0] += (double)vc * (double)Av;
+    }
+    __device__ void finish(const double* t)
+    {
+        S->v[S_PAP] = t[0];
+        S->v[S_ALPHA] = S->v[S_RZ] / t[0];
+    }
+};
+
+// pin/pout double-buffer the search direction: neighbouring tiles rebuild
+// p_new on their halo from p_old, so p_old must stay intact during the launch.
+template <typename T>
+void launch_cg_A2(const Geo& g, hipStream_t s, RedBuf rb, const T* src, const T* pin, T* pout, T* Ap, Scal* S,
+                  bool pointwise)
+{
+    OpCgA<T> op;
+    op.gate = &S->done;
+    op.src = src;
+    op.pin = pin;
+    op.pout = pout;
+    op.Ap = Ap;
+    op.S = S;
+    op.pointwise = pointwise;
+    run_stencil<T, 1>(g, s, op, rb);
+}
+
+template <typename T>
+struct OpCgB : Gate {
+    T* __restrict__ x;
+    const T* __restrict__ p;
+    T* __restrict__ r;
+    const T* __restrict__ Ap;
+    Scal* S;
+    bool pointwise, history;
+    T alpha, jac;
+    __device__ void load()
+    {
+        alpha = (T)S->v[S_ALPHA];
+        jac = (T)S->v[S_JAC];
+    }
+    __device__ void point(long long c, int, int, int, double* acc) const
+    {
+        x[c] = x[c] + alpha * p[c];
+        const T rv = r[c] - alpha * Ap[c];
+        r[c] = rv;
+        acc[0] += (double)rv * (double)rv;
+        acc[1] += (double)rv * (double)(jac * rv);
+    }
+    __device__ void finish(const double* t)
+    {
+        if (pointwise) {
+            S->v[S_BETA] = t[1] / S->v[S_RZ];
+            S->v[S_RZ] = t[1];
+        }
+        S->v[S_RR] = t[0];
+        finish_iteration(S, sqrt(t[0]), history);
+    }
+};
+
+template <typename T>
+void launch_cg_B(const Geo& g, hipStream_t s, RedBuf rb, T* x, const T* p, T* r, const T* Ap, Scal* S, bool pointwise,
+                 bool history)
+{
+    OpCgB<T> op;
+    op.gate = &S->done;
+    op.x = x;
+    op.p = p;
+    op.r = r;
+    op.Ap = Ap;
+    op.S = S;
+    op.pointwise = pointwise;
+    op.history = history;
+    run_vector<2>(g, s, op, rb);
+}
+
+template <typename T>
+struct OpCgRz : Gate {
+    const T* __restrict__ r;
+    const T* __restrict__ z;
+    Scal* S;
+    __device__ void load() {}
+    __device__ void point(long long c, int, int, int, double* acc) const { acc[0] += (double)r[c] * (double)z[c]; }
+    __device__ void finish(const double* t)
+    {
+        S->v[S_BETA] = S->iter == 0 ? 0.0 : t[0] / S->v[S_RZ];
+        S->v[S_RZ] = t[0];
+    }
+};
+
+template <typename T>
+void launch_cg_rz(const Geo& g, hipStream_t s, RedBuf rb, const T* r, const T* z, Scal* S)
+{
+    OpCgRz<T> op;
+    op.gate = &S->done;
+    op.r = r;
+    op.z = z;
+    op.S = S;
+    run_vector<1>(g, s, op, rb);
+}
+
+// ------------------------------------------------------------- BiCGSTAB
+
+// AMp = A (amp*Mp), r0.AMp ; alpha = rho0 / r0.AMp   (BiCGSTAB.hpp:142-164)
+template <typename T>
+struct OpBiK1 : Gate {
+    const T* __restrict__ Mp;
+    T* __restrict__ AMp;
+    const T* __restrict__ r0;
+    Scal* S;
+    T amp;
+    __device__ void load() {}
+    __device__ T in(long long c) const { return amp * Mp[c]; }
+    __device__ void out(long long c, T, T Av, double* acc) const
+    {
+        AMp[c] = Av;
+        acc[0] += (double)r0[c] * (double)Av;
+    }
+    __device__ void finish(const double* t) { S->v[S_ALPHA] = S->v[S_RHO0] / t[0]; }
+};
+
+template <typename T>
+void launch_bicg_k1(const Geo& g, hipStream_t s, RedBuf rb, const T* Mp, double amp, T* AMp, const T* r0, Scal* S)
+{
+    OpBiK1<T> op;
+    op.gate = &S->done;
+    op.Mp = Mp;
+    op.AMp = AMp;
+    op.r0 = r0;
+    op.S = S;
+    op.amp = (T)amp;
+    run_stencil<T, 1>(g, s, op, rb);
+}
+
+template <typename T>
+struct OpBiK2 : Gate {
+    T* __restrict__ r;
+    const T* __restrict__ AMp;
+    const Scal* S;
+    T alpha;
+    __device__ void load() { alpha = (T)S->v[S_ALPHA]; }
+    __device__ void point(long long c, int, int, int, double*) const { r[c] = r[c] - alpha * AMp[c]; }
+    __device__ void finish(const double*) {}
+};
+
+template <typename T>
+void launch_bicg_k2(const Geo& g, hipStream_t s, T* r, const T* AMp, const Scal* S)
+{
+    OpBiK2<T> op;
+    op.gate = &S->done;
+    op.r = r;
+    op.AMp = AMp;
+    op.S = S;
+    run_vector<0>(g, s, op, kNoRed);
+}
+
+// Az = A (az*z), r.Az, Az.Az ; omega   (BiCGSTAB.hpp:189-225)
+template <typename T>
+struct OpBiK3 : Gate {
+    const T* __restrict__ z;
+    T* __restrict__ Az;
+    const T* __restrict__ r;
+    Scal* S;
+    T az;
+    __device__ void load() {}
+    __device__ T in(long long c) const { return az * z[c]; }
+    __device__ void out(long long c, T, T Av, double* acc) const
+    {
+        Az[c] = Av;
+        acc[0] += (double)r[c] * (double)Av;
+        acc[1] += (double)Av * (double)Av;
+    }
+    __device__ void finish(const double* t) { S->v[S_OMEGA] = t[0] / t[1]; }
+};
+
+template <typename T>
+void launch_bicg_k3(const Geo& g, hipStream_t s, RedBuf rb, const T* z, double az, T* Az, const T* r, Scal* S)
+{
+    OpBiK3<T> op;
+    op.gate = &S->done;
+    op.z = z;
+    op.Az = Az;
+    op.r = r;
+    op.S = S;
+    op.az = (T)az;
+    run_stencil<T, 2>(g, s, op, rb);
+}
+
+// x += alpha Mp + omega z ; r -= omega Az ; r0.r, r.r ; beta   (BiCGSTAB.hpp:227-259)
+template <typename T>
+struct OpBiK45 : Gate {
+    T* __restrict__ x;
+    const T* __restrict__ Mp;
+    const T* __restrict__ z;
+    T* __restrict__ r;
+    const T* __restrict__ Az;
+    const T* __restrict__ r0;
+    Scal* S;
+    T amp, az, alpha, omega;
+    bool history;
+    __device__ void load()
+    {
+        alpha = (T)S->v[S_ALPHA];
+        omega = (T)S->v[S_OMEGA];
+    }
+    __device__ void point(long long c, int, int, int, double* acc) const
+    {
+        const T rv0 = r[c];
+        x[c] = x[c] + alpha * (amp * Mp[c]) + omega * (az * z[c]);
+        const T rv = rv0 - omega * Az[c];
+        r[c] = rv;
+        acc[0] += (double)r0[c] * (double)rv;
+        acc[1] += (double)rv * (double)rv;
+    }
+    __device__ void finish(const double* t)
+    {
+        const double rho1 = t[0];
+        S->v[S_BETA] = rho1 / S->v[S_RHO0] * S->v[S_ALPHA] / S->v[S_OMEGA];
+        S->v[S_RHO0] = rho1;
+        finish_iteration(S, sqrt(t[1]), history);
+    }
+};
+
+template <typename T>
+void launch_bicg_k45(const Geo& g, hipStream_t s, RedBuf rb, T* x, const T* Mp, double amp, const T* z, double az, T* r,
+                     const T* Az, const T* r0, Scal* S, bool history)
+{
+    OpBiK45<T> op;
+    op.gate = &S->done;
+    op.x = x;
+    op.Mp = Mp;
+    op.z = z;
+    op.r = r;
+    op.Az = Az;
+    op.r0 = r0;
+    op.S = S;
+    op.amp = (T)amp;
+    op.az = (T)az;
+    op.history = history;
+    run_vector<2>(g, s, op, rb);
+}
+
+// p = r + beta (p - omega AMp)   (BiCGSTAB.hpp:262-272)
+template <typename T>
+struct OpBiK6 : Gate {
+    T* __restrict__ p;
+    const T* __restrict__ r;
+    const T* __restrict__ AMp;
+    const Scal* S;
+    T beta, omega;
+    __device__ void load()
+    {
+        beta = (T)S->v[S_BETA];
+        omega = (T)S->v[S_OMEGA];
+    }
+    __device__ void point(long long c, int, int, int, double*) const { p[c] = r[c] + beta * (p[c] - omega * AMp[c]); }
+    __device__ void finish(const double*) {}
+};
+
+template <typename T>
+void launch_bicg_k6(const Geo& g, hipStream_t s, T* p, const T* r, const T* AMp, const Scal* S)
+{
+    OpBiK6<T> op;
+    op.gate = &S->done;
+    op.p = p;
+    op.r = r;
+    op.AMp = AMp;
+    op.S = S;
+    run_vector<0>(g, s, op, kNoRed);
+}
+
+// ------------------------------------------------------------ Chebyshev
+
+// Z = c0 B ; Y = c1 (2B + c0 A B)   (chebyshevIteration.hpp:79-90)
+template <typename T>
+struct OpChebFirst : Gate {
+    const T* __restrict__ B;
+    T* __restrict__ Z;
+    T* __restrict__ Y;
+    T c0, c1;
+    __device__ void load() {}
+    __device__ T in(long long c) const { return B[c]; }
+    __device__ void out(long long c, T vc, T Av, double*) const
+    {
+        Z[c] = c0 * vc;
+        Y[c] = c1 * (T(2) * vc + c0 * Av);
+    }
+    __device__ void finish(const double*) {}
+};
+
+template <typename T>
+void launch_cheb_first(const Geo& g, hipStream_t s, const T* B, T* Z, T* Y, double c0, double c1, const int* gate)
+{
+    OpChebFirst<T> op;
+    op.gate = gate;
+    op.B = B;
+    op.Z = Z;
+    op.Y = Y;
+    op.c0 = (T)c0;
+    op.c1 = (T)c1;
+    run_stencil<T, 0>(g, s, op, kNoRed);
+}
+
+template <typename T, typename TO>
+struct OpChebFirstOut : Gate {
+    const T* __restrict__ B;
+    TO* __restrict__ X;
+    T c0, c1;
+    int which;
+    double sign;
+    __device__ void load() {}
+    __device__ T in(long long c) const { return B[c]; }
+    __device__ void out(long long c, T vc, T Av, double*) const
+    {
+        const T y = which == 0 ? c0 * vc : c1 * (T(2) * vc + c0 * Av);
+        X[c] = (TO)(sign * (double)y);
+    }
+    __device__ void finish(const double*) {}
+};
+
+template <typename T, typename TO>
+void launch_cheb_first_out(const Geo& g, hipStream_t s, const T* B, TO* X, double c0, double c1, int which, double sign,
+                           const int* gate)
+{
+    OpChebFirstOut<T, TO> op;
+    op.gate = gate;
+    op.B = B;
+    op.X = X;
+    op.c0 = (T)c0;
+    op.c1 = (T)c1;
+    op.which = which;
+    op.sign = sign;
+    run_stencil<T, 0>(g, s, op, kNoRed);
+}
+
+// W = rho (2 sigma Y + 2/delta (B + A Y) - rho_old Z)   (chebyshevIteration.hpp:103-113)
+template <typename T, typename TO>
+struct OpChebSweep : Gate {
+    const T* __restrict__ Y;
+    const T* __restrict__ B;
+    const T* __restrict__ Z;
+    TO* __restrict__ W;
+    T a, b, cz;
+    double sign;
+    __device__ void load() {}
+    __device__ T in(long long c) const { return Y[c]; }
+    __device__ void out(long long c, T vc, T Av, double*) const
+    {
+        const T w = a * vc + b * (B[c] + Av) - cz * Z[c];
+        if constexpr (sizeof(TO) == sizeof(T)) W[c] = sign < 0 ? -w : w;
+        else W[c] = (TO)(sign * (double)w);
+    }
+    __device__ void finish(const double*) {}
+};
+
+template <typename T, typename TO>
+void launch_cheb_sweep(const Geo& g, hipStream_t s, const T* Y, const T* B, const T* Z, TO* W, double a, double b,
+                       double c, double sign, const int* gate)
+{
+    OpChebSweep<T, TO> op;
+    op.gate = gate;
+    op.Y = Y;
+    op.B = B;
+    op.Z = Z;
+    op.W = W;
+    op.a = (T)a;
+    op.b = (T)b;
+    op.cz = (T)c;
+    op.sign = sign;
+    run_stencil<T, 0>(g, s, op, kNoRed);
+}
+
+// ------------------------------------------------------------------ FCG
+
+template <typename T>
+struct OpFcgGamma : Gate {
+    const T* __restrict__ z;
+    const T* __restrict__ Apold;
+    Scal* S;
+    __device__ void load() {}
+    __device__ void point(long long c, int, int, int, double* acc) const { acc[0] += (double)z[c] * (double)Apold[c]; }
+    __device__ void finish(const double* t)
+    {
+        S->v[S_GAM] = t[0];
+        S->v[S_BETA] = S->iter == 0 ? 0.0 : t[0] / S->v[S_PAP_OLD];
+    }
+};
+
+template <typename T>
+void launch_fcg_gamma(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* Apold, Scal* S)
+{
+    OpFcgGamma<T> op;
+    op.gate = &S->done;
+    op.z = z;
+    op.Apold = Apold;
+    op.S = S;
+    run_vector<1>(g, s, op, rb);
+}
+
+// p = z - beta pold ; Ap = A p ; p.Ap, p.r ; alpha = p.r / p.Ap
+template <typename T>
+struct OpFcgP : Gate {
+    const T* __restrict__ z;
+    const T* __restrict__ pold;
+    T* __restrict__ p;
+    T* __restrict__ Ap;
+    const T* __restrict__ r;
+    Scal* S;
+    T beta;
+    __device__ void load() { beta = (T)S->v[S_BETA]; }
+    __device__ T in(long long c) const { return z[c] - beta * pold[c]; }
+    __device__ void out(long long c, T vc, T Av, double* acc) const
+    {
+        p[c] = vc;
+        Ap[c] = Av;
+        acc[0] += (double)vc * (double)Av;
+        acc[1] += (double)vc * (double)r[c];
+    }
+    __device__ void finish(const double* t)
+    {
+        S->v[S_PAP] = t[0];
+        S->v[S_ALPHA] = t[1] / t[0];
+    }
+};
+
+template <typename T>
+void launch_fcg_p(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* pold, T* p, T* Ap, const T* r, Scal* S)
+{
+    OpFcgP<T> op;
+    op.gate = &S->done;
+    op.z = z;
+    op.pold = pold;
+    op.p = p;
+    op.Ap = Ap;
+    op.r = r;
+    op.S = S;
+    run_stencil<T, 2>(g, s, op, rb);
+}
+
+template <typename T>
+struct OpFcgUpdate : Gate {
+    T* __restrict__ x;
+    const T* __restrict__ p;
+    T* __restrict__ r;
+    const T* __restrict__ Ap;
+    Scal* S;
+    bool history;
+    T alpha;
+    __device__ void load() { alpha = (T)S->v[S_ALPHA]; }
+    __device__ void point(long long c, int, int, int, double* acc) const
+    {
+        x[c] = x[c] + alpha * p[c];
+        const T rv = r[c] - alpha * Ap[c];
+        r[c] = rv;
+        acc[0] += (double)rv * (double)rv;
+    }
+    __device__ void finish(const double* t)
+    {
+        S->v[S_PAP_OLD] = S->v[S_PAP];
+        S->v[S_RR] = t[0];
+        finish_iteration(S, sqrt(t[0]), history);
+    }
+};
+
+template <typename T>
+void launch_fcg_update(const Geo& g, hipStream_t s, RedBuf rb, T* x, const T* p, T* r, const T* Ap, Scal* S, bool history)
+{
+    OpFcgUpdate<T> op;
+    op.gate = &S->done;
+    op.x = x;
+    op.p = p;
+    op.r = r;
+    op.Ap = Ap;
+    op.S = S;
+    op.history = history;
+    run_vector<1>(g, s, op, rb);
+}
+
+// --------------------------------------------------- iterative refinement
+
+struct OpIrRes : Gate {
+    const double* __restrict__ x;
+    const double* __restrict__ b;
+    float* __restrict__ r32;
+    Scal* So;
+    Scal* SI;
+    __device__ void load() {}
+    __device__ double in(long long c) const { return x[c]; }
+    __device__ void out(long long c, double, double Av, double* acc) const
+    {
+        const double rv = b[c] - Av;
+        const float rf = (float)rv;
+        r32[c] = rf;
+        acc[0] += rv * rv;
+        acc[1] += (double)rf * (double)rf;
+    }
+    __device__ void finish(const double* t)
+    {
+        const double e = sqrt(t[0]);
+        So->v[S_ERR] = e;
+        So->v[S_ERR_OP] = e;
+        const int o = So->outer;
+        if (So->hist != nullptr && o < So->hist_cap) So->hist[o] = e;
+        if (o == 0) So->v[S_ERR0] = e;
+        So->done = e < So->v[S_TOL] ? 1 : 0;
+        SI->v[S_INNER_NORM] = sqrt(t[1]);
+    }
+};
+
+void launch_ir_residual(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, float* r32, Scal* So,
+                        Scal* SI)
+{
+    OpIrRes op;
+    op.gate = nullptr;
+    op.x = x;
+    op.b = b;
+    op.r32 = r32;
+    op.So = So;
+    op.SI = SI;
+    run_stencil<double, 2>(g, s, op, rb);
+}
+
+struct OpIrUpdate : Gate {
+    double* __restrict__ x;
+    const float* __restrict__ d;
+    const Scal* SI;
+    float sc;
+    __device__ void load() { sc = (float)SI->v[S_INNER_NORM]; }
+    __device__ void point(long long c, int, int, int, double*) const { x[c] = x[c] + (double)(d[c] * sc); }
+    __device__ void finish(const double*) {}
+};
+
+void launch_ir_update(const Geo& g, hipStream_t s, double* x, const float* d, const Scal* SI)
+{
+    OpIrUpdate op;
+    op.gate = nullptr;
+    op.x = x;
+    op.d = d;
+    op.SI = SI;
+    run_vector<0>(g, s, op, kNoRed);
+}
+
+// inner solve start: the non-main normalisation of BaseCG (x = 0, b /= ||b||,
+// r0 = b) applied to r32 in place
+struct OpIrInit : Gate {
+    float* __restrict__ r;
+    float* __restrict__ x;
+    float* __restrict__ p;
+    Scal* SI;
+    bool pointwise;
+    float inv, jac;
+    __device__ void load()
+    {
+        inv = (float)(1.0 / SI->v[S_INNER_NORM]);
+        jac = (float)SI->v[S_JAC];
+    }
+    __device__ void point(long long c, int, int, int, double* acc) const
+    {
+        const float rv = r[c] * inv;
+        r[c] = rv;
+        x[c] = 0.0f;
+        p[c] = 0.0f;
+        acc[0] += (double)rv * (double)rv;
+        acc[1] += (double)rv * (double)(jac * rv);
+    }
+    __device__ void finish(const double* t)
+    {
+        const double e = sqrt(t[0]);
+        SI->v[S_ERR0] = e;
+        SI->v[S_ERR] = e;
+        SI->v[S_RZ] = t[1];
+        SI->v[S_BETA] = 0.0;
+        SI->v[S_PAP_OLD] = 1.0;
+        SI->iter = 0;
+        SI->done = e < SI->v[S_TOL] ? 2 : 0;
+    }
+};
+
+void launch_ir_inner_init(const Geo& g, hipStream_t s, RedBuf rb, float* r, float* x, float* p, Scal* SI, bool pointwise)
+{
+    OpIrInit op;
+    op.gate = nullptr;
+    op.r = r;
+    op.x = x;
+    op.p = p;
+    op.SI = SI;
+    op.pointwise = pointwise;
+    run_vector<2>(g, s, op, rb);
+}
+
+// ------------------------------------------------------ instantiations
+
+#define PPS_INST_T(T)                                                                                          \
+    template void launch_apply_A<T>(const Geo&, hipStream_t, const T*, T*);                                    \
+    template void launch_copy<T>(const Geo&, hipStream_t, const T*, T*);                                       \
+    template void launch_dot<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, double*, const int*);      \
+    template void launch_scale_box<T>(const Geo&, hipStream_t, T*, const Scal*, int);                          \
+    template void launch_fill<T>(const Geo&, hipStream_t, T*, T);                                              \
+    template void launch_ghost<T>(hipStream_t, const Layout&, const Faces&, T*, int, const Scal*, const int*);  \
+    template void launch_cg_A2<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, T*, T*, Scal*, bool);   \
+    template void launch_cg_B<T>(const Geo&, hipStream_t, RedBuf, T*, const T*, T*, const T*, Scal*, bool, bool); \
+    template void launch_cg_rz<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, Scal*);                 \
+    template void launch_pointwise<T>(const Geo&, hipStream_t, const T*, T*, double, const int*);              \
+    template void launch_bicg_k1<T>(const Geo&, hipStream_t, RedBuf, const T*, double, T*, const T*, Scal*);   \
+    template void launch_bicg_k2<T>(const Geo&, hipStream_t, T*, const T*, const Scal*);                       \
+    template void launch_bicg_k3<T>(const Geo&, hipStream_t, RedBuf, const T*, double, T*, const T*, Scal*);   \
+    template void launch_bicg_k45<T>(const Geo&, hipStream_t, RedBuf, T*, const T*, double, const T*, double, T*, \
+                                     const T*, const T*, Scal*, bool);                                         \
+    template void launch_bicg_k6<T>(const Geo&, hipStream_t, T*, const T*, const T*, const Scal*);             \
+    template void launch_cheb_first<T>(const Geo&, hipStream_t, const T*, T*, T*, double, double, const int*); \
+    template void launch_fcg_gamma<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, Scal*);             \
+    template void launch_fcg_p<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, T*, T*, const T*, Scal*); \
+    template void launch_fcg_update<T>(const Geo&, hipStream_t, RedBuf, T*, const T*, T*, const T*, Scal*, bool);
+
+PPS_INST_T(double)
+PPS_INST_T(float)
+
+template void launch_cheb_sweep<double, double>(const Geo&, hipStream_t, const double*, const double*, const double*,
+                                                double*, double, double, double, double, const int*);
+template void launch_cheb_sweep<float, float>(const Geo&, hipStream_t, const float*, const float*, const float*, float*,
+                                              double, double, double, double, const int*);
+template void launch_cheb_sweep<float, double>(const Geo&, hipStream_t, const float*, const float*, const float*,
+                                               double*, double, double, double, double, const int*);
+template void launch_cheb_first_out<double, double>(const Geo&, hipStream_t, const double*, double*, double, double, int,
+                                                    double, const int*);
+template void launch_cheb_first_out<float, float>(const Geo&, hipStream_t, const float*, float*, double, double, int,
+                                                  double, const int*);
+template void launch_cheb_first_out<float, double>(const Geo&, hipStream_t, const float*, double*, double, double, int,
+                                                   double, const int*);
+template void launch_cast<double, float>(const Geo&, hipStream_t, const double*, float*, const int*);
+template void launch_cast<float, double>(const Geo&, hipStream_t, const float*, double*, const int*);
+
+}  // namespace pps

@@ -1,0 +1,207 @@
+// pps_kernels.hpp -- device data layout, per-solver device scalars and the
+// launchers of the Krylov-path kernels (implemented in pps_kernels.hip).
+//
+// Data layout in HBM.  A field of the reference's dense guarded box
+// (n0+2)(n1+2)(n2+2) (iterativeSolverBase.hpp:34-35) is stored pitched:
+// element (i,j,k) of the guarded index space lives at
+//      ox + i + px*j + plane*k,   plane = px*(n1+2)
+// where ox puts the first solver-region column on a 128-byte boundary and px
+// (a multiple of 128 bytes) keeps every row aligned, so each 64-lane
+// wavefront reads whole cache lines of one x-row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pps {
+
+constexpr int kTX = 64;   // tile width in x: one wavefront per row
+constexpr int kTY = 4;    // rows per tile: 4 wavefronts per block
+constexpr int kThreads = kTX * kTY;
+constexpr int kMaxRedQ = 4;
+constexpr int kMinKz = 4;   // smallest z-chunk; sizes the reduction workspace
+
+struct Layout {
+    int ng[3];        // guarded extent per direction
+    int px;           // row pitch (elements)
+    int ox;           // column offset of guarded index 0
+    long long plane;  // px * ng[1]
+    long long total;  // elements allocated
+    __host__ __device__ long long at(int i, int j, int k) const { return ox + i + (long long)px * j + plane * k; }
+};
+
+// [lo, hi) guarded index ranges
+struct Region {
+    int lo[3], hi[3];
+    long long count() const { return (long long)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]); }
+};
+
+// Scalars of one Krylov solver instance, kept in device memory so that no
+// iteration needs a host round trip; kernels read them at entry and the last
+// block of each reduction writes them.
+enum ScalIdx {
+    S_ALPHA = 0, S_BETA, S_OMEGA, S_RHO0, S_RZ, S_PAP, S_PAP_OLD, S_ERR, S_ERR0, S_NORM,
+    S_TOL, S_JAC, S_GAM, S_ERR_OP, S_INNER_NORM, S_DSCALE, S_RR, S_AUX0, S_AUX1, S_AUX2,
+    S_COUNT
+};
+
+struct alignas(256) Scal {
+    double v[S_COUNT];
+    int iter;        // completed iterations
+    int done;        // 0 running, 1 converged or capped, 2 early exit before the loop
+    int maxit;
+    int hist_cap;    // capacity of hist (entries), 0 = no history
+    double* hist;    // residual history (errorFromIterationHistory_)
+    int outer;       // IR outer steps
+    int pad[3];
+};
+
+// Single-pass deterministic grid reduction workspace: per-block partial sums
+// and an arrival counter; the last block to arrive sums the partials in a
+// fixed order and runs the solver's scalar epilogue.
+struct RedBuf {
+    double* partials;   // kMaxRedQ * cap
+    unsigned* counter;
+    int cap;            // max blocks per launch
+};
+
+// Grid geometry handed to a launch: region to process and tiling.
+struct Geo {
+    long long plane;
+    int px, ox;
+    int lo0, hi0, lo1, hi1, lo2, hi2;
+    int ntx, nty, kz;
+    int nblocks;
+    double cx, cy, cz;  // 1/ds^2 per direction
+};
+
+Geo make_geo(const Layout& L, const Region& R, const double coef[3], int kz);
+
+// Neumann faces of the current grid, for the ghost-fill kernel.
+struct Faces {
+    int n;             // number of Neumann faces
+    int face[6];       // face ids (x- x+ y- y+ z- z+)
+    const double* flux[6];  // du/dn planes (device) or null
+    int lim_data[6];
+    double ds[3];
+    int fng[6][2];     // plane extents of each face (guarded)
+};
+
+// ---------------------------------------------------------------- launchers
+// Every loop kernel takes `gate`: when *gate != 0 the whole grid returns at
+// once (the solver has converged), so iterations can be queued ahead of the
+// host's convergence check without changing the iterate sequence.
+
+// y = A x on the solver region
+template <typename T>
+void launch_apply_A(const Geo& g, hipStream_t s, const T* x, T* y);
+// copy of the solver region (bandwidth reference)
+template <typename T>
+void launch_copy(const Geo& g, hipStream_t s, const T* x, T* y);
+// sum a.b over the region into *out (device)
+template <typename T>
+void launch_dot(const Geo& g, hipStream_t s, RedBuf rb, const T* a, const T* b, double* out, const int* gate);
+
+// r = b - A x; S[S_ERR0 or S_ERR_OP] = ||r||, S_RZ = r.(jac r); mode 0: start of solve
+// (history[0], iteration reset, early-exit flag), mode 1: final true residual
+void launch_residual(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, double* r, Scal* S, int mode);
+
+// ||b_adj||^2 of the boundary-adjusted rhs over the solver region -> S_NORM = sqrt
+void launch_bnorm(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, const int bc[6],
+                  const int lim_data[6], const int lim_solver[6], const double* const flux[6], const int fng[6][2],
+                  const double ds[3], Scal* S);
+// b <- adjusted b over the data region, in place (reads the Dirichlet values of x
+// one layer outside the adjusted point; in-place is safe because every adjusted
+// point only reads x and its own b)
+void adjust_b_field(const Geo& gdata, hipStream_t s, const double* x, double* b, const int bc[6], const int lim_data[6],
+                    const double* const flux[6], const int fng[6][2], const double ds[3]);
+// field over the whole guarded box: mode 0 divide by S_NORM, mode 1 multiply by S_NORM
+template <typename T>
+void launch_scale_box(const Geo& gbox, hipStream_t s, T* a, const Scal* S, int mode);
+// set a field to a constant over a region
+template <typename T>
+void launch_fill(const Geo& g, hipStream_t s, T* a, T v);
+// Neumann ghost fill; with flux: data-field form using S_NORM of `S` (or 1 if S null)
+template <typename T>
+void launch_ghost(hipStream_t s, const Layout& L, const Faces& F, T* field, int with_flux, const Scal* S,
+                  const int* gate);
+
+// ---- CG (plain / pointwise-preconditioned / buffer-preconditioned) ----
+// p_new = a*src + beta*p_old (a = S_JAC or 1), Ap = A p_new, pAp -> alpha = S_RZ / pAp.
+// pin/pout must be distinct buffers (halo tiles rebuild p_new from p_old).
+template <typename T>
+void launch_cg_A2(const Geo& g, hipStream_t s, RedBuf rb, const T* src, const T* pin, T* pout, T* Ap, Scal* S,
+                  bool pointwise);
+// x += alpha p, r -= alpha Ap, rr (and r.(jac r) when pointwise) -> err, beta, iter, done
+template <typename T>
+void launch_cg_B(const Geo& g, hipStream_t s, RedBuf rb, T* x, const T* p, T* r, const T* Ap, Scal* S,
+                 bool pointwise, bool history);
+// rz_new = r.z -> beta = rz_new / rz (0 at iteration 0), rz = rz_new (buffer-preconditioned CG)
+template <typename T>
+void launch_cg_rz(const Geo& g, hipStream_t s, RedBuf rb, const T* r, const T* z, Scal* S);
+
+// ---- pointwise preconditioners: out = c * in over the whole box ----
+template <typename T>
+void launch_pointwise(const Geo& gbox, hipStream_t s, const T* in, T* out, double c, const int* gate);
+
+// ---- BiCGSTAB (BiCGSTAB.hpp:131-292) ----
+template <typename T>
+void launch_bicg_k1(const Geo& g, hipStream_t s, RedBuf rb, const T* Mp, double amp, T* AMp, const T* r0, Scal* S);
+template <typename T>
+void launch_bicg_k2(const Geo& g, hipStream_t s, T* r, const T* AMp, const Scal* S);
+template <typename T>
+void launch_bicg_k3(const Geo& g, hipStream_t s, RedBuf rb, const T* z, double az, T* Az, const T* r, Scal* S);
+template <typename T>
+void launch_bicg_k45(const Geo& g, hipStream_t s, RedBuf rb, T* x, const T* Mp, double amp, const T* z, double az,
+                     T* r, const T* Az, const T* r0, Scal* S, bool history);
+template <typename T>
+void launch_bicg_k6(const Geo& g, hipStream_t s, T* p, const T* r, const T* AMp, const Scal* S);
+
+// ---- Chebyshev (chebyshevIteration.hpp:78-128) ----
+// coefficients for each launch are computed on the host (they depend only on
+// the grid's eigenvalue bounds): first: Z = c0*B, Y = c1*(2B + c0*AB)
+template <typename T>
+void launch_cheb_first(const Geo& g, hipStream_t s, const T* B, T* Z, T* Y, double c0, double c1, const int* gate);
+// W = a*Y + b*(B + AY) - c*Z  (output type TO may differ; sign -1 writes -W)
+template <typename T, typename TO>
+void launch_cheb_sweep(const Geo& g, hipStream_t s, const T* Y, const T* B, const T* Z, TO* W, double a, double b,
+                       double c, double sign, const int* gate);
+// X = sign*c0*B (degree-0/1 outputs) and X = sign*c1*(2B + c0*AB)
+template <typename T, typename TO>
+void launch_cheb_first_out(const Geo& g, hipStream_t s, const T* B, TO* X, double c0, double c1, int which,
+                           double sign, const int* gate);
+// dst = (TD) src over the whole box
+template <typename TS, typename TD>
+void launch_cast(const Geo& gbox, hipStream_t s, const TS* src, TD* dst, const int* gate);
+
+// ---- FCG (truncation 1) ----
+template <typename T>
+void launch_fcg_gamma(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* Apold, Scal* S);
+template <typename T>
+void launch_fcg_p(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* pold, T* p, T* Ap, const T* r, Scal* S);
+template <typename T>
+void launch_fcg_update(const Geo& g, hipStream_t s, RedBuf rb, T* x, const T* p, T* r, const T* Ap, Scal* S, bool history);
+
+// ---- mixed-precision iterative refinement ----
+// r = b - A x (FP64), r32 = (float) r; So: S_ERR = ||r||, history[outer];
+// SI: S_INNER_NORM = ||r32||
+void launch_ir_residual(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, float* r32, Scal* So,
+                        Scal* SI);
+// x += ||r32|| * d on the solver region (the inner solve's un-normalisation)
+void launch_ir_update(const Geo& g, hipStream_t s, double* x, const float* d, const Scal* SI);
+// inner start: r = r32 / ||r32||, x = 0, p = 0, err0, rz (pointwise jac) -> SI
+void launch_ir_inner_init(const Geo& g, hipStream_t s, RedBuf rb, float* r, float* x, float* p, Scal* SI, bool pointwise);
+
+}  // namespace pps
+
+namespace pps {
+// ---- seeded right-hand sides (pps_rhs.hip) ----
+void launch_rhs_normal(hipStream_t s, const Layout& L, const Region& R, const int np[3], uint64_t seed, double* b,
+                       double* scratch);
+void launch_rhs_gauss(hipStream_t s, const Layout& L, const Region& R, const double* src_dev, int nsrc,
+                      const double origin[3], const double ds[3], double* b);
+void launch_rhs_spectrum(hipStream_t s, const Layout& L, const Region& R, const int np[3], int K, const double* C,
+                         const double* Sx, const double* Sy, const double* Sz, double* T1, double* T2, int accumulate,
+                         double* b);
+}  // namespace pps
