@@ -249,8 +249,7 @@ class Solver:
         self.spec = spec
         self.blockGrid = blockGrid
         self.exact = exact
-        g = blockGrid
-        self.ctx = _lib.Context(g.getNpglobal(), g.getDs(), g.getOrigin(), g.getBcsType())
+        self._ctx = None
         self.numIterationFinal_ = 0
         self.numIterationPreconditionerFinal_ = 0
         self.errorFromIteration_ = -1.0
@@ -259,6 +258,14 @@ class Solver:
         self.normFieldB_ = 1.0
         self.errorFromIterationHistory_ = np.zeros(0)
         self.outerIterations_ = 0
+
+    @property
+    def ctx(self):
+        """Device context, created on first use (the solve needs the GPU; setProblem does not)."""
+        if self._ctx is None:
+            g = self.blockGrid
+            self._ctx = _lib.Context(g.getNpglobal(), g.getDs(), g.getOrigin(), g.getBcsType())
+        return self._ctx
 
     # iterativeSolverBase.hpp:51-55,537-603
     def setProblem(self, fieldX, fieldB):
