@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Benchmark: seconds to FP64 relative residual 1e-10 for the 3-D Poisson solve
+at 512^3 interior unknowns, on one MI355X.
+
+Workload (the mixed-precision configuration of the Krylov path): unit cube,
+7-point FD, homogeneous Dirichlet, f = seeded sum of 64 Gaussian sources
+(centres U[0,1]^3, widths U[0.01,0.1], amplitudes N(0,1)), x0 = 0, stop when
+the recursive FP64 relative residual drops below 1e-10 (baseCG.hpp:188,224).
+
+A step is one complete solve with the right-hand side resident in HBM: reset
+x = 0, restore b, normalise, iterate, true FP64 residual, un-normalise.  Each
+form is timed over K steps after W warm-up steps, bracketed by device
+synchronisation; `value` is the headline form's seconds per solve:
+  - mixed FP32/FP64 iterative refinement (FP32 CG inner to 1e-5, FP64 outer
+    residual and update), compared with
+  - plain FP64 CG (the reference method), reported beside it.
+The roofline entry is the dominant kernel of the headline solve, timed live
+with HIP events on the context's stream; the CPU baseline is the reference
+CPU solver (oracle/_ref, compiled from the reference sources) under MPI on the
+host's cores, timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_INTERIOR = 512
+SEED = 3
+NSRC = 64
+TOL = 1e-10
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def chains():
+    import parallelpoissonsolver_amd._lib as L
+    plain = [dict(kind=L.CG, tol=TOL, maxit=200000), dict(kind=L.NONE)]
+    mixed = [dict(kind=L.IR, tol=TOL, maxit=100), dict(kind=L.CG, precision=32, tol=1e-5, maxit=200000),
+             dict(kind=L.NONE, precision=32)]
+    return {"plain_fp64_cg": plain, "ir_fp32cg_fp64": mixed}
+
+
+def run_form(C, chain, steps, warmup, name):
+    import parallelpoissonsolver_amd._lib as L
+    res = None
+    for w in range(warmup):
+        C.fill(L.FIELD_X, 0.0)
+        C.copy_field(L.FIELD_B, L.FIELD_T0)
+        res = C.solve(chain)
+        log(f"{name}: warm-up {w + 1}/{warmup}: {res['iterations']} it, {res['seconds']:.3f} s device")
+    dev, wall = [], []
+    for s in range(steps):
+        C.fill(L.FIELD_X, 0.0)
+        C.copy_field(L.FIELD_B, L.FIELD_T0)
+        C.synchronize()
+        t0 = time.perf_counter()
+        res = C.solve(chain)
+        C.synchronize()
+        wall.append(time.perf_counter() - t0)
+        dev.append(res["seconds"])
+        log(f"{name}: step {s + 1}/{steps}: {res['iterations']} it (outer {res['outer_iterations']}), "
+            f"true rel residual {res['error_compute_operator']:.3e}, {wall[-1]:.3f} s")
+        if not res["error_compute_operator"] < TOL:
+            raise RuntimeError(f"{name} did not reach {TOL}: {res['error_compute_operator']}")
+    return {"s": sum(wall) / len(wall), "device_s": sum(dev) / len(dev), "iterations": res["iterations"],
+            "outer_iterations": res["outer_iterations"], "true_rel_residual": res["error_compute_operator"]}
+
+
+def roofline(C, kernels, reps):
+    out = {}
+    for name, kid in kernels.items():
+        ms, nbytes = C.time_kernel(kid, reps)
+        out[name] = {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9}
+        log(f"kernel {name}: {ms:.4f} ms/launch, {out[name]['gbs']:.0f} GB/s algorithmic")
+    return out
+
+
+def cpu_baseline(gpu_iterations, seconds_budget_hint):
+    """Reference CPU solver (compiled from the reference sources into oracle/_ref)
+    on the host cores: 10 CG iterations of the same 512^3 workload under MPI,
+    extrapolated to the iteration count the solve needs."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_solver_sample")
+    mpirun = shutil.which("mpirun") or "/opt/conda/bin/mpirun"
+    if not (os.path.exists(exe) and os.path.exists(mpirun)):
+        log("CPU baseline skipped: oracle/_ref/ref_solver_sample or mpirun missing")
+        return None
+    npg = N_INTERIOR + 2
+    ds = 1.0 / (npg - 1)
+    ranks = (2, 2, 2)  # 514 = 2 * 257: the reference needs ranks dividing npglobal (blockGrid.hpp:165)
+    nproc = ranks[0] * ranks[1] * ranks[2]
+    with tempfile.TemporaryDirectory() as tmp:
+        prefix = os.path.join(tmp, "cpu")
+        cmd = [mpirun, "-n", str(nproc), exe, "--ranks", *map(str, ranks), "--np", str(npg), str(npg), str(npg),
+               "--ds", repr(ds), repr(ds), repr(ds), "--problem", "2", "--rhs", "gauss", "--seed", str(SEED),
+               "--nsrc", str(NSRC), "--solver", "cg", "--precond", "none", "--tol", repr(TOL), "--out", prefix]
+        log("CPU baseline: " + " ".join(cmd[:3]) + " ... (reference CPU CG, 10 iterations)")
+        t0 = time.perf_counter()
+        p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        if p.returncode != 0:
+            log(f"CPU baseline failed: {p.stderr[-500:]}")
+            return None
+        meta = json.load(open(prefix + ".json"))
+    it = meta["iterations"]
+    per_it = meta["solver_in_function_time"] / it
+    setup = meta["solver_time"] - meta["solver_in_function_time"]
+    est = setup + per_it * gpu_iterations
+    log(f"CPU baseline: {it} iterations in {meta['solver_in_function_time']:.2f} s ({per_it:.3f} s/it), "
+        f"extrapolated {est:.1f} s for {gpu_iterations} iterations (run took {wall:.1f} s)")
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": est, "unit": "s", "cores": nproc, "kind": "reference",
+            "sample": f"reference solverPoissonMPI_CPU BaseCG+NoneSolver, 512^3 Gaussian-source rhs, {nproc} MPI ranks "
+                      f"(2x2x2), {it} iterations timed ({per_it:.3f} s/iteration) + setup, extrapolated to the "
+                      f"{gpu_iterations} iterations of the full solve",
+            "cpu_model": cpu_model}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=N_INTERIOR, help="interior points per direction")
+    ap.add_argument("--kz", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="plain_fp64_cg,ir_fp32cg_fp64")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {a.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import parallelpoissonsolver_amd._lib as L
+
+    npg = a.n + 2
+    ds = 1.0 / (npg - 1)
+    C = L.Context((npg,) * 3, (ds,) * 3, (0, 0, 0), (0,) * 6)
+    if a.kz:
+        C.set_tiling(a.kz)
+    C.fill(L.FIELD_X, 0.0)
+    C.fill(L.FIELD_B, 0.0)
+    C.rhs_gauss(SEED, NSRC, 0.01, 0.1)
+    C.copy_field(L.FIELD_T0, L.FIELD_B)
+    log(f"workload: {a.n}^3 interior, Gaussian sources (seed {SEED}, {NSRC} sources), tol {TOL}")
+
+    results = {}
+    forms = chains()
+    for name in a.forms.split(","):
+        results[name] = run_form(C, forms[name], a.steps, a.warmup, name)
+
+    headline = "ir_fp32cg_fp64" if "ir_fp32cg_fp64" in results else list(results)[0]
+    kern = roofline(C, {"cg_stencil_f64": L.K_CG_STENCIL, "cg_update_f64": L.K_CG_UPDATE,
+                        "cg_stencil_f32": L.K_CG_STENCIL_F32, "cg_update_f32": L.K_CG_UPDATE_F32,
+                        "residual_f64": L.K_RESIDUAL, "copy_f64": L.K_COPY}, reps=20)
+    # dominant kernel of the headline solve: the update kernel moves 6 words per point,
+    # the fused stencil 4; both run once per iteration
+    dom = "cg_update_f32" if headline == "ir_fp32cg_fp64" else "cg_update_f64"
+    k = kern[dom]
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(k["gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(k["gbs"] / PEAK_HBM_GBS, 4), "traffic": None,
+            "bytes_per_launch": k["bytes"], "ms_per_launch": k["ms"]}
+
+    cpu = None
+    if not a.no_cpu and a.n == N_INTERIOR:
+        try:
+            cpu = cpu_baseline(results["plain_fp64_cg"]["iterations"] if "plain_fp64_cg" in results
+                               else results[headline]["iterations"], 30)
+        except Exception as e:  # the baseline is reported, never the product
+            log(f"CPU baseline error: {e}")
+    C.close()
+
+    hv = results[headline]
+    out = {
+        "metric": "s to 1e-10 rel. residual at 512³: NN-FCG+FP32/FP64 vs plain FP64 CG; HBM GB/s",
+        "value": round(hv["s"], 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(hv["s"] * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if headline != "plain_fp64_cg" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"poisson3d_{a.n}^3_dirichlet_gauss{NSRC}", "n_interior": a.n, "rhs_seed": SEED,
+                   "solver": headline, "tol": TOL, "kz": a.kz or 32},
+        "forms": results,
+        "ratio_headline_vs_plain": (round(hv["s"] / results["plain_fp64_cg"]["s"], 4)
+                                    if "plain_fp64_cg" in results else None),
+        "kernels": {k2: {"gbs": round(v["gbs"], 1), "ms": round(v["ms"], 4)} for k2, v in kern.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

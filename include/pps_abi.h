@@ -117,6 +117,8 @@ int pps_get_extent(void* ctx, int ng[3]);
 int pps_upload(void* ctx, int field, const double* host_guarded);
 int pps_download(void* ctx, int field, double* host_guarded);
 int pps_fill(void* ctx, int field, double value);
+/* device-to-device copy of a whole field */
+int pps_copy_field(void* ctx, int dst, int src);
 /* du/dn data of a Neumann face (trueSolutionDdir on the face's data points);
  * plane of the two other directions' guarded extents, lower direction fastest */
 int pps_set_neumann_flux(void* ctx, int face, const double* host_plane);

@@ -82,6 +82,7 @@ SIGNATURES = {
     "pps_upload": ([_VP, ctypes.c_int, _DP], ctypes.c_int),
     "pps_download": ([_VP, ctypes.c_int, _DP], ctypes.c_int),
     "pps_fill": ([_VP, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+    "pps_copy_field": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "pps_set_neumann_flux": ([_VP, ctypes.c_int, _DP], ctypes.c_int),
     "pps_solve": ([_VP, ctypes.POINTER(CStage), ctypes.c_int, ctypes.POINTER(CResult)], ctypes.c_int),
     "pps_get_history": ([_VP, _DP, ctypes.c_int], ctypes.c_int),
@@ -177,6 +178,9 @@ class Context:
 
     def fill(self, field, v):
         self._chk(lib().pps_fill(self.h, field, float(v)))
+
+    def copy_field(self, dst, src):
+        self._chk(lib().pps_copy_field(self.h, dst, src))
 
     def set_flux(self, face, plane):
         plane = np.ascontiguousarray(plane, dtype=np.float64)

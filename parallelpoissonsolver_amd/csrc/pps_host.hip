@@ -681,8 +681,8 @@ struct IRStage : KrylovStage {
         const bool pw = (M == nullptr) || M->pointwise(&c);
         const bool flexible = in->cfg.kind == PPS_FCG;
         if ((!pw || flexible) && (rc = C.get_work(k + ".z", 32, z))) return rc;
-        rc = scal_init(C, SI, in->cfg.tol, in->cfg.max_iter, pw ? c : 1.0, nullptr, 0);
-        if (rc) return rc;
+        // SI was initialised by solve_main; the outer residual kernel has just
+        // written SI[S_INNER_NORM], and ir_inner_init resets iter/done
         const Geo g = C.gsolver(32);
         Field p[2] = {p0, p1}, Ap[2] = {A0, A1};
         launch_ir_inner_init(g, C.stream, C.rb, r32.f(), d.f(), p0.f(), SI, pw);
@@ -747,6 +747,10 @@ struct IRStage : KrylovStage {
         rc = scal_init(C, S, cfg.tol, cfg.max_iter, 1.0, C.hist_d, C.hist_cap);
         if (rc) return rc;
         Scal* SI = pre->S;
+        double c = 1.0;
+        const bool pw = (pre->pre == nullptr) || pre->pre->pointwise(&c);
+        rc = scal_init(C, SI, pre->cfg.tol, pre->cfg.max_iter, pw ? c : 1.0, nullptr, 0);
+        if (rc) return rc;
         C.ghost(x, true, S, nullptr);
         normalize(C, S, true, x, b);
         scale_box(C, x, S, 0);
@@ -1088,6 +1092,19 @@ int pps_fill(void* ctx, int field, double value)
     if (rc) return rc;
     launch_fill<double>(C->gbox(64), C->stream, f.d(), value);
     return C->hip(hipStreamSynchronize(C->stream), "fill");
+}
+
+int pps_copy_field(void* ctx, int dst, int src)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    Field fd, fs;
+    int rc = field_of(*C, dst, fd);
+    if (!rc) rc = field_of(*C, src, fs);
+    if (rc) return rc;
+    rc = C->hip(hipMemcpyAsync(fd.p, fs.p, sizeof(double) * C->L64.total, hipMemcpyDeviceToDevice, C->stream), "copy");
+    if (rc) return rc;
+    return C->hip(hipStreamSynchronize(C->stream), "sync");
 }
 
 int pps_set_neumann_flux(void* ctx, int face, const double* host)
