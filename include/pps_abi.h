@@ -163,8 +163,8 @@ enum {
  * the context stream, measured with HIP events; *bytes = algorithmic bytes
  * moved per launch. */
 int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes);
-/* tiling knobs of the stencil kernels (0 keeps the default) */
-int pps_set_tiling(void* ctx, int kz, int ty);
+/* z-chunk lengths of the stencil and vector kernels (0 keeps the default) */
+int pps_set_tiling(void* ctx, int kz_stencil, int kz_vector);
 
 #ifdef __cplusplus
 }

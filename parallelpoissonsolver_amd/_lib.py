@@ -244,8 +244,8 @@ class Context:
         self._chk(lib().pps_time_kernel(self.h, kernel, reps, ctypes.byref(ms), ctypes.byref(nbytes)))
         return ms.value, nbytes.value
 
-    def set_tiling(self, kz=0, ty=0):
-        self._chk(lib().pps_set_tiling(self.h, kz, ty))
+    def set_tiling(self, kz_stencil=0, kz_vector=0):
+        self._chk(lib().pps_set_tiling(self.h, kz_stencil, kz_vector))
 
     def synchronize(self):
         self._chk(lib().pps_synchronize(self.h))

@@ -86,7 +86,9 @@ struct Ctx {
     double* hist_d = nullptr;
     int hist_cap = 0;
     int hist_len = 0;
-    int kz = 32;
+    int kz_st = 16;   // z-chunk of stencil launches
+    int kz_vec = 32;  // z-chunk of vector launches
+    int nt = 0;       // non-temporal stores (set when a field exceeds the caches)
     std::string err;
     std::unique_ptr<NNPrecond> nn;
 
@@ -106,7 +108,7 @@ struct Ctx {
         return fail(e == hipErrorOutOfMemory ? PPS_ERR_NOMEM : PPS_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
     }
     const Layout& lay(int prec) const { return prec == 32 ? L32 : L64; }
-    Geo geo(const Region& R, int prec) const { return make_geo(lay(prec), R, coef, kz); }
+    Geo geo(const Region& R, int prec) const { return make_geo(lay(prec), R, coef, kz_st, kz_vec, nt); }
     Geo gsolver(int prec) const { return geo(solver, prec); }
     Geo gbox(int prec) const { return geo(box, prec); }
     Geo gdata(int prec) const { return geo(data, prec); }
@@ -487,9 +489,11 @@ struct CGStage : KrylovStage {
         Field p[2] = {p0, p1};
         int cur = 0;
         rc = run_loop(C, S, cfg.max_iter, pre_host(), [&]() -> int {
-            launch_cg_A2<double>(g, C.stream, C.rb, pw ? r.d() : z.d(), p[cur].d(), p[cur ^ 1].d(), Ap.d(), S, pw);
+            // iteration k reads p[k&1], writes p[(k+1)&1], and applies
+            // iteration k-1's x update on the way
+            launch_cg_A2<double>(g, C.stream, C.rb, pw ? r.d() : z.d(), p[cur].d(), p[cur ^ 1].d(), Ap.d(), x.d(), S, pw);
             cur ^= 1;
-            launch_cg_B<double>(g, C.stream, C.rb, x.d(), p[cur].d(), r.d(), Ap.d(), S, pw, main);
+            launch_cg_B<double>(g, C.stream, C.rb, r.d(), Ap.d(), S, pw, main);
             if (!pw) {
                 int e = pre_apply(z, r, &S->done);
                 if (e) return e;
@@ -498,6 +502,11 @@ struct CGStage : KrylovStage {
             return PPS_OK;
         });
         if (rc) return rc;
+        // the last executed iteration K-1 left x += alpha_{K-1} p_{K-1} pending; p_{K-1} is p[K&1]
+        Scal h;
+        rc = scal_read(C, S, h);
+        if (rc) return rc;
+        if (h.iter > 0) launch_cg_final_x<double>(g, C.stream, x.d(), p[h.iter & 1].d(), S);
         epilogue(main, x, b, r);
         return PPS_OK;
     }
@@ -711,9 +720,10 @@ struct IRStage : KrylovStage {
                 launch_fcg_update<float>(g, C.stream, C.rb, d.f(), p[cur].f(), r32.f(), Ap[cur].f(), SI, false);
                 cur ^= 1;
             } else {
-                launch_cg_A2<float>(g, C.stream, C.rb, pw ? r32.f() : z.f(), p[cur].f(), p[cur ^ 1].f(), Ap[0].f(), SI, pw);
+                launch_cg_A2<float>(g, C.stream, C.rb, pw ? r32.f() : z.f(), p[cur].f(), p[cur ^ 1].f(), Ap[0].f(), d.f(),
+                                    SI, pw);
                 cur ^= 1;
-                launch_cg_B<float>(g, C.stream, C.rb, d.f(), p[cur].f(), r32.f(), Ap[0].f(), SI, pw, false);
+                launch_cg_B<float>(g, C.stream, C.rb, r32.f(), Ap[0].f(), SI, pw, false);
                 if (!pw) {
                     int e = applyM(&SI->done);
                     if (e) return e;
@@ -726,6 +736,7 @@ struct IRStage : KrylovStage {
         Scal h;
         rc = scal_read(C, SI, h);
         if (rc) return rc;
+        if (!flexible && h.iter > 0) launch_cg_final_x<float>(g, C.stream, d.f(), p[h.iter & 1].f(), SI);
         inner_total += h.iter;
         in->prec_iters += h.iter;
         in->calls++;
@@ -939,14 +950,18 @@ int pps_create(const pps_grid* grid, void** out)
     for (int prec : {64, 32}) {
         Layout& L = prec == 32 ? C->L32 : C->L64;
         // One element layout for every precision: FP32 and FP64 fields share
-        // offsets (mixed-precision kernels index both with one offset); rows are
-        // 32 elements aligned = 128 B (FP32) / 256 B (FP64).
+        // offsets (mixed-precision kernels index both with one offset).  The row
+        // pitch is the guarded width rounded up to 32 elements and the first
+        // solver column sits on a 32-element boundary, so every row's interior
+        // starts 128-B (FP32) / 256-B (FP64) aligned; a row's trailing guard
+        // point may spill into the next row's leading pad columns, which no row
+        // uses (rows never overlap because px >= ng[0]).
         const int A = 32;
         for (int d = 0; d < 3; ++d) L.ng[d] = C->ng[d];
         L.ox = A - C->lim_solver[0];
-        L.px = ((L.ox + C->ng[0] + A - 1) / A) * A;
+        L.px = ((C->ng[0] + A - 1) / A) * A;
         L.plane = (long long)L.px * C->ng[1];
-        L.total = L.plane * C->ng[2] + A;
+        L.total = L.plane * C->ng[2] + L.ox + C->ng[0] + A;
     }
     Ctx& X = *C;
     if (X.hip(hipStreamCreateWithFlags(&X.stream, hipStreamNonBlocking), "stream")) {
@@ -960,8 +975,10 @@ int pps_create(const pps_grid* grid, void** out)
     rc = rc ? rc : X.hip(hipEventCreate(&X.ev_t1), "event");
     // reduction workspace: enough partials for the largest launch
     // partials for the largest launch: the whole box at the smallest kz allowed
-    const Geo gbig = make_geo(X.L64, X.box, X.coef, kMinKz);
+    const Geo gbig = make_geo(X.L64, X.box, X.coef, kMinKz, kMinKz, 0);
     X.rb.cap = gbig.nblocks + 8;
+    // fields beyond the 256 MB last-level cache stream past the caches
+    X.nt = (X.L64.total * 8 > (256ll << 20)) ? 1 : 0;
     rc = rc ? rc : X.hip(hipMalloc(&X.rb.partials, sizeof(double) * kMaxRedQ * X.rb.cap), "partials");
     rc = rc ? rc : X.hip(hipMalloc(&X.rb.counter, 256), "counter");
     rc = rc ? rc : X.hip(hipMemsetAsync(X.rb.counter, 0, 256, X.stream), "counter");
@@ -1266,13 +1283,14 @@ int pps_norm_b_adjusted(void* ctx, double* out)
     return PPS_OK;
 }
 
-int pps_set_tiling(void* ctx, int kz, int ty)
+int pps_set_tiling(void* ctx, int kz_stencil, int kz_vector)
 {
     Ctx* C = as_ctx(ctx);
     if (!C) return PPS_ERR_ARG;
-    if (ty != 0 && ty != kTY) return C->fail(PPS_ERR_UNSUPPORTED, "tile height is fixed at %d rows", kTY);
-    if (kz != 0 && (kz < kMinKz || kz > 4096)) return C->fail(PPS_ERR_ARG, "kz must be in [%d, 4096]", kMinKz);
-    if (kz > 0) C->kz = kz;
+    for (int kz : {kz_stencil, kz_vector})
+        if (kz != 0 && (kz < kMinKz || kz > 4096)) return C->fail(PPS_ERR_ARG, "kz must be in [%d, 4096]", kMinKz);
+    if (kz_stencil > 0) C->kz_st = kz_stencil;
+    if (kz_vector > 0) C->kz_vec = kz_vector;
     return PPS_OK;
 }
 
@@ -1309,13 +1327,15 @@ int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes)
     auto launch = [&](int it) {
         switch (kernel) {
         case PPS_K_CG_STENCIL:
-            launch_cg_A2<double>(g, C->stream, C->rb, a.d(), (it & 1) ? c.d() : b.d(), (it & 1) ? b.d() : c.d(), d.d(), S, true);
+            launch_cg_A2<double>(g, C->stream, C->rb, a.d(), (it & 1) ? c.d() : b.d(), (it & 1) ? b.d() : c.d(), d.d(),
+                                 e.d(), S, true);
             break;
         case PPS_K_CG_STENCIL_F32:
-            launch_cg_A2<float>(g, C->stream, C->rb, a.f(), (it & 1) ? c.f() : b.f(), (it & 1) ? b.f() : c.f(), d.f(), S, true);
+            launch_cg_A2<float>(g, C->stream, C->rb, a.f(), (it & 1) ? c.f() : b.f(), (it & 1) ? b.f() : c.f(), d.f(),
+                                e.f(), S, true);
             break;
-        case PPS_K_CG_UPDATE: launch_cg_B<double>(g, C->stream, C->rb, e.d(), b.d(), a.d(), c.d(), S, true, false); break;
-        case PPS_K_CG_UPDATE_F32: launch_cg_B<float>(g, C->stream, C->rb, e.f(), b.f(), a.f(), c.f(), S, true, false); break;
+        case PPS_K_CG_UPDATE: launch_cg_B<double>(g, C->stream, C->rb, a.d(), c.d(), S, true, false); break;
+        case PPS_K_CG_UPDATE_F32: launch_cg_B<float>(g, C->stream, C->rb, a.f(), c.f(), S, true, false); break;
         case PPS_K_RESIDUAL: launch_residual(g, C->stream, C->rb, a.d(), b.d(), c.d(), S, 1); break;
         case PPS_K_COPY: launch_copy<double>(g, C->stream, a.d(), e.d()); break;
         case PPS_K_CHEB_SWEEP: launch_cheb_sweep<double, double>(g, C->stream, a.d(), b.d(), c.d(), e.d(), 0.1, 0.2, 0.3, 1.0, nullptr); break;
@@ -1323,8 +1343,8 @@ int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes)
     };
     double per = 0;
     switch (kernel) {
-    case PPS_K_CG_STENCIL: case PPS_K_CG_STENCIL_F32: per = 4; break;   // read r, p ; write p, Ap
-    case PPS_K_CG_UPDATE: case PPS_K_CG_UPDATE_F32: per = 6; break;     // read x, p, r, Ap ; write x, r
+    case PPS_K_CG_STENCIL: case PPS_K_CG_STENCIL_F32: per = 6; break;   // read r, p, x ; write p, Ap, x
+    case PPS_K_CG_UPDATE: case PPS_K_CG_UPDATE_F32: per = 3; break;     // read r, Ap ; write r
     case PPS_K_RESIDUAL: per = 3; break;                                // read x, b ; write r
     case PPS_K_COPY: per = 2; break;
     case PPS_K_CHEB_SWEEP: per = 4; break;                              // read Y, B, Z ; write W

@@ -3,21 +3,28 @@
 // Two kernel skeletons carry every operation:
 //  * k_stencil: 2.5-D z-marching tile.  A block of 64x4 threads owns a 64x4
 //    column tile of one x-y slab and marches through kz planes.  Each thread
-//    keeps its column's (k-1, k, k+1) values in registers; the plane k tile and
+//    keeps its column's (k-1, k, k+1) values in registers; the plane-k tile and
 //    its one-point halo are staged in LDS (double-buffered, one barrier per
 //    plane), so x- and y-neighbours come from LDS and every value of the input
 //    field is produced once per plane (this is what lets the CG search
-//    direction p = z + beta*p be formed on the fly inside the stencil).
+//    direction p = z + beta*p be formed on the fly inside the stencil).  The
+//    column value and the halo of plane k+1 are loaded while plane k is
+//    computed, so the barrier never waits on HBM latency.
 //    The 7-point operator is the reference's +Laplacian
 //    (matrixFreeOperatorA.hpp:33-38) written in difference form,
 //    cx((x[i-1]-x)+(x[i+1]-x)) + cy(..) + cz(..), which keeps FP32 accurate
 //    for smooth fields.
-//  * k_vector: the same tiling without halo, for fused axpy/dot updates.
+//  * k_vector: the same tiling without halo, for fused axpy/dot updates; each
+//    thread issues the loads of kU planes before it computes and stores any of
+//    them, so a wavefront keeps kU x (loads per point) requests in flight.
 //
-// Reductions are FP64 in every precision: wavefront shuffles, LDS across the
-// four wavefronts, one partial per block, and the last block to arrive sums
-// the partials in a fixed order (deterministic, no host round trip) and runs
-// the solver's scalar epilogue (alpha, beta, omega, convergence flag).
+// Reductions are FP64 in every precision and deterministic: wavefront
+// shuffles, LDS across the four wavefronts, one partial per block, then a
+// one-block epilogue kernel sums the partials in a fixed order and runs the
+// solver's scalar update (alpha, beta, omega, convergence flag) on the device.
+// The epilogue runs as its own launch instead of a last-block-arrives scheme:
+// on gfx950 the agent-scope release fence that scheme needs in every block
+// writes back the XCD's L2, which costs more than the whole reduction.
 //
 // Blocks are remapped so that consecutive tiles land on the same XCD (the
 // hardware deals blocks round-robin over the 8 XCDs, each with its own L2):
@@ -31,7 +38,21 @@
 
 namespace pps {
 
-Geo make_geo(const Layout& L, const Region& R, const double coef[3], int kz)
+constexpr int kU = 4;  // planes whose loads a vector-kernel thread issues together
+
+// tiling of one launch: 64-wide tiles of `th` rows, z-chunks of kz planes
+static Geo retile(const Geo& in, int th, int kz)
+{
+    Geo g = in;
+    g.kz = kz > 0 ? kz : 1;
+    g.ntx = (g.hi0 - g.lo0 + kTX - 1) / kTX;
+    g.nty = (g.hi1 - g.lo1 + th - 1) / th;
+    const int ntz = (g.hi2 - g.lo2 + g.kz - 1) / g.kz;
+    g.nblocks = g.ntx * g.nty * ntz;
+    return g;
+}
+
+Geo make_geo(const Layout& L, const Region& R, const double coef[3], int kz_st, int kz_vec, int nt)
 {
     Geo g{};
     g.plane = L.plane;
@@ -40,15 +61,21 @@ Geo make_geo(const Layout& L, const Region& R, const double coef[3], int kz)
     g.lo0 = R.lo[0]; g.hi0 = R.hi[0];
     g.lo1 = R.lo[1]; g.hi1 = R.hi[1];
     g.lo2 = R.lo[2]; g.hi2 = R.hi[2];
-    g.kz = kz > 0 ? kz : 1;
-    g.ntx = (R.hi[0] - R.lo[0] + kTX - 1) / kTX;
-    g.nty = (R.hi[1] - R.lo[1] + kTY - 1) / kTY;
-    const int ntz = (R.hi[2] - R.lo[2] + g.kz - 1) / g.kz;
-    g.nblocks = g.ntx * g.nty * ntz;
+    g.kz_st = kz_st;
+    g.kz_vec = kz_vec;
+    g.nt = nt;
     g.cx = coef[0];
     g.cy = coef[1];
     g.cz = coef[2];
-    return g;
+    return retile(g, kTY, kz_vec);
+}
+
+// stores of streamed outputs: non-temporal when the fields exceed the caches
+template <typename T>
+__device__ __forceinline__ void put(T* p, T v, int nt)
+{
+    if (nt) __builtin_nontemporal_store(v, p);
+    else *p = v;
 }
 
 // ------------------------------------------------------------ common pieces
@@ -67,13 +94,13 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-template <int NQ, class Fin>
-__device__ __forceinline__ void grid_reduce(double (&v)[NQ], const RedBuf& rb, Fin& fin)
+// sum over the block in a fixed order; result valid in thread 0
+template <int NQ, int NTHREADS = kThreads>
+__device__ __forceinline__ void block_sum(double (&v)[NQ], double (&tot)[NQ])
 {
-    constexpr int NW = kThreads / 64;
+    constexpr int NW = NTHREADS / 64;
     __shared__ double sm[NQ][NW];
-    __shared__ int amLast;
-    const int tid = threadIdx.x + kTX * threadIdx.y;
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
     const int lane = tid & 63, w = tid >> 6;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -81,49 +108,54 @@ __device__ __forceinline__ void grid_reduce(double (&v)[NQ], const RedBuf& rb, F
         if (lane == 0) sm[q][w] = t;
     }
     __syncthreads();
-    if (tid == 0) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            double t = 0.0;
-#pragma unroll
-            for (int ww = 0; ww < NW; ++ww) t += sm[q][ww];
-            __hip_atomic_store(&rb.partials[(size_t)q * rb.cap + blockIdx.x], t, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __threadfence();
-        const unsigned ticket = atomicAdd(rb.counter, 1u);
-        amLast = (ticket == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!amLast) return;
-    __threadfence();
-    double acc[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-    for (int b = tid; b < (int)gridDim.x; b += kThreads) {
-#pragma unroll
-        for (int q = 0; q < NQ; ++q)
-            acc[q] += __hip_atomic_load(&rb.partials[(size_t)q * rb.cap + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const double t = wave_sum(acc[q]);
-        if (lane == 0) sm[q][w] = t;
+        double t = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww) t += sm[q][ww];
+        tot[q] = t;
     }
-    __syncthreads();
-    if (tid == 0) {
-        double tot[NQ];
+}
+
+template <int NQ>
+__device__ __forceinline__ void block_partials(double (&v)[NQ], const RedBuf& rb)
+{
+    double tot[NQ];
+    block_sum<NQ>(v, tot);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) rb.partials[(size_t)q * rb.cap + blockIdx.x] = tot[q];
+    }
+}
+
+// one block: sums the partials of the preceding launch and runs op.finish.
+// Thread t owns partials t, t+1024, ... summed in that order with two
+// interleaved accumulators (more loads in flight), then a fixed tree.
+constexpr int kFinThreads = 1024;
+template <int NQ, class Op>
+__global__ __launch_bounds__(kFinThreads) void k_finish(Op op, RedBuf rb, int nblocks)
+{
+    if (op.gated()) return;
+    double a0[NQ], a1[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) a0[q] = a1[q] = 0.0;
+    int b = threadIdx.x;
+    for (; b + kFinThreads < nblocks; b += 2 * kFinThreads) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            tot[q] = 0.0;
-#pragma unroll
-            for (int ww = 0; ww < NW; ++ww) tot[q] += sm[q][ww];
+            a0[q] += rb.partials[(size_t)q * rb.cap + b];
+            a1[q] += rb.partials[(size_t)q * rb.cap + b + kFinThreads];
         }
-        fin.finish(tot);
-        __threadfence();
-        __hip_atomic_store(rb.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    if (b < nblocks) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) a0[q] += rb.partials[(size_t)q * rb.cap + b];
+    }
+    double acc[NQ], tot[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = a0[q] + a1[q];
+    block_sum<NQ, kFinThreads>(acc, tot);
+    if (threadIdx.x == 0) op.finish(tot);
 }
 
 struct Gate {
@@ -148,7 +180,8 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
 {
     if (op.gated()) return;
     op.load();
-    __shared__ T lds[2][kTY + 2][kTX + 2];
+    // tile: 64 columns x kTH rows; thread (lx, ly) owns rows ly, ly+kTY, ...
+    __shared__ T lds[2][kTH + 2][kTX + 2];
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int tx0 = b % g.ntx;
     const int rr = b / g.ntx;
@@ -156,62 +189,80 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
     const int tz0 = rr / g.nty;
     const int lx = threadIdx.x, ly = threadIdx.y, tid = lx + kTX * ly;
     const int i = g.lo0 + tx0 * kTX + lx;
-    const int j = g.lo1 + ty0 * kTY + ly;
+    const int jt = g.lo1 + ty0 * kTH;  // first row of the tile
     const int k0 = g.lo2 + tz0 * g.kz;
     const int k1 = min(k0 + g.kz, g.hi2);
-    const bool act = (i < g.hi0) && (j < g.hi1);
-    const bool inb = (i <= g.hi0) && (j <= g.hi1);
 
-    // halo producers: 128 threads for the two y-halo rows, 8 for the x-halo columns
+    // halo producers: 128 threads for the two y-halo rows, 2*kTH for the x-halo columns
     int hr = 0, hc = 0, hi = 0, hj = 0;
     bool hv = false;
     if (tid < 2 * kTX) {
         const int col = tid % kTX;
         const bool top = tid >= kTX;
-        hr = top ? kTY + 1 : 0;
+        hr = top ? kTH + 1 : 0;
         hc = col + 1;
         hi = g.lo0 + tx0 * kTX + col;
-        hj = g.lo1 + ty0 * kTY + (top ? kTY : -1);
+        hj = jt + (top ? kTH : -1);
         hv = true;
-    } else if (tid < 2 * kTX + 2 * kTY) {
+    } else if (tid < 2 * kTX + 2 * kTH) {
         const int t = tid - 2 * kTX;
-        const int row = t % kTY;
-        const bool right = t >= kTY;
+        const int row = t % kTH;
+        const bool right = t >= kTH;
         hr = row + 1;
         hc = right ? kTX + 1 : 0;
         hi = g.lo0 + tx0 * kTX + (right ? kTX : -1);
-        hj = g.lo1 + ty0 * kTY + row;
+        hj = jt + row;
         hv = true;
     }
     const bool hin = hv && hi <= g.hi0 && hj <= g.hi1;
     const long long hbase = (long long)g.ox + hi + (long long)g.px * hj;
-    const long long base = (long long)g.ox + i + (long long)g.px * j;
     const T cx = (T)g.cx, cy = (T)g.cy, cz = (T)g.cz;
 
+    bool act[kRY], inb[kRY];
+    long long base[kRY];
+    T vm[kRY], vc[kRY];
+#pragma unroll
+    for (int q = 0; q < kRY; ++q) {
+        const int j = jt + ly + q * kTY;
+        act[q] = (i < g.hi0) && (j < g.hi1);
+        inb[q] = (i <= g.hi0) && (j <= g.hi1);
+        base[q] = (long long)g.ox + i + (long long)g.px * j;
+        vm[q] = inb[q] ? op.in(base[q] + g.plane * (k0 - 1)) : T(0);
+        vc[q] = inb[q] ? op.in(base[q] + g.plane * k0) : T(0);
+    }
     double acc[NQ > 0 ? NQ : 1];
 #pragma unroll
     for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) acc[q] = 0.0;
 
-    T vm = inb ? op.in(base + g.plane * (k0 - 1)) : T(0);
-    T vc = inb ? op.in(base + g.plane * k0) : T(0);
+    T hcur = hin ? op.in(hbase + g.plane * k0) : T(0);
     int buf = 0;
     for (int k = k0; k < k1; ++k) {
-        const long long c = base + g.plane * k;
-        lds[buf][ly + 1][lx + 1] = vc;
-        if (hv) lds[buf][hr][hc] = hin ? op.in(hbase + g.plane * k) : T(0);
-        const T vp = inb ? op.in(c + g.plane) : T(0);
+        // loads for plane k+1 first: they complete while plane k is computed
+        T vp[kRY];
+#pragma unroll
+        for (int q = 0; q < kRY; ++q) vp[q] = inb[q] ? op.in(base[q] + g.plane * (k + 1)) : T(0);
+        const T hnext = (hin && k + 1 < k1) ? op.in(hbase + g.plane * (k + 1)) : T(0);
+#pragma unroll
+        for (int q = 0; q < kRY; ++q) lds[buf][ly + q * kTY + 1][lx + 1] = vc[q];
+        if (hv) lds[buf][hr][hc] = hcur;
         __syncthreads();
-        if (act) {
-            const T xm = lds[buf][ly + 1][lx], xp = lds[buf][ly + 1][lx + 2];
-            const T ym = lds[buf][ly][lx + 1], yp = lds[buf][ly + 2][lx + 1];
-            const T Av = cx * ((xm - vc) + (xp - vc)) + cy * ((ym - vc) + (yp - vc)) + cz * ((vm - vc) + (vp - vc));
-            op.out(c, vc, Av, acc);
+#pragma unroll
+        for (int q = 0; q < kRY; ++q) {
+            if (act[q]) {
+                const int r = ly + q * kTY + 1;
+                const T v = vc[q];
+                const T Av = cx * ((lds[buf][r][lx] - v) + (lds[buf][r][lx + 2] - v)) +
+                             cy * ((lds[buf][r - 1][lx + 1] - v) + (lds[buf][r + 1][lx + 1] - v)) +
+                             cz * ((vm[q] - v) + (vp[q] - v));
+                op.out(base[q] + g.plane * k, v, Av, acc);
+            }
+            vm[q] = vc[q];
+            vc[q] = vp[q];
         }
-        vm = vc;
-        vc = vp;
+        hcur = hnext;
         buf ^= 1;
     }
-    if constexpr (NQ > 0) grid_reduce<NQ>(acc, rb, op);
+    if constexpr (NQ > 0) block_partials<NQ>(acc, rb);
 }
 
 template <int NQ, class Op>
@@ -232,36 +283,54 @@ __global__ __launch_bounds__(kThreads) void k_vector(Geo g, Op op, RedBuf rb)
 #pragma unroll
     for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) acc[q] = 0.0;
     if (i < g.hi0 && j < g.hi1) {
-        long long c = (long long)g.ox + i + (long long)g.px * j + g.plane * k0;
-#pragma unroll 4
-        for (int k = k0; k < k1; ++k, c += g.plane) op.point(c, i, j, k, acc);
+        const long long c0 = (long long)g.ox + i + (long long)g.px * j;
+        int k = k0;
+        for (; k + kU <= k1; k += kU) {
+            typename Op::R r[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) r[u] = op.fetch(c0 + g.plane * (k + u));
+#pragma unroll
+            for (int u = 0; u < kU; ++u) op.apply(c0 + g.plane * (k + u), r[u], i, j, k + u, acc);
+        }
+        for (; k < k1; ++k) {
+            const typename Op::R r = op.fetch(c0 + g.plane * k);
+            op.apply(c0 + g.plane * k, r, i, j, k, acc);
+        }
     }
-    if constexpr (NQ > 0) grid_reduce<NQ>(acc, rb, op);
-}
-
-template <typename T, int NQ, class Op>
-static void run_stencil(const Geo& g, hipStream_t s, const Op& op, RedBuf rb)
-{
-    if (g.nblocks <= 0) return;
-    if (NQ > 0 && g.nblocks > rb.cap) {
-        std::fprintf(stderr, "pps: %d blocks exceed the reduction workspace (%d); launch skipped\n", g.nblocks, rb.cap);
-        return;
-    }
-    hipLaunchKernelGGL((k_stencil<T, NQ, Op>), dim3(g.nblocks), dim3(kTX, kTY), 0, s, g, op, rb);
+    if constexpr (NQ > 0) block_partials<NQ>(acc, rb);
 }
 
 template <int NQ, class Op>
-static void run_vector(const Geo& g, hipStream_t s, const Op& op, RedBuf rb)
+static bool check_capacity(const Geo& g, const RedBuf& rb)
 {
-    if (g.nblocks <= 0) return;
     if (NQ > 0 && g.nblocks > rb.cap) {
         std::fprintf(stderr, "pps: %d blocks exceed the reduction workspace (%d); launch skipped\n", g.nblocks, rb.cap);
-        return;
+        return false;
     }
+    return true;
+}
+
+template <typename T, int NQ, class Op>
+static void run_stencil(const Geo& g0, hipStream_t s, const Op& op, RedBuf rb)
+{
+    const Geo g = retile(g0, kTH, g0.kz_st);
+    if (g.nblocks <= 0 || !check_capacity<NQ, Op>(g, rb)) return;
+    hipLaunchKernelGGL((k_stencil<T, NQ, Op>), dim3(g.nblocks), dim3(kTX, kTY), 0, s, g, op, rb);
+    if constexpr (NQ > 0) hipLaunchKernelGGL((k_finish<NQ, Op>), dim3(1), dim3(kFinThreads), 0, s, op, rb, g.nblocks);
+}
+
+template <int NQ, class Op>
+static void run_vector(const Geo& g0, hipStream_t s, const Op& op, RedBuf rb)
+{
+    const Geo g = retile(g0, kTY, g0.kz_vec);
+    if (g.nblocks <= 0 || !check_capacity<NQ, Op>(g, rb)) return;
     hipLaunchKernelGGL((k_vector<NQ, Op>), dim3(g.nblocks), dim3(kTX, kTY), 0, s, g, op, rb);
+    if constexpr (NQ > 0) hipLaunchKernelGGL((k_finish<NQ, Op>), dim3(1), dim3(kFinThreads), 0, s, op, rb, g.nblocks);
 }
 
 static const RedBuf kNoRed{nullptr, nullptr, 0};
+
+struct Empty {};
 
 // ------------------------------------------------------------ basic kernels
 
@@ -289,8 +358,10 @@ template <typename T>
 struct OpCopy : Gate {
     const T* __restrict__ x;
     T* __restrict__ y;
+    struct R { T a; };
     __device__ void load() {}
-    __device__ void point(long long c, int, int, int, double*) const { y[c] = x[c]; }
+    __device__ R fetch(long long c) const { return {x[c]}; }
+    __device__ void apply(long long c, const R& r, int, int, int, double*) const { y[c] = r.a; }
     __device__ void finish(const double*) {}
 };
 
@@ -309,8 +380,10 @@ struct OpDot : Gate {
     const T* __restrict__ a;
     const T* __restrict__ b;
     double* out;
+    struct R { T a, b; };
     __device__ void load() {}
-    __device__ void point(long long c, int, int, int, double* acc) const { acc[0] += (double)a[c] * (double)b[c]; }
+    __device__ R fetch(long long c) const { return {a[c], b[c]}; }
+    __device__ void apply(long long, const R& r, int, int, int, double* acc) const { acc[0] += (double)r.a * (double)r.b; }
     __device__ void finish(const double* t) { *out = t[0]; }
 };
 
@@ -329,8 +402,10 @@ template <typename T>
 struct OpFill : Gate {
     T* __restrict__ a;
     T v;
+    using R = Empty;
     __device__ void load() {}
-    __device__ void point(long long c, int, int, int, double*) const { a[c] = v; }
+    __device__ R fetch(long long) const { return {}; }
+    __device__ void apply(long long c, const R&, int, int, int, double*) const { a[c] = v; }
     __device__ void finish(const double*) {}
 };
 
@@ -350,13 +425,15 @@ struct OpScaleBox : Gate {
     const Scal* S;
     int mode;
     double nrm;
+    struct R { T a; };
     __device__ void load() { nrm = S->v[S_NORM]; }
-    __device__ void point(long long c, int, int, int, double*) const
+    __device__ R fetch(long long c) const { return {a[c]}; }
+    __device__ void apply(long long c, const R& r, int, int, int, double*) const
     {
         // normalizeProblemToFieldBNorm divides (iterativeSolverBase.hpp:227-231);
         // the solvers' epilogues multiply (baseCG.hpp:248-252)
-        if (mode == 0) a[c] = (T)((double)a[c] / nrm);
-        else a[c] = (T)((double)a[c] * nrm);
+        if (mode == 0) a[c] = (T)((double)r.a / nrm);
+        else a[c] = (T)((double)r.a * nrm);
     }
     __device__ void finish(const double*) {}
 };
@@ -377,8 +454,10 @@ struct OpPointwise : Gate {
     const T* __restrict__ in;
     T* __restrict__ out;
     T c;
+    struct R { T a; };
     __device__ void load() {}
-    __device__ void point(long long p, int, int, int, double*) const { out[p] = c * in[p]; }
+    __device__ R fetch(long long p) const { return {in[p]}; }
+    __device__ void apply(long long p, const R& r, int, int, int, double*) const { out[p] = c * r.a; }
     __device__ void finish(const double*) {}
 };
 
@@ -397,8 +476,10 @@ template <typename TS, typename TD>
 struct OpCast : Gate {
     const TS* __restrict__ src;
     TD* __restrict__ dst;
+    struct R { TS a; };
     __device__ void load() {}
-    __device__ void point(long long c, int, int, int, double*) const { dst[c] = (TD)src[c]; }
+    __device__ R fetch(long long c) const { return {src[c]}; }
+    __device__ void apply(long long c, const R& r, int, int, int, double*) const { dst[c] = (TD)r.a; }
     __device__ void finish(const double*) {}
 };
 
@@ -441,6 +522,7 @@ struct OpResidual : Gate {
             S->v[S_RZ] = t[1];
             S->v[S_BETA] = 0.0;
             S->v[S_RHO0] = 1.0;
+            S->v[S_ALPHA] = 0.0;  // no pending x update before the first iteration
             S->iter = 0;
             if (S->hist != nullptr && S->hist_cap > 0) S->hist[0] = e;
             S->done = (e < S->v[S_TOL]) ? 2 : 0;
@@ -462,16 +544,16 @@ void launch_residual(const Geo& g, hipStream_t s, RedBuf rb, const double* x, co
     run_stencil<double, 2>(g, s, op, rb);
 }
 
-// ||b_adj||^2 (iterativeSolverBase.hpp:171-201 with adjustFieldBForDirichletNeumanBCs :429-534)
-struct OpBNorm : Gate {
+// boundary adjustment of b at one point (adjustFieldBForDirichletNeumanBCs,
+// iterativeSolverBase.hpp:429-534): Dirichlet faces subtract the boundary value
+// of x over ds^2 on the first interior layer, Neumann faces add -+ 2 du/dn / ds
+// on the boundary layer; applied in the reference's face order x- x+ y- y+ z- z+
+struct BAdjust {
     const double* __restrict__ x;
-    const double* __restrict__ b;
     const double* flux[6];
     int bc[6], ld[6], fng[6][2];
     double ds[3];
     long long stride[3];
-    Scal* S;
-    __device__ void load() {}
     __device__ long long fpos(int d, int i, int j, int k) const
     {
         const int f = 2 * d;
@@ -479,26 +561,57 @@ struct OpBNorm : Gate {
         if (d == 1) return i + (long long)fng[f][0] * k;
         return i + (long long)fng[f][0] * j;
     }
-    __device__ void point(long long c, int i, int j, int k, double* acc) const
+    __device__ double adjust(double v, long long c, int i, int j, int k) const
     {
-        double v = b[c];
         const int idx[3] = {i, j, k};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const double dd = ds[d] * ds[d];
-            // low face
             if (bc[2 * d] == 0) {
                 if (idx[d] == ld[2 * d] + 1) v -= x[c - stride[d]] / dd;
             } else if (bc[2 * d] == 1 && flux[2 * d] != nullptr) {
                 if (idx[d] == ld[2 * d]) v += 2 * flux[2 * d][fpos(d, i, j, k)] / ds[d];
             }
-            // high face
             if (bc[2 * d + 1] == 0) {
                 if (idx[d] == ld[2 * d + 1] - 2) v -= x[c + stride[d]] / dd;
             } else if (bc[2 * d + 1] == 1 && flux[2 * d + 1] != nullptr) {
                 if (idx[d] == ld[2 * d + 1] - 1) v -= 2 * flux[2 * d + 1][fpos(d, i, j, k)] / ds[d];
             }
         }
+        return v;
+    }
+};
+
+static BAdjust make_badjust(const Geo& g, const double* x, const int bc[6], const int lim_data[6],
+                            const double* const flux[6], const int fng[6][2], const double ds[3])
+{
+    BAdjust a;
+    a.x = x;
+    for (int f = 0; f < 6; ++f) {
+        a.flux[f] = flux[f];
+        a.bc[f] = bc[f];
+        a.ld[f] = lim_data[f];
+        a.fng[f][0] = fng[f][0];
+        a.fng[f][1] = fng[f][1];
+    }
+    for (int d = 0; d < 3; ++d) a.ds[d] = ds[d];
+    a.stride[0] = 1;
+    a.stride[1] = g.px;
+    a.stride[2] = g.plane;
+    return a;
+}
+
+// ||b_adj||^2 over the solver region (iterativeSolverBase.hpp:171-201)
+struct OpBNorm : Gate {
+    const double* __restrict__ b;
+    BAdjust adj;
+    Scal* S;
+    struct R { double b; };
+    __device__ void load() {}
+    __device__ R fetch(long long c) const { return {b[c]}; }
+    __device__ void apply(long long c, const R& r, int i, int j, int k, double* acc) const
+    {
+        const double v = adj.adjust(r.b, c, i, j, k);
         acc[0] += v * v;
     }
     __device__ void finish(const double* t) { S->v[S_NORM] = sqrt(t[0]); }
@@ -511,47 +624,22 @@ void launch_bnorm(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const
     (void)lim_solver;
     OpBNorm op;
     op.gate = nullptr;
-    op.x = x;
     op.b = b;
-    for (int f = 0; f < 6; ++f) {
-        op.flux[f] = flux[f];
-        op.bc[f] = bc[f];
-        op.ld[f] = lim_data[f];
-        op.fng[f][0] = fng[f][0];
-        op.fng[f][1] = fng[f][1];
-    }
-    for (int d = 0; d < 3; ++d) op.ds[d] = ds[d];
-    op.stride[0] = 1;
-    op.stride[1] = g.px;
-    op.stride[2] = g.plane;
+    op.adj = make_badjust(g, x, bc, lim_data, flux, fng, ds);
     op.S = S;
     run_vector<1>(g, s, op, rb);
 }
 
-// adjustFieldBForDirichletNeumanBCs (iterativeSolverBase.hpp:429-534) written
-// into dst over the data region (the main-loop Chebyshev works on this copy)
-struct OpAdjustB : OpBNorm {
-    double* __restrict__ dst;
-    __device__ void point(long long c, int i, int j, int k, double*) const
-    {
-        double v = b[c];
-        const int idx[3] = {i, j, k};
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double dd = ds[d] * ds[d];
-            if (bc[2 * d] == 0) {
-                if (idx[d] == ld[2 * d] + 1) v -= x[c - stride[d]] / dd;
-            } else if (bc[2 * d] == 1 && flux[2 * d] != nullptr) {
-                if (idx[d] == ld[2 * d]) v += 2 * flux[2 * d][fpos(d, i, j, k)] / ds[d];
-            }
-            if (bc[2 * d + 1] == 0) {
-                if (idx[d] == ld[2 * d + 1] - 2) v -= x[c + stride[d]] / dd;
-            } else if (bc[2 * d + 1] == 1 && flux[2 * d + 1] != nullptr) {
-                if (idx[d] == ld[2 * d + 1] - 1) v -= 2 * flux[2 * d + 1][fpos(d, i, j, k)] / ds[d];
-            }
-        }
-        dst[c] = v;
-    }
+// the adjusted right-hand side written in place over the data region (the
+// main-loop Chebyshev works on this copy; chebyshevIteration.hpp:61-67)
+struct OpAdjustB : Gate {
+    double* __restrict__ b;
+    BAdjust adj;
+    struct R { double b; };
+    __device__ void load() {}
+    __device__ R fetch(long long c) const { return {b[c]}; }
+    __device__ void apply(long long c, const R& r, int i, int j, int k, double*) const { b[c] = adj.adjust(r.b, c, i, j, k); }
+    __device__ void finish(const double*) {}
 };
 
 void adjust_b_field(const Geo& gdata, hipStream_t s, const double* x, double* b, const int bc[6], const int lim_data[6],
@@ -559,21 +647,8 @@ void adjust_b_field(const Geo& gdata, hipStream_t s, const double* x, double* b,
 {
     OpAdjustB op;
     op.gate = nullptr;
-    op.x = x;
     op.b = b;
-    op.dst = b;
-    for (int f = 0; f < 6; ++f) {
-        op.flux[f] = flux[f];
-        op.bc[f] = bc[f];
-        op.ld[f] = lim_data[f];
-        op.fng[f][0] = fng[f][0];
-        op.fng[f][1] = fng[f][1];
-    }
-    for (int d = 0; d < 3; ++d) op.ds[d] = ds[d];
-    op.stride[0] = 1;
-    op.stride[1] = gdata.px;
-    op.stride[2] = gdata.plane;
-    op.S = nullptr;
+    op.adj = make_badjust(gdata, x, bc, lim_data, flux, fng, ds);
     run_vector<0>(gdata, s, op, kNoRed);
 }
 
@@ -630,25 +705,33 @@ void launch_ghost(hipStream_t s, const Layout& L, const Faces& F, T* field, int 
 // ------------------------------------------------------------------- CG
 
 // p_new = a*src + beta*p_old, Ap = A p_new, p.Ap ; alpha = rz / pAp  (baseCG.hpp:126-150,197-207)
+// The reference updates x and r together after alpha is known (baseCG.hpp:154-165);
+// here x's update for iteration k-1 rides along with iteration k's stencil
+// pass, which reads p_{k-1} anyway: the same x after every iteration, one
+// fewer field pass per iteration.  The last update runs after the loop.
 template <typename T>
 struct OpCgA : Gate {
     const T* __restrict__ src;
     const T* __restrict__ pin;
     T* __restrict__ pout;
     T* __restrict__ Ap;
+    T* __restrict__ x;
     Scal* S;
     bool pointwise;
-    T a, beta;
+    int nt;
+    T a, beta, alpha_prev;
     __device__ void load()
     {
         a = pointwise ? (T)S->v[S_JAC] : T(1);
         beta = (T)S->v[S_BETA];
+        alpha_prev = (T)S->v[S_ALPHA];
     }
     __device__ T in(long long c) const { return a * src[c] + beta * pin[c]; }
     __device__ void out(long long c, T vc, T Av, double* acc) const
     {
-        pout[c] = vc;
-        Ap[c] = Av;
+        put(pout + c, vc, nt);
+        put(Ap + c, Av, nt);
+        put(x + c, x[c] + alpha_prev * pin[c], nt);
         acc[0] += (double)vc * (double)Av;
     }
     __device__ void finish(const double* t)
@@ -661,7 +744,7 @@ struct OpCgA : Gate {
 // pin/pout double-buffer the search direction: neighbouring tiles rebuild
 // p_new on their halo from p_old, so p_old must stay intact during the launch.
 template <typename T>
-void launch_cg_A2(const Geo& g, hipStream_t s, RedBuf rb, const T* src, const T* pin, T* pout, T* Ap, Scal* S,
+void launch_cg_A2(const Geo& g, hipStream_t s, RedBuf rb, const T* src, const T* pin, T* pout, T* Ap, T* x, Scal* S,
                   bool pointwise)
 {
     OpCgA<T> op;
@@ -670,30 +753,57 @@ void launch_cg_A2(const Geo& g, hipStream_t s, RedBuf rb, const T* src, const T*
     op.pin = pin;
     op.pout = pout;
     op.Ap = Ap;
+    op.x = x;
     op.S = S;
     op.pointwise = pointwise;
+    op.nt = g.nt;
     run_stencil<T, 1>(g, s, op, rb);
 }
 
 template <typename T>
-struct OpCgB : Gate {
+struct OpCgFinalX : Gate {
     T* __restrict__ x;
     const T* __restrict__ p;
+    const Scal* S;
+    T alpha;
+    struct R { T x, p; };
+    __device__ void load() { alpha = (T)S->v[S_ALPHA]; }
+    __device__ R fetch(long long c) const { return {x[c], p[c]}; }
+    __device__ void apply(long long c, const R& v, int, int, int, double*) const { x[c] = v.x + alpha * v.p; }
+    __device__ void finish(const double*) {}
+};
+
+template <typename T>
+void launch_cg_final_x(const Geo& g, hipStream_t s, T* x, const T* p, const Scal* S)
+{
+    OpCgFinalX<T> op;
+    op.gate = nullptr;
+    op.x = x;
+    op.p = p;
+    op.S = S;
+    run_vector<0>(g, s, op, kNoRed);
+}
+
+// r -= alpha Ap, r.r, r.(jac r)  (baseCG.hpp:154-194)
+template <typename T>
+struct OpCgB : Gate {
     T* __restrict__ r;
     const T* __restrict__ Ap;
     Scal* S;
     bool pointwise, history;
+    int nt;
     T alpha, jac;
+    struct R { T r, ap; };
     __device__ void load()
     {
         alpha = (T)S->v[S_ALPHA];
         jac = (T)S->v[S_JAC];
     }
-    __device__ void point(long long c, int, int, int, double* acc) const
+    __device__ R fetch(long long c) const { return {r[c], Ap[c]}; }
+    __device__ void apply(long long c, const R& v, int, int, int, double* acc) const
     {
-        x[c] = x[c] + alpha * p[c];
-        const T rv = r[c] - alpha * Ap[c];
-        r[c] = rv;
+        const T rv = v.r - alpha * v.ap;
+        put(r + c, rv, nt);
         acc[0] += (double)rv * (double)rv;
         acc[1] += (double)rv * (double)(jac * rv);
     }
@@ -709,18 +819,16 @@ struct OpCgB : Gate {
 };
 
 template <typename T>
-void launch_cg_B(const Geo& g, hipStream_t s, RedBuf rb, T* x, const T* p, T* r, const T* Ap, Scal* S, bool pointwise,
-                 bool history)
+void launch_cg_B(const Geo& g, hipStream_t s, RedBuf rb, T* r, const T* Ap, Scal* S, bool pointwise, bool history)
 {
     OpCgB<T> op;
     op.gate = &S->done;
-    op.x = x;
-    op.p = p;
     op.r = r;
     op.Ap = Ap;
     op.S = S;
     op.pointwise = pointwise;
     op.history = history;
+    op.nt = g.nt;
     run_vector<2>(g, s, op, rb);
 }
 
@@ -729,8 +837,10 @@ struct OpCgRz : Gate {
     const T* __restrict__ r;
     const T* __restrict__ z;
     Scal* S;
+    struct R { T r, z; };
     __device__ void load() {}
-    __device__ void point(long long c, int, int, int, double* acc) const { acc[0] += (double)r[c] * (double)z[c]; }
+    __device__ R fetch(long long c) const { return {r[c], z[c]}; }
+    __device__ void apply(long long, const R& v, int, int, int, double* acc) const { acc[0] += (double)v.r * (double)v.z; }
     __device__ void finish(const double* t)
     {
         S->v[S_BETA] = S->iter == 0 ? 0.0 : t[0] / S->v[S_RZ];
@@ -788,8 +898,10 @@ struct OpBiK2 : Gate {
     const T* __restrict__ AMp;
     const Scal* S;
     T alpha;
+    struct R { T r, a; };
     __device__ void load() { alpha = (T)S->v[S_ALPHA]; }
-    __device__ void point(long long c, int, int, int, double*) const { r[c] = r[c] - alpha * AMp[c]; }
+    __device__ R fetch(long long c) const { return {r[c], AMp[c]}; }
+    __device__ void apply(long long c, const R& v, int, int, int, double*) const { r[c] = v.r - alpha * v.a; }
     __device__ void finish(const double*) {}
 };
 
@@ -848,18 +960,19 @@ struct OpBiK45 : Gate {
     Scal* S;
     T amp, az, alpha, omega;
     bool history;
+    struct R { T x, mp, z, r, az, r0; };
     __device__ void load()
     {
         alpha = (T)S->v[S_ALPHA];
         omega = (T)S->v[S_OMEGA];
     }
-    __device__ void point(long long c, int, int, int, double* acc) const
+    __device__ R fetch(long long c) const { return {x[c], Mp[c], z[c], r[c], Az[c], r0[c]}; }
+    __device__ void apply(long long c, const R& v, int, int, int, double* acc) const
     {
-        const T rv0 = r[c];
-        x[c] = x[c] + alpha * (amp * Mp[c]) + omega * (az * z[c]);
-        const T rv = rv0 - omega * Az[c];
+        x[c] = v.x + alpha * (amp * v.mp) + omega * (az * v.z);
+        const T rv = v.r - omega * v.az;
         r[c] = rv;
-        acc[0] += (double)r0[c] * (double)rv;
+        acc[0] += (double)v.r0 * (double)rv;
         acc[1] += (double)rv * (double)rv;
     }
     __device__ void finish(const double* t)
@@ -898,12 +1011,14 @@ struct OpBiK6 : Gate {
     const T* __restrict__ AMp;
     const Scal* S;
     T beta, omega;
+    struct R { T p, r, a; };
     __device__ void load()
     {
         beta = (T)S->v[S_BETA];
         omega = (T)S->v[S_OMEGA];
     }
-    __device__ void point(long long c, int, int, int, double*) const { p[c] = r[c] + beta * (p[c] - omega * AMp[c]); }
+    __device__ R fetch(long long c) const { return {p[c], r[c], AMp[c]}; }
+    __device__ void apply(long long c, const R& v, int, int, int, double*) const { p[c] = v.r + beta * (v.p - omega * v.a); }
     __device__ void finish(const double*) {}
 };
 
@@ -1027,8 +1142,10 @@ struct OpFcgGamma : Gate {
     const T* __restrict__ z;
     const T* __restrict__ Apold;
     Scal* S;
+    struct R { T z, a; };
     __device__ void load() {}
-    __device__ void point(long long c, int, int, int, double* acc) const { acc[0] += (double)z[c] * (double)Apold[c]; }
+    __device__ R fetch(long long c) const { return {z[c], Apold[c]}; }
+    __device__ void apply(long long, const R& v, int, int, int, double* acc) const { acc[0] += (double)v.z * (double)v.a; }
     __device__ void finish(const double* t)
     {
         S->v[S_GAM] = t[0];
@@ -1096,11 +1213,13 @@ struct OpFcgUpdate : Gate {
     Scal* S;
     bool history;
     T alpha;
+    struct R { T x, p, r, ap; };
     __device__ void load() { alpha = (T)S->v[S_ALPHA]; }
-    __device__ void point(long long c, int, int, int, double* acc) const
+    __device__ R fetch(long long c) const { return {x[c], p[c], r[c], Ap[c]}; }
+    __device__ void apply(long long c, const R& v, int, int, int, double* acc) const
     {
-        x[c] = x[c] + alpha * p[c];
-        const T rv = r[c] - alpha * Ap[c];
+        x[c] = v.x + alpha * v.p;
+        const T rv = v.r - alpha * v.ap;
         r[c] = rv;
         acc[0] += (double)rv * (double)rv;
     }
@@ -1175,8 +1294,10 @@ struct OpIrUpdate : Gate {
     const float* __restrict__ d;
     const Scal* SI;
     float sc;
+    struct R { double x; float d; };
     __device__ void load() { sc = (float)SI->v[S_INNER_NORM]; }
-    __device__ void point(long long c, int, int, int, double*) const { x[c] = x[c] + (double)(d[c] * sc); }
+    __device__ R fetch(long long c) const { return {x[c], d[c]}; }
+    __device__ void apply(long long c, const R& v, int, int, int, double*) const { x[c] = v.x + (double)(v.d * sc); }
     __device__ void finish(const double*) {}
 };
 
@@ -1199,14 +1320,16 @@ struct OpIrInit : Gate {
     Scal* SI;
     bool pointwise;
     float inv, jac;
+    struct R { float r; };
     __device__ void load()
     {
         inv = (float)(1.0 / SI->v[S_INNER_NORM]);
         jac = (float)SI->v[S_JAC];
     }
-    __device__ void point(long long c, int, int, int, double* acc) const
+    __device__ R fetch(long long c) const { return {r[c]}; }
+    __device__ void apply(long long c, const R& v, int, int, int, double* acc) const
     {
-        const float rv = r[c] * inv;
+        const float rv = v.r * inv;
         r[c] = rv;
         x[c] = 0.0f;
         p[c] = 0.0f;
@@ -1221,6 +1344,7 @@ struct OpIrInit : Gate {
         SI->v[S_RZ] = t[1];
         SI->v[S_BETA] = 0.0;
         SI->v[S_PAP_OLD] = 1.0;
+        SI->v[S_ALPHA] = 0.0;
         SI->iter = 0;
         SI->done = e < SI->v[S_TOL] ? 2 : 0;
     }
@@ -1247,8 +1371,9 @@ void launch_ir_inner_init(const Geo& g, hipStream_t s, RedBuf rb, float* r, floa
     template void launch_scale_box<T>(const Geo&, hipStream_t, T*, const Scal*, int);                          \
     template void launch_fill<T>(const Geo&, hipStream_t, T*, T);                                              \
     template void launch_ghost<T>(hipStream_t, const Layout&, const Faces&, T*, int, const Scal*, const int*);  \
-    template void launch_cg_A2<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, T*, T*, Scal*, bool);   \
-    template void launch_cg_B<T>(const Geo&, hipStream_t, RedBuf, T*, const T*, T*, const T*, Scal*, bool, bool); \
+    template void launch_cg_A2<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, T*, T*, T*, Scal*, bool); \
+    template void launch_cg_final_x<T>(const Geo&, hipStream_t, T*, const T*, const Scal*);                      \
+    template void launch_cg_B<T>(const Geo&, hipStream_t, RedBuf, T*, const T*, Scal*, bool, bool);            \
     template void launch_cg_rz<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, Scal*);                 \
     template void launch_pointwise<T>(const Geo&, hipStream_t, const T*, T*, double, const int*);              \
     template void launch_bicg_k1<T>(const Geo&, hipStream_t, RedBuf, const T*, double, T*, const T*, Scal*);   \

@@ -17,8 +17,10 @@
 namespace pps {
 
 constexpr int kTX = 64;   // tile width in x: one wavefront per row
-constexpr int kTY = 4;    // rows per tile: 4 wavefronts per block
+constexpr int kTY = 4;    // thread rows per block: 4 wavefronts per block
 constexpr int kThreads = kTX * kTY;
+constexpr int kRY = 4;    // rows per thread in the stencil tile
+constexpr int kTH = kTY * kRY;  // stencil tile height (rows)
 constexpr int kMaxRedQ = 4;
 constexpr int kMinKz = 4;   // smallest z-chunk; sizes the reduction workspace
 
@@ -71,12 +73,16 @@ struct Geo {
     long long plane;
     int px, ox;
     int lo0, hi0, lo1, hi1, lo2, hi2;
-    int ntx, nty, kz;
+    int ntx, nty, kz;   // tiling of one launch (set by the launcher)
     int nblocks;
+    int kz_st, kz_vec;  // z-chunk of stencil and vector launches
+    int nt;             // stream outputs past the caches (fields larger than the caches)
     double cx, cy, cz;  // 1/ds^2 per direction
 };
 
-Geo make_geo(const Layout& L, const Region& R, const double coef[3], int kz);
+// kz_st / kz_vec: z-chunk lengths of stencil and vector kernels; nt: use
+// non-temporal stores
+Geo make_geo(const Layout& L, const Region& R, const double coef[3], int kz_st, int kz_vec, int nt);
 
 // Neumann faces of the current grid, for the ghost-fill kernel.
 struct Faces {
@@ -128,15 +134,19 @@ void launch_ghost(hipStream_t s, const Layout& L, const Faces& F, T* field, int 
                   const int* gate);
 
 // ---- CG (plain / pointwise-preconditioned / buffer-preconditioned) ----
-// p_new = a*src + beta*p_old (a = S_JAC or 1), Ap = A p_new, pAp -> alpha = S_RZ / pAp.
-// pin/pout must be distinct buffers (halo tiles rebuild p_new from p_old).
+// p_new = a*src + beta*p_old (a = S_JAC or 1), Ap = A p_new, pAp -> alpha = S_RZ / pAp,
+// and the previous iteration's solution update x += alpha_prev p_old on the
+// tile's own points.  pin/pout must be distinct buffers (halo tiles rebuild
+// p_new from p_old).
 template <typename T>
-void launch_cg_A2(const Geo& g, hipStream_t s, RedBuf rb, const T* src, const T* pin, T* pout, T* Ap, Scal* S,
+void launch_cg_A2(const Geo& g, hipStream_t s, RedBuf rb, const T* src, const T* pin, T* pout, T* Ap, T* x, Scal* S,
                   bool pointwise);
-// x += alpha p, r -= alpha Ap, rr (and r.(jac r) when pointwise) -> err, beta, iter, done
+// r -= alpha Ap, rr (and r.(jac r) when pointwise) -> err, beta, iter, done
 template <typename T>
-void launch_cg_B(const Geo& g, hipStream_t s, RedBuf rb, T* x, const T* p, T* r, const T* Ap, Scal* S,
-                 bool pointwise, bool history);
+void launch_cg_B(const Geo& g, hipStream_t s, RedBuf rb, T* r, const T* Ap, Scal* S, bool pointwise, bool history);
+// x += S[S_ALPHA] p: the last iteration's solution update, after the loop
+template <typename T>
+void launch_cg_final_x(const Geo& g, hipStream_t s, T* x, const T* p, const Scal* S);
 // rz_new = r.z -> beta = rz_new / rz (0 at iteration 0), rz = rz_new (buffer-preconditioned CG)
 template <typename T>
 void launch_cg_rz(const Geo& g, hipStream_t s, RedBuf rb, const T* r, const T* z, Scal* S);
