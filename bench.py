@@ -79,6 +79,24 @@ def run_form(C, chain, steps, warmup, name):
             "outer_iterations": res["outer_iterations"], "true_rel_residual": res["error_compute_operator"]}
 
 
+KERNELS = {"cg_stencil_f64": 0, "cg_update_f64": 1, "residual_f64": 2, "copy_f64": 3, "cheb_sweep_f64": 4,
+           "cg_stencil_f32": 5, "cg_update_f32": 6}  # PPS_K_* ids of include/pps_abi.h
+
+
+def profiled_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
+    counter summary (scaled by a plain copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", f"krylov_{n}_hbm_traffic.md")
+    try:
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == kernel:
+                return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def roofline(C, kernels, reps):
     out = {}
     for name, kid in kernels.items():
@@ -144,6 +162,8 @@ def main():
     ap.add_argument("--kz", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="plain_fp64_cg,ir_fp32cg_fp64")
+    ap.add_argument("--reps", type=int, default=20, help="launches per kernel timing")
+    ap.add_argument("--kernels-only", action="store_true", help="only the per-kernel timings")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {a.gpus})", file=sys.stderr)
@@ -164,19 +184,22 @@ def main():
 
     results = {}
     forms = chains()
+    if a.kernels_only:
+        # the per-kernel timings alone (short enough to run under hardware counters)
+        roofline(C, KERNELS, reps=a.reps)
+        C.close()
+        return
     for name in a.forms.split(","):
         results[name] = run_form(C, forms[name], a.steps, a.warmup, name)
 
     headline = "ir_fp32cg_fp64" if "ir_fp32cg_fp64" in results else list(results)[0]
-    kern = roofline(C, {"cg_stencil_f64": L.K_CG_STENCIL, "cg_update_f64": L.K_CG_UPDATE,
-                        "cg_stencil_f32": L.K_CG_STENCIL_F32, "cg_update_f32": L.K_CG_UPDATE_F32,
-                        "residual_f64": L.K_RESIDUAL, "copy_f64": L.K_COPY}, reps=20)
-    # dominant kernel of the headline solve: the update kernel moves 6 words per point,
-    # the fused stencil 4; both run once per iteration
-    dom = "cg_update_f32" if headline == "ir_fp32cg_fp64" else "cg_update_f64"
+    kern = roofline(C, KERNELS, reps=a.reps)
+    # dominant kernel of the headline solve: per CG iteration the fused stencil pass
+    # moves 6 words per point (r, p, x in; p, Ap, x out), the update pass 3
+    dom = "cg_stencil_f32" if headline == "ir_fp32cg_fp64" else "cg_stencil_f64"
     k = kern[dom]
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(k["gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(k["gbs"] / PEAK_HBM_GBS, 4), "traffic": None,
+            "frac": round(k["gbs"] / PEAK_HBM_GBS, 4), "traffic": profiled_traffic(dom, a.n),
             "bytes_per_launch": k["bytes"], "ms_per_launch": k["ms"]}
 
     cpu = None
