@@ -89,8 +89,8 @@ typedef struct pps_stage {
     double cheb_rescale_min; /* rescaleEigMin                                       */
     double cheb_rescale_max; /* rescaleEigMax                                       */
     int cheb_local;      /* alpaka `local` eigenvalue variant                       */
-    int stall_window;    /* PPS_NN: iterations per stall test (0 = never fall back) */
-    double stall_ratio;  /* PPS_NN: fall back if residual drop over the window > this */
+    int stall_window;    /* FCG over PPS_NN: iterations per stall test (0 = never fall back); set on either stage */
+    double stall_ratio;  /* fall back to Jacobi when the residual drop over the window is above this factor */
 } pps_stage;
 
 typedef struct pps_result {
@@ -165,6 +165,61 @@ enum {
 int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes);
 /* z-chunk lengths of the stencil and vector kernels (0 keeps the default) */
 int pps_set_tiling(void* ctx, int kz_stencil, int kz_vector);
+
+/* ---- learned preconditioner (new: the reference has no counterpart) ------
+ * A 3-level convolutional network (16 channels, 3^3 kernels, ReLU, BF16
+ * weights/activations, FP32 accumulation) used as the PPS_NN stage below a
+ * PPS_FCG stage.  Trained on the GPU by its own forward/backward kernels on
+ * (residual, error) pairs of plain CG iterates. */
+typedef struct pps_nn_train_cfg {
+    int n;                /* training grid: n^3 interior points, Dirichlet unit cube  */
+    int nrhs;             /* right-hand sides (k^-2 power-law spectra, seeds seed+s)  */
+    int snapshots;        /* CG iterates per right-hand side (pairs = nrhs*snapshots) */
+    int kmax;             /* spectrum modes per direction                             */
+    unsigned long long seed;
+    int batch;            /* pairs per Adam step                                      */
+    int steps;            /* Adam steps                                               */
+    double lr, lr_final;  /* cosine learning-rate schedule                            */
+    double white_fraction;/* share of right-hand sides drawn as white noise N(0,1)
+                             instead of the k^-2 spectrum (rough residuals)           */
+} pps_nn_train_cfg;
+
+typedef struct pps_nn_train_stats {
+    long long pairs;
+    int steps;
+    double seconds;       /* training time (generation excluded)                      */
+    double loss_first, loss_last;  /* mean 1 - cos^2 in the energy norm (0 = exact)   */
+} pps_nn_train_stats;
+
+/* network with He-initialised weights (std::mt19937_64 seeded by `seed`) */
+int pps_nn_create(uint64_t seed, void** nn);
+void pps_nn_destroy(void* nn);
+const char* pps_nn_last_error(void* nn);
+/* stored FP32 parameters (14 layers x (27*16*16 + 16)) and the effective count */
+int pps_nn_num_params(void* nn, long long* stored, long long* effective);
+int pps_nn_get_params(void* nn, float* host, long long n);
+int pps_nn_set_params(void* nn, const float* host, long long n);
+/* network output for a dense batch [batch][n2][n1][n0] of inputs */
+int pps_nn_forward(void* nn, const float* in, int batch, const int n[3], float* out);
+/* activation `index` (0..12) of the last forward pass, [batch][z][y][x][16] */
+int pps_nn_activation(void* nn, int index, float* out);
+/* training loss of a batch (in = normalised residuals, g = their energy norms)
+ * and its gradient with respect to all stored parameters */
+int pps_nn_loss_grad(void* nn, const float* in, const double* g, int batch, const int n[3], double* loss,
+                     float* grad);
+/* training set: plain FP64 CG on the cfg grid; per right-hand side the
+ * converged u* and `snapshots` iterates x_k; pairs (r_k / rms(r_k), energy
+ * norm of e_k = u* - x_k) kept on the device */
+int pps_nn_generate(void* nn, const pps_nn_train_cfg* cfg, long long* pairs);
+/* one pair of the training set (r: n^3 floats, may be null) */
+int pps_nn_dataset(void* nn, long long index, float* r, double* g);
+/* Adam on the training set */
+int pps_nn_train(void* nn, const pps_nn_train_cfg* cfg, pps_nn_train_stats* stats);
+/* out = M(in) with the attached network on FP64 fields, as a PPS_NN stage
+ * applies it (input scaled to unit RMS over the solver region) */
+int pps_nn_apply_field(void* ctx, int out, int in);
+/* PPS_NN stages of ctx use this network (nn = NULL detaches) */
+int pps_nn_attach(void* ctx, void* nn);
 
 #ifdef __cplusplus
 }

@@ -69,10 +69,37 @@ class CResult(ctypes.Structure):
     ]
 
 
+class CNNTrainCfg(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int),
+        ("nrhs", ctypes.c_int),
+        ("snapshots", ctypes.c_int),
+        ("kmax", ctypes.c_int),
+        ("seed", ctypes.c_ulonglong),
+        ("batch", ctypes.c_int),
+        ("steps", ctypes.c_int),
+        ("lr", ctypes.c_double),
+        ("lr_final", ctypes.c_double),
+        ("white_fraction", ctypes.c_double),
+    ]
+
+
+class CNNTrainStats(ctypes.Structure):
+    _fields_ = [
+        ("pairs", ctypes.c_longlong),
+        ("steps", ctypes.c_int),
+        ("seconds", ctypes.c_double),
+        ("loss_first", ctypes.c_double),
+        ("loss_last", ctypes.c_double),
+    ]
+
+
 # every symbol declared in include/pps_abi.h, with its ctypes signature
 _VP = ctypes.c_void_p
 _DP = ctypes.POINTER(ctypes.c_double)
 _IP = ctypes.POINTER(ctypes.c_int)
+_FP = ctypes.POINTER(ctypes.c_float)
+_LLP = ctypes.POINTER(ctypes.c_longlong)
 SIGNATURES = {
     "pps_abi_version": ([], ctypes.c_int),
     "pps_create": ([ctypes.POINTER(CGrid), ctypes.POINTER(_VP)], ctypes.c_int),
@@ -97,6 +124,20 @@ SIGNATURES = {
     "pps_rhs_spectrum": ([_VP, ctypes.c_uint64, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "pps_time_kernel": ([_VP, ctypes.c_int, ctypes.c_int, _DP, _DP], ctypes.c_int),
     "pps_set_tiling": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "pps_nn_create": ([ctypes.c_uint64, ctypes.POINTER(_VP)], ctypes.c_int),
+    "pps_nn_destroy": ([_VP], None),
+    "pps_nn_last_error": ([_VP], ctypes.c_char_p),
+    "pps_nn_num_params": ([_VP, _LLP, _LLP], ctypes.c_int),
+    "pps_nn_get_params": ([_VP, _FP, ctypes.c_longlong], ctypes.c_int),
+    "pps_nn_set_params": ([_VP, _FP, ctypes.c_longlong], ctypes.c_int),
+    "pps_nn_forward": ([_VP, _FP, ctypes.c_int, _IP, _FP], ctypes.c_int),
+    "pps_nn_activation": ([_VP, ctypes.c_int, _FP], ctypes.c_int),
+    "pps_nn_loss_grad": ([_VP, _FP, _DP, ctypes.c_int, _IP, _DP, _FP], ctypes.c_int),
+    "pps_nn_generate": ([_VP, ctypes.POINTER(CNNTrainCfg), _LLP], ctypes.c_int),
+    "pps_nn_dataset": ([_VP, ctypes.c_longlong, _FP, _DP], ctypes.c_int),
+    "pps_nn_train": ([_VP, ctypes.POINTER(CNNTrainCfg), ctypes.POINTER(CNNTrainStats)], ctypes.c_int),
+    "pps_nn_apply_field": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "pps_nn_attach": ([_VP, _VP], ctypes.c_int),
 }
 
 _lib = None
@@ -244,8 +285,124 @@ class Context:
         self._chk(lib().pps_time_kernel(self.h, kernel, reps, ctypes.byref(ms), ctypes.byref(nbytes)))
         return ms.value, nbytes.value
 
+    def nn_apply(self, out, inp):
+        """out = M(inp) with the attached network (FP64 fields)."""
+        self._chk(lib().pps_nn_apply_field(self.h, out, inp))
+
     def set_tiling(self, kz_stencil=0, kz_vector=0):
         self._chk(lib().pps_set_tiling(self.h, kz_stencil, kz_vector))
 
     def synchronize(self):
         self._chk(lib().pps_synchronize(self.h))
+
+
+def _fp(a):
+    return a.ctypes.data_as(_FP)
+
+
+class Network:
+    """Learned multilevel preconditioner (pps_nn_*): 14 BF16 MFMA convolutions
+    over 3 levels, trained on the GPU; attach it to a Context to use PPS_NN
+    stages (below an FCG stage)."""
+
+    N_ACTS = 13
+    LEVEL = (0, 0, 1, 1, 2, 2, 2, 2, 2, 2, 1, 1, 0)  # level of activation i
+
+    def __init__(self, seed=0):
+        self.h = _VP()
+        rc = lib().pps_nn_create(seed, ctypes.byref(self.h))
+        if rc != 0:
+            raise PPSError(rc, lib().pps_nn_last_error(None).decode())
+        st, eff = ctypes.c_longlong(), ctypes.c_longlong()
+        lib().pps_nn_num_params(self.h, ctypes.byref(st), ctypes.byref(eff))
+        self.n_params, self.n_effective = st.value, eff.value
+
+    def close(self):
+        if self.h:
+            lib().pps_nn_destroy(self.h)
+            self.h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc < 0:
+            raise PPSError(rc, lib().pps_nn_last_error(self.h).decode())
+        return rc
+
+    def get_params(self):
+        p = np.empty(self.n_params, np.float32)
+        self._chk(lib().pps_nn_get_params(self.h, _fp(p), p.size))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self._chk(lib().pps_nn_set_params(self.h, _fp(p), p.size))
+
+    @staticmethod
+    def _dims(x):
+        if x.ndim != 4:
+            raise ValueError("batch must be [B][n2][n1][n0]")
+        return (ctypes.c_int * 3)(x.shape[3], x.shape[2], x.shape[1])
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.empty_like(x)
+        self._chk(lib().pps_nn_forward(self.h, _fp(x), x.shape[0], self._dims(x), _fp(out)))
+        self._last = x.shape
+        return out
+
+    def activation(self, i):
+        B, n2, n1, n0 = self._last
+        d = [n2, n1, n0]
+        for _ in range(self.LEVEL[i]):
+            d = [(v + 1) // 2 for v in d]
+        out = np.empty((B, *d, 16), np.float32)
+        self._chk(lib().pps_nn_activation(self.h, i, _fp(out)))
+        return out
+
+    def loss_grad(self, x, g):
+        x = np.ascontiguousarray(x, np.float32)
+        g = np.ascontiguousarray(g, np.float64)
+        loss = ctypes.c_double()
+        grad = np.empty(self.n_params, np.float32)
+        self._chk(lib().pps_nn_loss_grad(self.h, _fp(x), _dp(g), x.shape[0], self._dims(x), ctypes.byref(loss),
+                                         _fp(grad)))
+        self._last = x.shape
+        return loss.value, grad
+
+    @staticmethod
+    def _cfg(n=64, nrhs=256, snapshots=16, kmax=32, seed=1, batch=16, steps=1000, lr=1e-3, lr_final=1e-5,
+             white_fraction=0.0):
+        c = CNNTrainCfg()
+        c.n, c.nrhs, c.snapshots, c.kmax, c.seed = n, nrhs, snapshots, kmax, seed
+        c.batch, c.steps, c.lr, c.lr_final = batch, steps, lr, lr_final
+        c.white_fraction = white_fraction
+        return c
+
+    def generate(self, **kw):
+        c = self._cfg(**kw)
+        pairs = ctypes.c_longlong()
+        self._chk(lib().pps_nn_generate(self.h, ctypes.byref(c), ctypes.byref(pairs)))
+        self._gen_n = c.n
+        return pairs.value
+
+    def dataset(self, index):
+        n = self._gen_n
+        r = np.empty((n, n, n), np.float32)
+        g = ctypes.c_double()
+        self._chk(lib().pps_nn_dataset(self.h, index, _fp(r), ctypes.byref(g)))
+        return r, g.value
+
+    def train(self, **kw):
+        c = self._cfg(**kw)
+        st = CNNTrainStats()
+        self._chk(lib().pps_nn_train(self.h, ctypes.byref(c), ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in CNNTrainStats._fields_}
+
+    def attach(self, ctx):
+        self._chk(lib().pps_nn_attach(ctx.h, self.h))
+        ctx._nn = self  # keep the network alive while the context uses it

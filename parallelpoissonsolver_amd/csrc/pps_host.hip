@@ -90,7 +90,8 @@ struct Ctx {
     int kz_vec = 32;  // z-chunk of vector launches
     int nt = 0;       // non-temporal stores (set when a field exceeds the caches)
     std::string err;
-    std::unique_ptr<NNPrecond> nn;
+    std::shared_ptr<NNPrecond> nn;  // learned preconditioner of PPS_NN stages (pps_nn_attach)
+    int fallback_switches = 0;      // FCG stages that switched to the fallback, this solve
 
     int fail(int code, const char* fmt, ...)
     {
@@ -171,7 +172,8 @@ static void scal_op(Ctx& C, Scal* S, int op, int idx = 0, double v = 0.0)
 }
 
 // initialise a scalar block from the host (stream ordered through pinned staging)
-static int scal_init(Ctx& C, Scal* S, double tol, int maxit, double jac, double* hist, int hist_cap)
+static int scal_init(Ctx& C, Scal* S, double tol, int maxit, double jac, double* hist, int hist_cap,
+                     int stall_window = 0, double stall_ratio = 0.0)
 {
     int rc = C.hip(hipStreamSynchronize(C.stream), "sync");
     if (rc) return rc;
@@ -182,6 +184,9 @@ static int scal_init(Ctx& C, Scal* S, double tol, int maxit, double jac, double*
     h.v[S_BETA] = 0.0;
     h.v[S_RHO0] = 1.0;
     h.v[S_PAP_OLD] = 1.0;
+    h.v[S_AUX1] = stall_ratio;
+    h.v[S_AUX2] = 1e300;
+    h.stall_window = stall_window > 0 ? stall_window : 0;
     h.maxit = maxit;
     h.hist = hist;
     h.hist_cap = hist_cap;
@@ -403,6 +408,14 @@ struct ChebStage : Stage {
 
 // ------------------------------------------------------------------ Krylov base
 
+// stall test of a flexible Krylov stage: configured on its PPS_NN
+// preconditioner stage if it has one, else on the stage itself
+static const pps_stage& stall_cfg(const Stage* k)
+{
+    if (k->pre && k->pre->cfg.kind == PPS_NN) return k->pre->cfg;
+    return k->cfg;
+}
+
 struct KrylovStage : Stage {
     using Stage::Stage;
     bool host_driven() const override { return true; }
@@ -620,7 +633,9 @@ struct FCGStage : KrylovStage {
         if (!rc) rc = C.get_work(k + ".A1", 64, A1);
         if (rc) return rc;
         for (Field f : {r, z, p0, p1, A0, A1}) C.zero(f);
-        rc = scal_init(C, S, cfg.tol, cfg.max_iter, 1.0, main ? C.hist_d : nullptr, main ? C.hist_cap : 0);
+        const pps_stage& sc = stall_cfg(this);
+        rc = scal_init(C, S, cfg.tol, cfg.max_iter, 1.0, main ? C.hist_d : nullptr, main ? C.hist_cap : 0,
+                       sc.stall_window, sc.stall_ratio);
         if (rc) return rc;
         int early = 0;
         rc = prologue(main, x, b, r, &early);
@@ -639,6 +654,10 @@ struct FCGStage : KrylovStage {
             return PPS_OK;
         });
         if (rc) return rc;
+        Scal h;
+        rc = scal_read(C, S, h);
+        if (rc) return rc;
+        C.fallback_switches += h.fallback;
         epilogue(main, x, b, r);
         return PPS_OK;
     }
@@ -656,14 +675,59 @@ struct FCGStage : KrylovStage {
 
 // --------------------------------------------------------- learned preconditioner
 
+// the fallback preconditioner (Jacobi) of an NN stage: runs only once the
+// calling FCG stage has set `fallback`
+template <typename T>
+__global__ void k_fallback(T* out, const T* in, Layout L, Region R, T c, const int* done, const int* fallback)
+{
+    if (*done || !*fallback) return;
+    const int n0 = R.hi[0] - R.lo[0], n1 = R.hi[1] - R.lo[1], n2 = R.hi[2] - R.lo[2];
+    const long long n = (long long)n0 * n1 * n2;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % n0);
+        const long long q = t / n0;
+        const long long o = L.at(R.lo[0] + i, R.lo[1] + (int)(q % n1), R.lo[2] + (int)(q / n1));
+        out[o] = c * in[o];
+    }
+}
+
+// a preconditioner is called with gate = &caller->done of the Krylov stage it
+// preconditions; that stage's scalar block holds ||r||^2 and the fallback flag
+static Scal* caller_of(const int* gate)
+{
+    return reinterpret_cast<Scal*>(reinterpret_cast<char*>(const_cast<int*>(gate)) - offsetof(Scal, done));
+}
+
 struct NNStage : Stage {
     using Stage::Stage;
     int apply(Field out, Field in, const int* gate) override
     {
-        if (!C.nn) return C.fail(PPS_ERR_STATE, "PPS_NN stage without a trained network");
+        if (!C.nn) return C.fail(PPS_ERR_STATE, "PPS_NN stage without a network (pps_nn_attach)");
+        if (!gate) return C.fail(PPS_ERR_STATE, "PPS_NN stage called outside a Krylov stage");
+        if (out.prec != in.prec) return C.fail(PPS_ERR_ARG, "PPS_NN: input and output precision differ");
         calls++;
         prec_iters++;
-        return C.nn->apply(C, out.p, out.prec, in.p, in.prec, gate);
+        Scal* caller = caller_of(gate);
+        const Layout& L = C.lay(in.prec);
+        NNFieldGeom g;
+        g.off = L.at(C.solver.lo[0], C.solver.lo[1], C.solver.lo[2]);
+        g.sy = L.px;
+        g.sz = L.plane;
+        g.n0 = C.solver.hi[0] - C.solver.lo[0];
+        g.n1 = C.solver.hi[1] - C.solver.lo[1];
+        g.n2 = C.solver.hi[2] - C.solver.lo[2];
+        if (C.nn->apply(C.stream, g, out.p, out.prec, in.p, in.prec, &caller->done, &caller->fallback,
+                        &caller->v[S_RR]))
+            return C.fail(PPS_ERR_HIP, "learned preconditioner: %s", C.nn->error());
+        const double* ds = C.grid.ds;
+        const double jac = 1.0 / (-2.0 * (1.0 / (ds[0] * ds[0]) + 1.0 / (ds[1] * ds[1]) + 1.0 / (ds[2] * ds[2])));
+        if (in.prec == 32)
+            hipLaunchKernelGGL(k_fallback<float>, dim3(2048), dim3(256), 0, C.stream, out.f(), in.f(), L, C.solver,
+                               (float)jac, &caller->done, &caller->fallback);
+        else
+            hipLaunchKernelGGL(k_fallback<double>, dim3(2048), dim3(256), 0, C.stream, out.d(), in.d(), L, C.solver, jac,
+                               &caller->done, &caller->fallback);
+        return PPS_OK;
     }
 };
 
@@ -738,6 +802,7 @@ struct IRStage : KrylovStage {
         if (rc) return rc;
         if (!flexible && h.iter > 0) launch_cg_final_x<float>(g, C.stream, d.f(), p[h.iter & 1].f(), SI);
         inner_total += h.iter;
+        if (flexible) C.fallback_switches += h.fallback;
         in->prec_iters += h.iter;
         in->calls++;
         return PPS_OK;
@@ -760,7 +825,9 @@ struct IRStage : KrylovStage {
         Scal* SI = pre->S;
         double c = 1.0;
         const bool pw = (pre->pre == nullptr) || pre->pre->pointwise(&c);
-        rc = scal_init(C, SI, pre->cfg.tol, pre->cfg.max_iter, pw ? c : 1.0, nullptr, 0);
+        const pps_stage& sc = stall_cfg(pre);
+        rc = scal_init(C, SI, pre->cfg.tol, pre->cfg.max_iter, pw ? c : 1.0, nullptr, 0, sc.stall_window,
+                       sc.stall_ratio);
         if (rc) return rc;
         C.ghost(x, true, S, nullptr);
         normalize(C, S, true, x, b);
@@ -1143,6 +1210,7 @@ int pps_solve(void* ctx, const pps_stage* chain, int nstages, pps_result* res)
     int rc = build_chain(*C, chain, nstages, st);
     if (rc) return rc;
     C->err.clear();
+    C->fallback_switches = 0;
     rc = C->hip(hipEventRecord(C->ev_t0, C->stream), "event");
     if (rc) return rc;
     rc = st[0]->solve_main();
@@ -1182,7 +1250,7 @@ int pps_solve(void* ctx, const pps_stage* chain, int nstages, pps_result* res)
         long long pi = 0;
         for (size_t i = 1; i < st.size(); ++i) pi += st[i]->prec_iters;
         res->precond_iterations = pi;
-        res->fallback_switches = C->nn ? C->nn->fallback_switches() : 0;
+        res->fallback_switches = C->fallback_switches;
         C->hist_len = res->history_len;
     }
     return PPS_OK;
@@ -1472,6 +1540,267 @@ int pps_rhs_spectrum(void* ctx, uint64_t seed, int K, int accumulate)
                         static_cast<double*>(dsy.p), static_cast<double*>(dsz.p), static_cast<double*>(t1.p),
                         static_cast<double*>(t2.p), accumulate, C->fields[PPS_FIELD_B].d());
     return C->hip(hipStreamSynchronize(C->stream), "rhs_spectrum");
+}
+
+
+// ------------------------------------------------------- learned preconditioner
+
+struct NNHandle {
+    std::shared_ptr<nn::Net> net;
+};
+
+static NNHandle* as_nn(void* p) { return static_cast<NNHandle*>(p); }
+static thread_local std::string g_nn_error;
+
+static int nn_fail(NNHandle* H, int code)
+{
+    g_nn_error = H ? H->net->error() : "null network handle";
+    return code;
+}
+
+// e = u* - x over the whole box
+__global__ void k_nn_sub(double* e, const double* us, const double* x, long long n)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) e[t] = us[t] - x[t];
+}
+
+// dense copy of the solver region, scaled: dst[i + n0 (j + n1 k)] = s * f(lo + (i,j,k))
+__global__ void k_nn_gather(const double* f, Layout L, Region R, double s, float* dst)
+{
+    const int n0 = R.hi[0] - R.lo[0], n1 = R.hi[1] - R.lo[1], n2 = R.hi[2] - R.lo[2];
+    const long long n = (long long)n0 * n1 * n2;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % n0);
+        const long long q = t / n0;
+        dst[t] = (float)(s * f[L.at(R.lo[0] + i, R.lo[1] + (int)(q % n1), R.lo[2] + (int)(q / n1))]);
+    }
+}
+
+const char* pps_nn_last_error(void* nn)
+{
+    (void)nn;
+    return g_nn_error.c_str();
+}
+
+int pps_nn_create(uint64_t seed, void** out)
+{
+    if (!out) return PPS_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_nn_error = "no GPU visible";
+        return PPS_ERR_NOGPU;
+    }
+    auto* H = new NNHandle;
+    H->net = std::make_shared<nn::Net>();
+    if (H->net->init(seed)) {
+        g_nn_error = H->net->error();
+        delete H;
+        return PPS_ERR_HIP;
+    }
+    *out = H;
+    return PPS_OK;
+}
+
+void pps_nn_destroy(void* nn) { delete as_nn(nn); }
+
+int pps_nn_num_params(void* nn, long long* stored, long long* effective)
+{
+    if (!nn) return PPS_ERR_ARG;
+    if (stored) *stored = nn::kParams;
+    if (effective) *effective = nn::Net::effective_params();
+    return PPS_OK;
+}
+
+int pps_nn_get_params(void* nn, float* host, long long n)
+{
+    NNHandle* H = as_nn(nn);
+    if (!H || !host) return PPS_ERR_ARG;
+    return H->net->get_params(host, n) ? nn_fail(H, PPS_ERR_ARG) : PPS_OK;
+}
+
+int pps_nn_set_params(void* nn, const float* host, long long n)
+{
+    NNHandle* H = as_nn(nn);
+    if (!H || !host) return PPS_ERR_ARG;
+    return H->net->set_params(host, n) ? nn_fail(H, PPS_ERR_ARG) : PPS_OK;
+}
+
+int pps_nn_forward(void* nn, const float* in, int batch, const int n[3], float* out)
+{
+    NNHandle* H = as_nn(nn);
+    if (!H || !in || !out || !n || batch < 1 || n[0] < 1 || n[1] < 1 || n[2] < 1) return PPS_ERR_ARG;
+    return H->net->forward_host(in, batch, n, out) ? nn_fail(H, PPS_ERR_HIP) : PPS_OK;
+}
+
+int pps_nn_activation(void* nn, int index, float* out)
+{
+    NNHandle* H = as_nn(nn);
+    if (!H || !out) return PPS_ERR_ARG;
+    return H->net->activation_host(index, out) ? nn_fail(H, PPS_ERR_ARG) : PPS_OK;
+}
+
+int pps_nn_loss_grad(void* nn, const float* in, const double* g, int batch, const int n[3], double* loss, float* grad)
+{
+    NNHandle* H = as_nn(nn);
+    if (!H || !in || !g || !loss || !grad || !n || batch < 1 || n[0] < 1 || n[1] < 1 || n[2] < 1) return PPS_ERR_ARG;
+    return H->net->loss_grad_host(in, g, batch, n, loss, grad) ? nn_fail(H, PPS_ERR_HIP) : PPS_OK;
+}
+
+int pps_nn_generate(void* nn, const pps_nn_train_cfg* cfg, long long* pairs)
+{
+    NNHandle* H = as_nn(nn);
+    if (!H || !cfg) return PPS_ERR_ARG;
+    const int n = cfg->n;
+    if (n < 4 || n > 512 || cfg->nrhs < 1 || cfg->snapshots < 1 || cfg->kmax < 1) {
+        g_nn_error = "pps_nn_generate: need 4 <= n <= 512, nrhs >= 1, snapshots >= 1, kmax >= 1";
+        return PPS_ERR_ARG;
+    }
+    pps_grid g{};
+    g.dim = 3;
+    for (int d = 0; d < 3; ++d) {
+        g.nranks[d] = 1;
+        g.npglobal[d] = n + 2;
+        g.ds[d] = 1.0 / (n + 1);
+        g.guards[d] = 1;
+    }
+    void* cx = nullptr;
+    int rc = pps_create(&g, &cx);
+    if (rc) {
+        g_nn_error = std::string("pps_nn_generate: ") + pps_last_error(nullptr);
+        return rc;
+    }
+    std::unique_ptr<void, void (*)(void*)> guard(cx, pps_destroy);
+    Ctx* C = as_ctx(cx);
+    // scratch fields are allocated on first use: the raw kernels below write T1/T2
+    for (int id : {PPS_FIELD_T0, PPS_FIELD_T1, PPS_FIELD_T2, PPS_FIELD_T3}) {
+        Field f;
+        if ((rc = field_of(*C, id, f))) {
+            g_nn_error = std::string("pps_nn_generate: ") + C->err;
+            return rc;
+        }
+    }
+    const int dims[3] = {n, n, n};
+    const long long npairs = (long long)cfg->nrhs * cfg->snapshots;
+    if (H->net->dataset_reserve(npairs, dims)) return nn_fail(H, PPS_ERR_NOMEM);
+    std::vector<double>& G = H->net->dataset_g();
+    const double h = 1.0 / (n + 1);
+    const double nvox = (double)n * n * n;
+    pps_stage cg[2]{};
+    cg[0].kind = PPS_CG;
+    cg[0].precision = 64;
+    cg[1].kind = PPS_NONE;
+    cg[1].precision = 64;
+    pps_result res{};
+    long long k = 0;
+    auto fail_ctx = [&](int code) {
+        g_nn_error = std::string("pps_nn_generate: ") + C->err;
+        return code;
+    };
+    for (int s = 0; s < cfg->nrhs; ++s) {
+        // converged solution u* of a k^-2 spectrum (or white-noise) right-hand side;
+        // the white ones are spread evenly over the sequence
+        const bool white = (int)((s + 1) * cfg->white_fraction) > (int)(s * cfg->white_fraction);
+        if ((rc = pps_fill(cx, PPS_FIELD_X, 0.0)) || (rc = pps_fill(cx, PPS_FIELD_B, 0.0)) ||
+            (rc = white ? pps_rhs_normal(cx, cfg->seed + (uint64_t)s) : pps_rhs_spectrum(cx, cfg->seed + (uint64_t)s, cfg->kmax, 0)) ||
+            (rc = pps_copy_field(cx, PPS_FIELD_T3, PPS_FIELD_B)))
+            return fail_ctx(rc);
+        cg[0].tol = 1e-11;
+        cg[0].max_iter = 100000;
+        if ((rc = pps_solve(cx, cg, 2, &res))) return fail_ctx(rc);
+        const int kstar = res.iterations;
+        if ((rc = pps_copy_field(cx, PPS_FIELD_T0, PPS_FIELD_X))) return fail_ctx(rc);
+        for (int j = 0; j < cfg->snapshots; ++j, ++k) {
+            // CG iterate x_j of the same problem from x = 0 (j = 0: x = 0, r = b)
+            const int it = j == 0 ? 0 : 1 + (int)((long long)(j - 1) * (9LL * kstar) / (10LL * std::max(1, cfg->snapshots - 1)));
+            if ((rc = pps_fill(cx, PPS_FIELD_X, 0.0)) || (rc = pps_copy_field(cx, PPS_FIELD_B, PPS_FIELD_T3)))
+                return fail_ctx(rc);
+            if (it > 0) {
+                cg[0].tol = 0.0;
+                cg[0].max_iter = it;
+                if ((rc = pps_solve(cx, cg, 2, &res))) return fail_ctx(rc);
+            }
+            double rn = 0.0, er = 0.0;
+            if ((rc = pps_residual(cx, PPS_FIELD_T1, PPS_FIELD_X, PPS_FIELD_B, &rn))) return fail_ctx(rc);
+            const long long tot = C->L64.total;
+            hipLaunchKernelGGL(k_nn_sub, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, C->stream,
+                               C->fields[PPS_FIELD_T2].d(), C->fields[PPS_FIELD_T0].d(), C->fields[PPS_FIELD_X].d(), tot);
+            if ((rc = pps_dot(cx, PPS_FIELD_T2, PPS_FIELD_T1, &er))) return fail_ctx(rc);
+            const double rho = rn / std::sqrt(nvox);
+            if (!(rho > 0.0)) {
+                g_nn_error = "pps_nn_generate: zero residual snapshot";
+                return PPS_ERR_STATE;
+            }
+            hipLaunchKernelGGL(k_nn_gather, dim3(1024), dim3(256), 0, C->stream, C->fields[PPS_FIELD_T1].d(), C->L64,
+                               C->solver, 1.0 / rho, H->net->dataset_sample(k));
+            // energy norm of the normalised error in index-space units: -(e.r) / (h^2 rho^2)
+            G[k] = -er / (h * h * rho * rho);
+        }
+    }
+    if ((rc = C->hip(hipStreamSynchronize(C->stream), "sync"))) return fail_ctx(rc);
+    if (pairs) *pairs = npairs;
+    return PPS_OK;
+}
+
+int pps_nn_dataset(void* nn, long long index, float* r, double* g)
+{
+    NNHandle* H = as_nn(nn);
+    if (!H || index < 0 || index >= H->net->dataset_size()) return PPS_ERR_ARG;
+    if (r && H->net->hip(hipMemcpy(r, H->net->dataset_sample(index), (size_t)H->net->sample_size() * sizeof(float),
+                                   hipMemcpyDeviceToHost),
+                         "download sample"))
+        return nn_fail(H, PPS_ERR_HIP);
+    if (g) *g = H->net->dataset_g()[index];
+    return PPS_OK;
+}
+
+int pps_nn_train(void* nn, const pps_nn_train_cfg* cfg, pps_nn_train_stats* st)
+{
+    NNHandle* H = as_nn(nn);
+    if (!H || !cfg) return PPS_ERR_ARG;
+    nn::TrainCfg c{cfg->batch, cfg->steps, (float)cfg->lr, (float)cfg->lr_final, cfg->seed};
+    nn::TrainStats s{};
+    if (H->net->train(c, &s)) return nn_fail(H, PPS_ERR_STATE);
+    if (st) {
+        st->pairs = H->net->dataset_size();
+        st->steps = s.steps;
+        st->seconds = s.seconds;
+        st->loss_first = s.loss_first;
+        st->loss_last = s.loss_last;
+    }
+    return PPS_OK;
+}
+
+int pps_nn_apply_field(void* ctx, int out, int in)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    if (!C->nn) return C->fail(PPS_ERR_STATE, "no network attached (pps_nn_attach)");
+    Field fo, fi;
+    int rc = field_of(*C, out, fo);
+    if (!rc) rc = field_of(*C, in, fi);
+    if (rc) return rc;
+    if (out == in) return C->fail(PPS_ERR_ARG, "pps_nn_apply_field: out and in must differ");
+    // a scratch scalar block stands in for the calling Krylov stage: ||in||^2, open gates
+    Scal* S = C->scal(C->nscal - 1);
+    rc = scal_init(*C, S, 0.0, 0, 1.0, nullptr, 0);
+    if (rc) return rc;
+    launch_dot<double>(C->gsolver(64), C->stream, C->rb, fi.d(), fi.d(), &S->v[S_RR], nullptr);
+    pps_stage st{};
+    st.kind = PPS_NN;
+    NNStage stage(*C, st, 1);
+    rc = stage.apply(fo, fi, &S->done);
+    if (rc) return rc;
+    return C->hip(hipStreamSynchronize(C->stream), "sync");
+}
+
+int pps_nn_attach(void* ctx, void* nn)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    C->nn = nn ? std::static_pointer_cast<NNPrecond>(as_nn(nn)->net) : nullptr;
+    return PPS_OK;
 }
 
 }  // extern "C"

@@ -520,6 +520,7 @@ struct OpResidual : Gate {
             S->v[S_ERR] = e;
             S->v[S_ERR_OP] = e;
             S->v[S_RZ] = t[1];
+            S->v[S_RR] = t[0];
             S->v[S_BETA] = 0.0;
             S->v[S_RHO0] = 1.0;
             S->v[S_ALPHA] = 0.0;  // no pending x update before the first iteration
@@ -1149,7 +1150,8 @@ struct OpFcgGamma : Gate {
     __device__ void finish(const double* t)
     {
         S->v[S_GAM] = t[0];
-        S->v[S_BETA] = S->iter == 0 ? 0.0 : t[0] / S->v[S_PAP_OLD];
+        // restart after a discarded step: the old direction may hold non-finite values
+        S->v[S_BETA] = (S->iter == 0 || S->restart) ? 0.0 : t[0] / S->v[S_PAP_OLD];
     }
 };
 
@@ -1175,7 +1177,8 @@ struct OpFcgP : Gate {
     Scal* S;
     T beta;
     __device__ void load() { beta = (T)S->v[S_BETA]; }
-    __device__ T in(long long c) const { return z[c] - beta * pold[c]; }
+    // beta = 0 takes z as it is, so a restart never multiplies a non-finite pold
+    __device__ T in(long long c) const { return beta == (T)0 ? z[c] : z[c] - beta * pold[c]; }
     __device__ void out(long long c, T vc, T Av, double* acc) const
     {
         p[c] = vc;
@@ -1186,7 +1189,17 @@ struct OpFcgP : Gate {
     __device__ void finish(const double* t)
     {
         S->v[S_PAP] = t[0];
-        S->v[S_ALPHA] = t[1] / t[0];
+        const double a = t[1] / t[0];
+        // a non-finite step (the preconditioner returned inf/NaN or a null
+        // direction): discard it, switch to the fallback preconditioner, restart
+        if (!isfinite(a)) {
+            S->restart = 1;
+            S->fallback = 1;
+            S->v[S_ALPHA] = 0.0;
+        } else {
+            S->restart = 0;
+            S->v[S_ALPHA] = a;
+        }
     }
 };
 
@@ -1213,21 +1226,36 @@ struct OpFcgUpdate : Gate {
     Scal* S;
     bool history;
     T alpha;
+    int skip;
     struct R { T x, p, r, ap; };
-    __device__ void load() { alpha = (T)S->v[S_ALPHA]; }
+    __device__ void load()
+    {
+        alpha = (T)S->v[S_ALPHA];
+        skip = S->restart;
+    }
     __device__ R fetch(long long c) const { return {x[c], p[c], r[c], Ap[c]}; }
     __device__ void apply(long long c, const R& v, int, int, int, double* acc) const
     {
-        x[c] = v.x + alpha * v.p;
-        const T rv = v.r - alpha * v.ap;
-        r[c] = rv;
+        T rv = v.r;
+        if (!skip) {
+            x[c] = v.x + alpha * v.p;
+            rv = v.r - alpha * v.ap;
+            r[c] = rv;
+        }
         acc[0] += (double)rv * (double)rv;
     }
     __device__ void finish(const double* t)
     {
         S->v[S_PAP_OLD] = S->v[S_PAP];
         S->v[S_RR] = t[0];
-        finish_iteration(S, sqrt(t[0]), history);
+        const double err = sqrt(t[0]);
+        finish_iteration(S, err, history);
+        // stall monitor: every stall_window iterations the residual must have
+        // dropped by the factor v[S_AUX1], else the fallback preconditioner takes over
+        if (S->stall_window > 0 && !S->fallback && S->iter % S->stall_window == 0) {
+            if (err > S->v[S_AUX1] * S->v[S_AUX2]) S->fallback = 1;
+            S->v[S_AUX2] = err;
+        }
     }
 };
 
@@ -1342,9 +1370,13 @@ struct OpIrInit : Gate {
         SI->v[S_ERR0] = e;
         SI->v[S_ERR] = e;
         SI->v[S_RZ] = t[1];
+        SI->v[S_RR] = t[0];
         SI->v[S_BETA] = 0.0;
         SI->v[S_PAP_OLD] = 1.0;
         SI->v[S_ALPHA] = 0.0;
+        SI->v[S_AUX2] = 1e300;
+        SI->fallback = 0;
+        SI->restart = 0;
         SI->iter = 0;
         SI->done = e < SI->v[S_TOL] ? 2 : 0;
     }

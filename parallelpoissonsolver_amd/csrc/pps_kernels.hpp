@@ -56,7 +56,9 @@ struct alignas(256) Scal {
     int hist_cap;    // capacity of hist (entries), 0 = no history
     double* hist;    // residual history (errorFromIterationHistory_)
     int outer;       // IR outer steps
-    int pad[3];
+    int fallback;    // FCG: 1 once the plain preconditioner replaced the learned one
+    int restart;     // FCG: 1 = discard this iteration's direction (non-finite step) and restart
+    int stall_window;  // FCG: iterations per stall check (0 = off)
 };
 
 // Single-pass deterministic grid reduction workspace: per-block partial sums

@@ -1,2 +1,1065 @@
-// pps_nn.hip -- learned multilevel preconditioner (placeholder translation unit)
+// pps_nn.hip -- BF16 MFMA convolutions of the learned preconditioner, their
+// backward passes, the training loss, Adam, and the network driver.
+//
+// Convolution = implicit GEMM on v_mfma_f32_16x16x32_bf16:
+//   A (16 x 32)  weights: row = output channel, k = (tap pair, input channel)
+//   B (32 x 16)  inputs:  column = one voxel of a 16-voxel x-row
+//   C (16 x 16)  lane l, register r -> channel 4*(l/16)+r of voxel l%16
+// A lane holds row l%16 and the 8 consecutive k of block l/16, B likewise,
+// so one 16-byte LDS read per lane feeds one MFMA: tap 2s + (l/16)/2,
+// channels 8*((l/16)&1) .. +7 of voxel l%16.  14 steps cover 28 taps
+// (the 28th has zero weights).  All 14 weight operands stay in registers.
+//
+// A block stages 18 x 18 voxel input planes (16 x 16 output tile + halo) in
+// LDS and marches along z with a ring of 4 planes, one barrier per plane.
+// Each of the 4 wavefronts computes 4 x-rows of 16 voxels per plane.
 #include "pps_nn.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <random>
+
+namespace pps {
+namespace nn {
+
+const LayerDef kNet[kLayers] = {
+    {1, 16, 0, IN_FIELD, -1, -1, 1},   // lift
+    {16, 16, 0, IN_PLAIN, 0, -1, 1},   // a1
+    {16, 16, 1, IN_POOL, 1, -1, 1},    // b0
+    {16, 16, 1, IN_PLAIN, 2, -1, 1},   // b1
+    {16, 16, 2, IN_POOL, 3, -1, 1},    // c0
+    {16, 16, 2, IN_PLAIN, 4, -1, 1},   // c1
+    {16, 16, 2, IN_PLAIN, 5, -1, 1},   // c2
+    {16, 16, 2, IN_PLAIN, 6, -1, 1},   // c3
+    {16, 16, 2, IN_PLAIN, 7, -1, 1},   // c4
+    {16, 16, 2, IN_PLAIN, 8, -1, 1},   // c5
+    {16, 16, 1, IN_UPADD, 9, 3, 1},    // e0
+    {16, 16, 1, IN_PLAIN, 10, -1, 1},  // e1
+    {16, 16, 0, IN_UPADD, 11, 1, 1},   // f0
+    {16, 1, 0, IN_PLAIN, 12, -1, 0},   // out
+};
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct ConvArgs {
+    int mode_in, mode_out;
+    ActRef in, in2;   // in2: skip (UPADD) or ReLU mask (MASK); POOL reads the finer level through `in`
+    FieldRef fin, fout;
+    ActRef out;       // output activation; its extent is the level's (p unused for OUT_FIELD)
+    const uint16_t* wl;
+    const float* bias;
+    const int* gate;
+    const int* gate2;
+};
+
+struct DwArgs {
+    int mode_in;      // the layer's input, as in the forward pass
+    ActRef in, in2;
+    FieldRef fin;
+    int dz_mode;      // 0: dZ = dY masked by Y > 0; 1: dZ = dY; 2: dZ = one-channel FP32 field
+    ActRef dy, y;
+    FieldRef fdy;
+    int B, D, H, W;   // extent of the layer's level
+    float* part;      // per-block partial sums, kLayerParams each
+};
+
+constexpr int kTW = 16, kTHt = 16;          // output tile per plane
+constexpr int kLW = kTW + 2, kLH = kTHt + 2;  // staged input plane with halo
+constexpr int kPlane = kLH * 2 * kLW;        // 16-byte chunks per staged plane: [y][chunk][x]
+constexpr int kZPlane = kTHt * 2 * kTW;      // chunks of one staged gradient plane
+
+// ------------------------------------------------------------ BF16 helpers
+
+__host__ __device__ __forceinline__ unsigned f2bf(float f)
+{
+    unsigned u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return ((u >> 16) | 0x40u) & 0xffffu;  // quiet NaN
+    return ((u + 0x7fffu + ((u >> 16) & 1u)) >> 16) & 0xffffu;                  // round to nearest even
+}
+__device__ __forceinline__ float bf2f(unsigned h) { return __uint_as_float(h << 16); }
+__device__ __forceinline__ void unpack8(const u32x4 c, float* f)
+{
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        f[2 * i] = __uint_as_float(c[i] << 16);
+        f[2 * i + 1] = __uint_as_float(c[i] & 0xffff0000u);
+    }
+}
+__device__ __forceinline__ u32x4 pack8(const float* f)
+{
+    u32x4 c;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c[i] = f2bf(f[2 * i]) | (f2bf(f[2 * i + 1]) << 16);
+    return c;
+}
+__device__ __forceinline__ bool bf_pos(unsigned h) { return (h & 0x8000u) == 0 && (h & 0x7fffu) != 0; }
+
+__device__ __forceinline__ long long vox(const ActRef& a, int b, int x, int y, int z)
+{
+    return ((((long long)b * a.D + z) * a.H + y) * a.W + x) * kC;
+}
+
+__device__ __forceinline__ float field_scale(const FieldRef& f)
+{
+    if (!f.scale_sum) return f.scale;
+    const double ms = *f.scale_sum / f.scale_count;
+    return ms > 0.0 ? (float)(1.0 / sqrt(ms)) : 1.0f;
+}
+__device__ __forceinline__ long long fidx(const FieldRef& f, int b, int x, int y, int z)
+{
+    return f.off + x * f.sx + y * f.sy + z * f.sz + b * f.sb;
+}
+
+// one 16-byte chunk (8 channels) of the convolution input at voxel (x,y,z)
+// of a level with extent D x H x W; zero outside (Dirichlet padding)
+template <int MI>
+__device__ __forceinline__ u32x4 load_chunk(const ActRef& in, const ActRef& in2, const FieldRef& f, float fscale,
+                                            int D, int H, int W, int b, int x, int y, int z, int ch)
+{
+    u32x4 r = {0u, 0u, 0u, 0u};
+    if (x < 0 || y < 0 || z < 0 || x >= W || y >= H || z >= D) return r;
+    if constexpr (MI == IN_PLAIN) {
+        return *(const u32x4*)(in.p + vox(in, b, x, y, z) + ch * 8);
+    } else if constexpr (MI == IN_MASK) {
+        const long long o = vox(in, b, x, y, z) + ch * 8;
+        const u32x4 v = *(const u32x4*)(in.p + o);
+        const u32x4 m = *(const u32x4*)(in2.p + o);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            r[i] = (bf_pos(m[i] & 0xffffu) ? (v[i] & 0xffffu) : 0u) | (bf_pos(m[i] >> 16) ? (v[i] & 0xffff0000u) : 0u);
+        return r;
+    } else if constexpr (MI == IN_POOL) {
+        float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int fx = 2 * x + (c & 1), fy = 2 * y + ((c >> 1) & 1), fz = 2 * z + (c >> 2);
+            if (fx < in.W && fy < in.H && fz < in.D) {
+                float t[8];
+                unpack8(*(const u32x4*)(in.p + vox(in, b, fx, fy, fz) + ch * 8), t);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) acc[e] += t[e];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc[e] *= 0.125f;
+        return pack8(acc);
+    } else if constexpr (MI == IN_UPADD) {
+        float a[8], s[8];
+        unpack8(*(const u32x4*)(in.p + vox(in, b, x >> 1, y >> 1, z >> 1) + ch * 8), a);
+        unpack8(*(const u32x4*)(in2.p + vox(in2, b, x, y, z) + ch * 8), s);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a[e] += s[e];
+        return pack8(a);
+    } else {  // IN_FIELD: one channel
+        if (ch) return r;
+        const long long i = fidx(f, b, x, y, z);
+        const float v = f.is64 ? (float)(((const double*)f.p)[i] * (double)fscale) : ((const float*)f.p)[i] * fscale;
+        r[0] = f2bf(v);
+        return r;
+    }
+}
+
+// ------------------------------------------------------------ convolution
+
+template <int MI, int MO>
+__global__ __launch_bounds__(256) void k_conv(ConvArgs a, int zc)
+{
+    if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
+    __shared__ u32x4 lds[4 * kPlane];
+    const int D = a.out.D, H = a.out.H, W = a.out.W;
+    const int nzc = (D + zc - 1) / zc;
+    const int b = blockIdx.z / nzc;
+    const int z0 = (blockIdx.z - b * nzc) * zc;
+    const int z1 = min(z0 + zc, D);
+    const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTHt;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = lane >> 4, n = lane & 15;
+    float fscale = 1.0f;
+    if constexpr (MI == IN_FIELD) fscale = field_scale(a.fin);
+
+    bf16x8 wr[kSteps];
+    const u32x4* wl = (const u32x4*)a.wl;
+#pragma unroll
+    for (int s = 0; s < kSteps; ++s) wr[s] = __builtin_bit_cast(bf16x8, wl[s * 64 + lane]);
+    float bias[4] = {0.f, 0.f, 0.f, 0.f};
+    if (MO != OUT_LINEAR && a.bias) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bias[r] = a.bias[4 * j + r];
+    }
+
+    auto stage = [&](int zi) {
+        u32x4* dst = lds + (zi & 3) * kPlane;
+        for (int t = tid; t < kPlane; t += 256) {
+            const int yy = t / (2 * kLW);
+            const int rem = t - yy * (2 * kLW);
+            const int xx = rem >> 1, ch = rem & 1;
+            dst[(yy * 2 + ch) * kLW + xx] =
+                load_chunk<MI>(a.in, a.in2, a.fin, fscale, D, H, W, b, x0 - 1 + xx, y0 - 1 + yy, zi, ch);
+        }
+    };
+    stage(z0 - 1);
+    stage(z0);
+    const int x = x0 + n;
+    for (int z = z0; z < z1; ++z) {
+        stage(z + 1);
+        __syncthreads();
+        f32x4 acc[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < kSteps; ++s) {
+            const int tap = min(2 * s + (j >> 1), 26);
+            const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+            const u32x4* pl = lds + ((z + dz - 1) & 3) * kPlane + (j & 1) * kLW + n + dx;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const u32x4 v = pl[(wave * 4 + i + dy) * 2 * kLW];
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[s], __builtin_bit_cast(bf16x8, v), acc[i], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int y = y0 + wave * 4 + i;
+            if (x >= W || y >= H) continue;
+            if constexpr (MO == OUT_FIELD) {
+                if (j == 0) {
+                    const long long o = fidx(a.fout, b, x, y, z);
+                    const float v = acc[i][0] + bias[0];
+                    if (a.fout.is64) ((double*)a.fout.p)[o] = (double)v;
+                    else ((float*)a.fout.p)[o] = v;
+                }
+            } else {
+                float v[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    v[r] = acc[i][r] + bias[r];
+                    if (MO == OUT_RELU) v[r] = fmaxf(v[r], 0.0f);
+                }
+                u32x2 o;
+                o[0] = f2bf(v[0]) | (f2bf(v[1]) << 16);
+                o[1] = f2bf(v[2]) | (f2bf(v[3]) << 16);
+                *(u32x2*)(a.out.p + vox(a.out, b, x, y, z) + 4 * j) = o;
+            }
+        }
+    }
+}
+
+// z-chunk: enough blocks to fill the chip, at most one chunk per 16 planes
+static int pick_zc(int D, int H, int W, int B)
+{
+    const long long tiles = (long long)((W + kTW - 1) / kTW) * ((H + kTHt - 1) / kTHt) * B;
+    int zc = D;
+    while (zc > 16 && tiles * ((D + zc - 1) / zc) < 8192) zc = (zc + 1) / 2;
+    return std::max(zc, 1);
+}
+static dim3 conv_grid(int D, int H, int W, int B, int zc)
+{
+    return dim3((unsigned)((W + kTW - 1) / kTW), (unsigned)((H + kTHt - 1) / kTHt), (unsigned)(B * ((D + zc - 1) / zc)));
+}
+
+template <int MI>
+static void conv_mo(const ConvArgs& a, hipStream_t s, dim3 grid, int zc)
+{
+    switch (a.mode_out) {
+    case OUT_RELU: hipLaunchKernelGGL((k_conv<MI, OUT_RELU>), grid, dim3(256), 0, s, a, zc); break;
+    case OUT_LINEAR: hipLaunchKernelGGL((k_conv<MI, OUT_LINEAR>), grid, dim3(256), 0, s, a, zc); break;
+    default: hipLaunchKernelGGL((k_conv<MI, OUT_FIELD>), grid, dim3(256), 0, s, a, zc); break;
+    }
+}
+
+void launch_conv(const ConvArgs& a, hipStream_t s)
+{
+    const int zc = pick_zc(a.out.D, a.out.H, a.out.W, a.out.B);
+    const dim3 grid = conv_grid(a.out.D, a.out.H, a.out.W, a.out.B, zc);
+    switch (a.mode_in) {
+    case IN_PLAIN: conv_mo<IN_PLAIN>(a, s, grid, zc); break;
+    case IN_POOL: conv_mo<IN_POOL>(a, s, grid, zc); break;
+    case IN_UPADD: conv_mo<IN_UPADD>(a, s, grid, zc); break;
+    case IN_MASK: conv_mo<IN_MASK>(a, s, grid, zc); break;
+    default: conv_mo<IN_FIELD>(a, s, grid, zc); break;
+    }
+}
+
+// ------------------------------------------------------------ weight gradient
+//
+// dW[tap][ci][co] = sum_v X[v + off(tap)][ci] * dZ[v][co]: per tap one MFMA
+// per 32 voxels (two x-rows of the tile) with A = X^T (row = ci, k = voxel)
+// and B = dZ (k = voxel, column = co).  Wavefront w owns taps 7w .. 7w+6;
+// wavefront 3's spare slot sums the bias gradient.  Each block writes its
+// partial sums; a second kernel adds the partials in a fixed order.
+
+template <int MI, int MZ>
+__global__ __launch_bounds__(256) void k_dw(DwArgs a, int zc)
+{
+    __shared__ u32x4 lx[4 * kPlane];
+    __shared__ u32x4 lz[2 * kZPlane];
+    const int D = a.D, H = a.H, W = a.W;
+    const int nzc = (D + zc - 1) / zc;
+    const int b = blockIdx.z / nzc;
+    const int z0 = (blockIdx.z - b * nzc) * zc;
+    const int z1 = min(z0 + zc, D);
+    const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTHt;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = lane >> 4, n = lane & 15;
+    float fscale = 1.0f;
+    if constexpr (MI == IN_FIELD) fscale = field_scale(a.fin);
+
+    auto stage_x = [&](int zi) {
+        u32x4* dst = lx + (zi & 3) * kPlane;
+        for (int t = tid; t < kPlane; t += 256) {
+            const int yy = t / (2 * kLW);
+            const int rem = t - yy * (2 * kLW);
+            const int xx = rem >> 1, ch = rem & 1;
+            dst[(yy * 2 + ch) * kLW + xx] =
+                load_chunk<MI>(a.in, a.in2, a.fin, fscale, D, H, W, b, x0 - 1 + xx, y0 - 1 + yy, zi, ch);
+        }
+    };
+    constexpr int MZI = MZ == 0 ? IN_MASK : (MZ == 1 ? IN_PLAIN : IN_FIELD);
+    auto stage_z = [&](int zi) {
+        u32x4* dst = lz + (zi & 1) * kZPlane;
+        for (int t = tid; t < kZPlane; t += 256) {
+            const int yy = t >> 5, rem = t & 31;
+            const int xx = rem >> 1, ch = rem & 1;
+            dst[(yy * 2 + ch) * kTW + xx] = load_chunk<MZI>(a.dy, a.y, a.fdy, 1.0f, D, H, W, b, x0 + xx, y0 + yy, zi, ch);
+        }
+    };
+
+    f32x4 acc[7];
+#pragma unroll
+    for (int t = 0; t < 7; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float db = 0.0f;
+    const int ntap = wave == 3 ? 6 : 7;
+
+    stage_x(z0 - 1);
+    stage_x(z0);
+    for (int z = z0; z < z1; ++z) {
+        stage_x(z + 1);
+        stage_z(z);
+        __syncthreads();
+        const uint16_t* X = (const uint16_t*)lx;
+        const uint16_t* Zp = (const uint16_t*)(lz + (z & 1) * kZPlane);
+        for (int rp = 0; rp < kTHt / 2; ++rp) {
+            const int row = 2 * rp + (j >> 1), xb = 8 * (j & 1);
+            const uint16_t* zb = Zp + ((row * 2 + (n >> 3)) * kTW + xb) * 8 + (n & 7);
+            u32x4 bw;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) bw[e] = (unsigned)zb[16 * e] | ((unsigned)zb[16 * e + 8] << 16);
+            if (wave == 3) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) db += bf2f(bw[e] & 0xffffu) + __uint_as_float(bw[e] & 0xffff0000u);
+            }
+            const bf16x8 Bv = __builtin_bit_cast(bf16x8, bw);
+#pragma unroll
+            for (int t = 0; t < 7; ++t) {
+                if (t >= ntap) break;
+                const int tap = 7 * wave + t;
+                const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+                const uint16_t* xp =
+                    X + ((z + dz - 1) & 3) * kPlane * 8 + (((row + dy) * 2 + (n >> 3)) * kLW + xb + dx) * 8 + (n & 7);
+                u32x4 aw;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) aw[e] = (unsigned)xp[16 * e] | ((unsigned)xp[16 * e + 8] << 16);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, aw), Bv, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    const long long blk = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    float* part = a.part + blk * kLayerParams;
+#pragma unroll
+    for (int t = 0; t < 7; ++t) {
+        if (t >= ntap) break;
+        const int tap = 7 * wave + t;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) part[tap * 256 + (4 * j + r) * 16 + n] = acc[t][r];
+    }
+    if (wave == 3) {
+        db += __shfl_xor(db, 16);
+        db += __shfl_xor(db, 32);
+        if (j == 0) part[kTaps * 256 + n] = db;
+    }
+}
+
+int dw_blocks(const DwArgs& a)
+{
+    const int zc = pick_zc(a.D, a.H, a.W, a.B);
+    const dim3 g = conv_grid(a.D, a.H, a.W, a.B, zc);
+    return (int)(g.x * g.y * g.z);
+}
+
+template <int MI>
+static void dw_mz(const DwArgs& a, hipStream_t s, dim3 grid, int zc)
+{
+    switch (a.dz_mode) {
+    case 0: hipLaunchKernelGGL((k_dw<MI, 0>), grid, dim3(256), 0, s, a, zc); break;
+    case 1: hipLaunchKernelGGL((k_dw<MI, 1>), grid, dim3(256), 0, s, a, zc); break;
+    default: hipLaunchKernelGGL((k_dw<MI, 2>), grid, dim3(256), 0, s, a, zc); break;
+    }
+}
+
+void launch_dw(const DwArgs& a, hipStream_t s)
+{
+    const int zc = pick_zc(a.D, a.H, a.W, a.B);
+    const dim3 grid = conv_grid(a.D, a.H, a.W, a.B, zc);
+    switch (a.mode_in) {
+    case IN_PLAIN: dw_mz<IN_PLAIN>(a, s, grid, zc); break;
+    case IN_POOL: dw_mz<IN_POOL>(a, s, grid, zc); break;
+    case IN_UPADD: dw_mz<IN_UPADD>(a, s, grid, zc); break;
+    default: dw_mz<IN_FIELD>(a, s, grid, zc); break;
+    }
+}
+
+__global__ void k_dw_reduce(const float* __restrict__ part, int nblocks, float* __restrict__ g)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kLayerParams) return;
+    float s = 0.0f;
+    for (int b = 0; b < nblocks; ++b) s += part[(long long)b * kLayerParams + i];
+    g[i] = s;
+}
+
+void launch_dw_reduce(const float* part, int nblocks, float* g, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dw_reduce, dim3((kLayerParams + 255) / 256), dim3(256), 0, s, part, nblocks, g);
+}
+
+// ------------------------------------------------ gradient routing of pool / up
+
+// dfine[v] (+)= dcoarse[parent(v)] / 8
+__global__ void k_pool_back(ActRef dc, ActRef df, int accumulate)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nch = (long long)df.B * df.D * df.H * df.W * 2;
+    if (t >= nch) return;
+    const int ch = (int)(t & 1);
+    long long v = t >> 1;
+    const int x = (int)(v % df.W);
+    v /= df.W;
+    const int y = (int)(v % df.H);
+    v /= df.H;
+    const int z = (int)(v % df.D);
+    const int b = (int)(v / df.D);
+    float c[8];
+    unpack8(*(const u32x4*)(dc.p + vox(dc, b, x >> 1, y >> 1, z >> 1) + ch * 8), c);
+    u32x4* dst = (u32x4*)(df.p + vox(df, b, x, y, z) + ch * 8);
+    float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (accumulate) unpack8(*dst, f);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) f[e] += 0.125f * c[e];
+    *dst = pack8(f);
+}
+
+void launch_pool_back(const ActRef& dcoarse, const ActRef& dfine, int accumulate, hipStream_t s)
+{
+    const long long nch = (long long)dfine.B * dfine.D * dfine.H * dfine.W * 2;
+    hipLaunchKernelGGL(k_pool_back, dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, s, dcoarse, dfine, accumulate);
+}
+
+// dskip[v] (+)= dx[v] for the 8 children v of a coarse voxel; dcoarse = their sum
+__global__ void k_upadd_back(ActRef dx, ActRef dskip, int accumulate_skip, ActRef dco)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nch = (long long)dco.B * dco.D * dco.H * dco.W * 2;
+    if (t >= nch) return;
+    const int ch = (int)(t & 1);
+    long long v = t >> 1;
+    const int x = (int)(v % dco.W);
+    v /= dco.W;
+    const int y = (int)(v % dco.H);
+    v /= dco.H;
+    const int z = (int)(v % dco.D);
+    const int b = (int)(v / dco.D);
+    float sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int fx = 2 * x + (c & 1), fy = 2 * y + ((c >> 1) & 1), fz = 2 * z + (c >> 2);
+        if (fx >= dx.W || fy >= dx.H || fz >= dx.D) continue;
+        const long long o = vox(dx, b, fx, fy, fz) + ch * 8;
+        float g[8];
+        unpack8(*(const u32x4*)(dx.p + o), g);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) sum[e] += g[e];
+        if (accumulate_skip) {
+            float s[8];
+            unpack8(*(const u32x4*)(dskip.p + o), s);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) s[e] += g[e];
+            *(u32x4*)(dskip.p + o) = pack8(s);
+        } else {
+            *(u32x4*)(dskip.p + o) = *(const u32x4*)(dx.p + o);
+        }
+    }
+    *(u32x4*)(dco.p + vox(dco, b, x, y, z) + ch * 8) = pack8(sum);
+}
+
+void launch_upadd_back(const ActRef& dx, const ActRef& dskip, int accumulate_skip, const ActRef& dcoarse, hipStream_t s)
+{
+    const long long nch = (long long)dcoarse.B * dcoarse.D * dcoarse.H * dcoarse.W * 2;
+    hipLaunchKernelGGL(k_upadd_back, dim3((unsigned)((nch + 255) / 256)), dim3(256), 0, s, dx, dskip, accumulate_skip,
+                       dcoarse);
+}
+
+// ------------------------------------------------------------ weights, Adam
+
+// lane-arranged A operands: wl[s][lane][e] = W'[tap = 2s + (lane/16)/2][k = 8((lane/16)&1) + e][m = lane%16]
+// forward:        W'[tap][k][m] = W[tap][ci = k][co = m]
+// backward data:  W'[tap][k][m] = W[26 - tap][ci = m][co = k]   (flipped, transposed)
+__global__ void k_pack(const float* __restrict__ W, int cin, int cout, int transposed, uint16_t* __restrict__ wl)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= kPackedHalves) return;
+    const int e = t & 7, lane = (t >> 3) & 63, s = t >> 9;
+    const int j = lane >> 4, m = lane & 15;
+    const int tap = 2 * s + (j >> 1), k = 8 * (j & 1) + e;
+    float v = 0.0f;
+    if (tap < kTaps) {
+        if (!transposed) {
+            if (k < cin && m < cout) v = W[tap * 256 + k * 16 + m];
+        } else {
+            if (m < cin && k < cout) v = W[(kTaps - 1 - tap) * 256 + m * 16 + k];
+        }
+    }
+    wl[t] = (uint16_t)f2bf(v);
+}
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       int n, float lr, float b1, float b2, float eps, float bc1, float bc2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+// ------------------------------------------------------------ training loss
+//
+// Per sample s (residual r normalised to unit RMS, network output y):
+//   loss_s = 1 - (y.r)^2 / ((y.Ky) G_s),   K = -(index-space 7-point Laplacian)
+// which is 1 - cos^2 of the angle between y and the error e in the energy
+// norm of K, with G_s = e.Ke (both sides in the units of the normalised r).
+// It is invariant to the scale of y, as FCG is.  Its gradient:
+//   dloss_s/dy = 2 a^2/(c^2 G) Ky - 2 a/(c G) r,   a = y.r, c = y.Ky
+// Batch loss = mean over samples.
+
+__device__ __forceinline__ double block_sum_d(double v, double* sh)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_loss1(const float* __restrict__ y, const float* __restrict__ r,
+                                               float* __restrict__ ky, int n0, int n1, int n2, int per_block,
+                                               double* __restrict__ part)
+{
+    __shared__ double sh[256];
+    const int b = blockIdx.y;
+    const long long nv = (long long)n0 * n1 * n2;
+    const float* yb = y + b * nv;
+    const float* rb = r + b * nv;
+    float* kb = ky + b * nv;
+    double sa = 0.0, sc = 0.0;
+    const long long v0 = (long long)blockIdx.x * per_block;
+    const long long v1 = min(v0 + per_block, nv);
+    for (long long v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
+        const int x = (int)(v % n0);
+        const int yy = (int)((v / n0) % n1);
+        const int z = (int)(v / ((long long)n0 * n1));
+        const float c = yb[v];
+        float s = 6.0f * c;
+        if (x > 0) s -= yb[v - 1];
+        if (x < n0 - 1) s -= yb[v + 1];
+        if (yy > 0) s -= yb[v - n0];
+        if (yy < n1 - 1) s -= yb[v + n0];
+        if (z > 0) s -= yb[v - (long long)n0 * n1];
+        if (z < n2 - 1) s -= yb[v + (long long)n0 * n1];
+        kb[v] = s;
+        sa += (double)c * (double)rb[v];
+        sc += (double)c * (double)s;
+    }
+    sa = block_sum_d(sa, sh);
+    sc = block_sum_d(sc, sh);
+    if (threadIdx.x == 0) {
+        part[((long long)b * gridDim.x + blockIdx.x) * 2] = sa;
+        part[((long long)b * gridDim.x + blockIdx.x) * 2 + 1] = sc;
+    }
+}
+
+__global__ void k_loss2(const double* __restrict__ part, int nblk, const double* __restrict__ G, int B,
+                        double* __restrict__ coef, double* __restrict__ lossv)
+{
+    const int b = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    double a = 0.0, c = 0.0;
+    for (int i = 0; i < nblk; ++i) {
+        a += part[((long long)b * nblk + i) * 2];
+        c += part[((long long)b * nblk + i) * 2 + 1];
+    }
+    c = fmax(c, 1e-300);
+    const double g = G[b];
+    lossv[b] = 1.0 - a * a / (c * g);
+    coef[2 * b] = 2.0 * a * a / (c * c * g * B);
+    coef[2 * b + 1] = -2.0 * a / (c * g * B);
+}
+
+__global__ void k_loss3(const float* __restrict__ ky, const float* __restrict__ r, const double* __restrict__ coef,
+                        long long nv, int B, float* __restrict__ dy)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv * B) return;
+    const int b = (int)(t / nv);
+    dy[t] = (float)(coef[2 * b] * (double)ky[t] + coef[2 * b + 1] * (double)r[t]);
+}
+
+// ================================================================ Net
+
+int Net::fail(const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    err_ = buf;
+    return -1;
+}
+
+int Net::hip(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return 0;
+    fail("%s: %s", what, hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? -3 : -2;
+}
+
+Net::~Net()
+{
+    release(inf_);
+    release(trn_);
+    for (void* p : {(void*)params_, (void*)grads_, (void*)adam_m_, (void*)adam_v_, (void*)wl_, (void*)wlt_, (void*)data_r_})
+        if (p) (void)hipFree(p);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+long long Net::effective_params()
+{
+    long long s = 0;
+    for (const LayerDef& d : kNet) s += (long long)kTaps * d.cin * d.cout + d.cout;
+    return s;
+}
+
+int Net::init(unsigned long long seed)
+{
+    int rc = hip(hipStreamCreate(&stream_), "hipStreamCreate");
+    if (!rc) rc = hip(hipMalloc(&params_, kParams * sizeof(float)), "hipMalloc params");
+    if (!rc) rc = hip(hipMalloc(&grads_, kParams * sizeof(float)), "hipMalloc grads");
+    if (!rc) rc = hip(hipMalloc(&adam_m_, kParams * sizeof(float)), "hipMalloc adam");
+    if (!rc) rc = hip(hipMalloc(&adam_v_, kParams * sizeof(float)), "hipMalloc adam");
+    if (!rc) rc = hip(hipMalloc(&wl_, (size_t)kLayers * kPackedHalves * 2), "hipMalloc weights");
+    if (!rc) rc = hip(hipMalloc(&wlt_, (size_t)kLayers * kPackedHalves * 2), "hipMalloc weights");
+    if (rc) return rc;
+    // He initialisation (std = sqrt(2 / fan_in)) of the ReLU layers, sqrt(1 / fan_in) for the output
+    std::vector<float> h(kParams, 0.0f);
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    for (int l = 0; l < kLayers; ++l) {
+        const LayerDef& d = kNet[l];
+        const double sd = std::sqrt((d.relu ? 2.0 : 1.0) / (kTaps * d.cin));
+        for (int t = 0; t < kTaps; ++t)
+            for (int ci = 0; ci < d.cin; ++ci)
+                for (int co = 0; co < d.cout; ++co) h[(size_t)l * kLayerParams + t * 256 + ci * 16 + co] = (float)(sd * nd(rng));
+    }
+    return set_params(h.data(), kParams);
+}
+
+void Net::pack_all()
+{
+    for (int l = 0; l < kLayers; ++l) {
+        const float* W = params_ + (size_t)l * kLayerParams;
+        hipLaunchKernelGGL(k_pack, dim3(kPackedHalves / 256), dim3(256), 0, stream_, W, kNet[l].cin, kNet[l].cout, 0,
+                           wl_ + (size_t)l * kPackedHalves);
+        hipLaunchKernelGGL(k_pack, dim3(kPackedHalves / 256), dim3(256), 0, stream_, W, kNet[l].cin, kNet[l].cout, 1,
+                           wlt_ + (size_t)l * kPackedHalves);
+    }
+}
+
+int Net::set_params(const float* host, long long n)
+{
+    if (n != kParams) return fail("parameter vector must have %d entries (got %lld)", kParams, n);
+    // entries outside a layer's channel counts must stay zero
+    std::vector<float> h(host, host + n);
+    for (int l = 0; l < kLayers; ++l) {
+        const LayerDef& d = kNet[l];
+        for (int t = 0; t < kTaps; ++t)
+            for (int ci = 0; ci < 16; ++ci)
+                for (int co = 0; co < 16; ++co)
+                    if (ci >= d.cin || co >= d.cout) h[(size_t)l * kLayerParams + t * 256 + ci * 16 + co] = 0.0f;
+        for (int co = d.cout; co < 16; ++co) h[(size_t)l * kLayerParams + kTaps * 256 + co] = 0.0f;
+    }
+    int rc = hip(hipMemcpyAsync(params_, h.data(), n * sizeof(float), hipMemcpyHostToDevice, stream_), "upload params");
+    if (!rc) rc = hip(hipMemsetAsync(adam_m_, 0, kParams * sizeof(float), stream_), "memset");
+    if (!rc) rc = hip(hipMemsetAsync(adam_v_, 0, kParams * sizeof(float), stream_), "memset");
+    if (rc) return rc;
+    adam_t_ = 0;
+    pack_all();
+    return hip(hipStreamSynchronize(stream_), "sync");
+}
+
+int Net::get_params(float* host, long long n)
+{
+    if (n != kParams) return fail("parameter vector must have %d entries (got %lld)", kParams, n);
+    int rc = hip(hipMemcpyAsync(host, params_, n * sizeof(float), hipMemcpyDeviceToHost, stream_), "download params");
+    if (rc) return rc;
+    return hip(hipStreamSynchronize(stream_), "sync");
+}
+
+void Net::release(BufSet& bs)
+{
+    for (void* p : bs.owned) (void)hipFree(p);
+    bs = BufSet{};
+}
+
+ActRef Net::level(const BufSet& bs, int lv, uint16_t* p) const
+{
+    ActRef a;
+    a.p = p;
+    a.B = bs.B;
+    a.D = bs.dims[lv][0];
+    a.H = bs.dims[lv][1];
+    a.W = bs.dims[lv][2];
+    return a;
+}
+ActRef Net::act(const BufSet& bs, int i) const { return level(bs, kNet[i].level, bs.act[i]); }
+ActRef Net::dact(const BufSet& bs, int i) const { return level(bs, kNet[i].level, bs.dact[i]); }
+
+int Net::ensure(BufSet& bs, int B, const int n[3], bool train)
+{
+    if (bs.B == B && bs.n[0] == n[0] && bs.n[1] == n[1] && bs.n[2] == n[2] && bs.train == train) return 0;
+    release(bs);
+    bs.B = B;
+    bs.train = train;
+    for (int d = 0; d < 3; ++d) bs.n[d] = n[d];
+    bs.dims[0][0] = n[2];
+    bs.dims[0][1] = n[1];
+    bs.dims[0][2] = n[0];
+    for (int l = 1; l < 3; ++l)
+        for (int d = 0; d < 3; ++d) bs.dims[l][d] = (bs.dims[l - 1][d] + 1) / 2;
+    auto vox_count = [&](int lv) { return (size_t)B * bs.dims[lv][0] * bs.dims[lv][1] * bs.dims[lv][2]; };
+    auto alloc = [&](size_t bytes, void** p) -> int {
+        int rc = hip(hipMalloc(p, std::max<size_t>(bytes, 256)), "hipMalloc network buffers");
+        if (!rc) bs.owned.push_back(*p);
+        return rc;
+    };
+    bs.act.assign(kActs, nullptr);
+    bs.dact.assign(kActs, nullptr);
+    int rc = 0;
+    if (!train) {
+        // inference: two buffers per level, reused as soon as an activation is dead
+        static const int slot[kActs] = {0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0};
+        uint16_t* lvbuf[3][2] = {};
+        for (int l = 0; l < 3 && !rc; ++l)
+            for (int k = 0; k < 2 && !rc; ++k) rc = alloc(vox_count(l) * kC * 2, (void**)&lvbuf[l][k]);
+        if (rc) return rc;
+        for (int i = 0; i < kActs; ++i) bs.act[i] = lvbuf[kNet[i].level][slot[i]];
+        return 0;
+    }
+    for (int i = 0; i < kActs && !rc; ++i) rc = alloc(vox_count(kNet[i].level) * kC * 2, (void**)&bs.act[i]);
+    for (int i = 0; i < kActs && !rc; ++i) rc = alloc(vox_count(kNet[i].level) * kC * 2, (void**)&bs.dact[i]);
+    for (int l = 0; l < 3 && !rc; ++l) rc = alloc(vox_count(l) * kC * 2, (void**)&bs.dx[l]);
+    const size_t nv = vox_count(0);
+    if (!rc) rc = alloc(nv * 4, (void**)&bs.in);
+    if (!rc) rc = alloc(nv * 4, (void**)&bs.y);
+    if (!rc) rc = alloc(nv * 4, (void**)&bs.dy);
+    if (!rc) rc = alloc(nv * 4, (void**)&bs.ky);
+    if (!rc) rc = alloc(B * 8, (void**)&bs.g);
+    if (!rc) rc = alloc(B * 16, (void**)&bs.coef);
+    if (!rc) rc = alloc(B * 8, (void**)&bs.lossv);
+    const long long per_sample = (long long)n[0] * n[1] * n[2];
+    bs.lblocks = (int)std::min<long long>(256, (per_sample + 1023) / 1024);
+    if (!rc) rc = alloc((size_t)B * bs.lblocks * 16, (void**)&bs.lpart);
+    long long pb = 0;
+    for (int l = 0; l < kLayers; ++l) {
+        DwArgs w{};
+        w.B = B;
+        w.D = bs.dims[kNet[l].level][0];
+        w.H = bs.dims[kNet[l].level][1];
+        w.W = bs.dims[kNet[l].level][2];
+        pb = std::max<long long>(pb, dw_blocks(w));
+    }
+    bs.part_blocks = pb;
+    if (!rc) rc = alloc((size_t)pb * kLayerParams * 4, (void**)&bs.part);
+    return rc;
+}
+
+void Net::forward(BufSet& bs, const FieldRef& fin, const FieldRef& fout, const int* gate, const int* gate2)
+{
+    for (int i = 0; i < kLayers; ++i) {
+        const LayerDef& d = kNet[i];
+        ConvArgs a{};
+        a.mode_in = d.mode_in;
+        a.mode_out = i == kLayers - 1 ? OUT_FIELD : OUT_RELU;
+        a.out = i < kActs ? act(bs, i) : level(bs, 0, nullptr);
+        if (d.mode_in == IN_FIELD) a.fin = fin;
+        else a.in = act(bs, d.in_act);
+        if (d.mode_in == IN_UPADD) a.in2 = act(bs, d.skip_act);
+        a.fout = fout;
+        a.wl = wl_ + (size_t)i * kPackedHalves;
+        a.bias = params_ + (size_t)i * kLayerParams + kTaps * 256;
+        a.gate = gate;
+        a.gate2 = gate2;
+        launch_conv(a, stream_);
+    }
+}
+
+void Net::backward(BufSet& bs, const FieldRef& fin)
+{
+    FieldRef fdy;
+    fdy.p = bs.dy;
+    fdy.sx = 1;
+    fdy.sy = bs.n[0];
+    fdy.sz = (long long)bs.n[0] * bs.n[1];
+    fdy.sb = fdy.sz * bs.n[2];
+    for (int i = kLayers - 1; i >= 0; --i) {
+        const LayerDef& d = kNet[i];
+        DwArgs w{};
+        w.mode_in = d.mode_in;
+        if (d.mode_in == IN_FIELD) w.fin = fin;
+        else w.in = act(bs, d.in_act);
+        if (d.mode_in == IN_UPADD) w.in2 = act(bs, d.skip_act);
+        if (i == kLayers - 1) {
+            w.dz_mode = 2;
+            w.fdy = fdy;
+        } else {
+            w.dz_mode = 0;
+            w.dy = dact(bs, i);
+            w.y = act(bs, i);
+        }
+        w.B = bs.B;
+        w.D = bs.dims[d.level][0];
+        w.H = bs.dims[d.level][1];
+        w.W = bs.dims[d.level][2];
+        w.part = bs.part;
+        launch_dw(w, stream_);
+        launch_dw_reduce(bs.part, dw_blocks(w), grads_ + (size_t)i * kLayerParams, stream_);
+        if (i == 0) break;
+        ConvArgs c{};
+        if (i == kLayers - 1) {
+            c.mode_in = IN_FIELD;
+            c.fin = fdy;
+        } else {
+            c.mode_in = IN_MASK;
+            c.in = dact(bs, i);
+            c.in2 = act(bs, i);
+        }
+        c.mode_out = OUT_LINEAR;
+        c.wl = wlt_ + (size_t)i * kPackedHalves;
+        const bool plain = d.mode_in == IN_PLAIN;
+        c.out = plain ? dact(bs, d.in_act) : level(bs, d.level, bs.dx[d.level]);
+        launch_conv(c, stream_);
+        if (d.mode_in == IN_POOL) launch_pool_back(c.out, dact(bs, d.in_act), 1, stream_);
+        if (d.mode_in == IN_UPADD) launch_upadd_back(c.out, dact(bs, d.skip_act), 0, dact(bs, d.in_act), stream_);
+    }
+}
+
+void Net::loss(BufSet& bs, int B, long long nv)
+{
+    const int per_block = (int)((nv + bs.lblocks - 1) / bs.lblocks);
+    hipLaunchKernelGGL(k_loss1, dim3(bs.lblocks, B), dim3(256), 0, stream_, bs.y, bs.in, bs.ky, bs.n[0], bs.n[1],
+                       bs.n[2], per_block, bs.lpart);
+    hipLaunchKernelGGL(k_loss2, dim3(B), dim3(64), 0, stream_, bs.lpart, bs.lblocks, bs.g, B, bs.coef, bs.lossv);
+    const long long nt = nv * B;
+    hipLaunchKernelGGL(k_loss3, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, stream_, bs.ky, bs.in, bs.coef, nv, B,
+                       bs.dy);
+}
+
+static FieldRef dense_field(float* p, const int n[3])
+{
+    FieldRef f;
+    f.p = p;
+    f.is64 = 0;
+    f.sx = 1;
+    f.sy = n[0];
+    f.sz = (long long)n[0] * n[1];
+    f.sb = f.sz * n[2];
+    return f;
+}
+
+int Net::apply(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
+               const int* gate, const int* gate2, const double* rr)
+{
+    const int n[3] = {g.n0, g.n1, g.n2};
+    int rc = ensure(inf_, 1, n, false);
+    if (rc) return rc;
+    const hipStream_t own = stream_;
+    stream_ = s;
+    FieldRef fi, fo;
+    fi.p = const_cast<void*>(in);
+    fi.is64 = in_prec == 64;
+    fi.off = g.off;
+    fi.sx = 1;
+    fi.sy = g.sy;
+    fi.sz = g.sz;
+    fi.sb = 0;
+    fi.scale_sum = rr;
+    fi.scale_count = (double)g.n0 * g.n1 * g.n2;
+    fo = fi;
+    fo.p = out;
+    fo.is64 = out_prec == 64;
+    fo.scale_sum = nullptr;
+    forward(inf_, fi, fo, gate, gate2);
+    stream_ = own;
+    return hip(hipGetLastError(), "network launch");
+}
+
+int Net::forward_host(const float* in, int B, const int n[3], float* out)
+{
+    int rc = ensure(trn_, B, n, true);
+    if (rc) return rc;
+    const size_t nv = (size_t)B * n[0] * n[1] * n[2];
+    rc = hip(hipMemcpyAsync(trn_.in, in, nv * 4, hipMemcpyHostToDevice, stream_), "upload");
+    if (rc) return rc;
+    forward(trn_, dense_field(trn_.in, n), dense_field(trn_.y, n), nullptr, nullptr);
+    rc = hip(hipGetLastError(), "network launch");
+    if (!rc) rc = hip(hipMemcpyAsync(out, trn_.y, nv * 4, hipMemcpyDeviceToHost, stream_), "download");
+    if (!rc) rc = hip(hipStreamSynchronize(stream_), "sync");
+    return rc;
+}
+
+int Net::activation_host(int i, float* out)
+{
+    if (i < 0 || i >= kActs) return fail("activation index %d out of range", i);
+    if (!trn_.B) return fail("no forward pass to read from");
+    const ActRef a = act(trn_, i);
+    const size_t nh = (size_t)a.B * a.D * a.H * a.W * kC;
+    std::vector<uint16_t> h(nh);
+    int rc = hip(hipMemcpyAsync(h.data(), a.p, nh * 2, hipMemcpyDeviceToHost, stream_), "download");
+    if (!rc) rc = hip(hipStreamSynchronize(stream_), "sync");
+    if (rc) return rc;
+    for (size_t k = 0; k < nh; ++k) {
+        const unsigned u = (unsigned)h[k] << 16;
+        memcpy(out + k, &u, 4);
+    }
+    return 0;
+}
+
+int Net::loss_grad_host(const float* in, const double* G, int B, const int n[3], double* loss_out, float* grad)
+{
+    int rc = ensure(trn_, B, n, true);
+    if (rc) return rc;
+    const long long nv = (long long)n[0] * n[1] * n[2];
+    rc = hip(hipMemcpyAsync(trn_.in, in, (size_t)nv * B * 4, hipMemcpyHostToDevice, stream_), "upload");
+    if (!rc) rc = hip(hipMemcpyAsync(trn_.g, G, (size_t)B * 8, hipMemcpyHostToDevice, stream_), "upload");
+    if (rc) return rc;
+    const FieldRef fin = dense_field(trn_.in, n);
+    forward(trn_, fin, dense_field(trn_.y, n), nullptr, nullptr);
+    loss(trn_, B, nv);
+    backward(trn_, fin);
+    rc = hip(hipGetLastError(), "network launch");
+    std::vector<double> lv(B);
+    if (!rc) rc = hip(hipMemcpyAsync(grad, grads_, kParams * sizeof(float), hipMemcpyDeviceToHost, stream_), "download");
+    if (!rc) rc = hip(hipMemcpyAsync(lv.data(), trn_.lossv, B * 8, hipMemcpyDeviceToHost, stream_), "download");
+    if (!rc) rc = hip(hipStreamSynchronize(stream_), "sync");
+    if (rc) return rc;
+    *loss_out = std::accumulate(lv.begin(), lv.end(), 0.0) / B;
+    return 0;
+}
+
+int Net::dataset_reserve(long long npairs, const int n[3])
+{
+    const long long nv = (long long)n[0] * n[1] * n[2];
+    if (data_r_) (void)hipFree(data_r_);
+    data_r_ = nullptr;
+    int rc = hip(hipMalloc(&data_r_, (size_t)npairs * nv * 4), "hipMalloc training set");
+    if (rc) return rc;
+    data_n_ = nv;
+    data_cap_ = npairs;
+    data_pairs_ = npairs;
+    data_g_.assign(npairs, 0.0);
+    for (int d = 0; d < 3; ++d) data_dims_[d] = n[d];
+    return 0;
+}
+
+int Net::train(const TrainCfg& c, TrainStats* st)
+{
+    if (data_pairs_ < 1) return fail("empty training set");
+    if (c.batch < 1 || c.steps < 1) return fail("batch and steps must be positive");
+    const int B = (int)std::min<long long>(c.batch, data_pairs_);
+    int rc = ensure(trn_, B, data_dims_, true);
+    if (rc) return rc;
+    const long long nv = data_n_;
+    std::mt19937_64 rng(c.seed);
+    std::vector<long long> perm(data_pairs_);
+    std::iota(perm.begin(), perm.end(), 0LL);
+    std::shuffle(perm.begin(), perm.end(), rng);
+    long long pos = 0;
+    std::vector<double> gb(B), lv(B);
+    const FieldRef fin = dense_field(trn_.in, data_dims_);
+    const FieldRef fout = dense_field(trn_.y, data_dims_);
+    const auto t0 = std::chrono::steady_clock::now();
+    double first = -1.0, last = -1.0;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int step = 0; step < c.steps; ++step) {
+        for (int k = 0; k < B; ++k) {
+            if (pos == data_pairs_) {
+                std::shuffle(perm.begin(), perm.end(), rng);
+                pos = 0;
+            }
+            const long long i = perm[pos++];
+            rc = hip(hipMemcpyAsync(trn_.in + k * nv, data_r_ + i * nv, (size_t)nv * 4, hipMemcpyDeviceToDevice, stream_),
+                     "batch copy");
+            if (rc) return rc;
+            gb[k] = data_g_[i];
+        }
+        rc = hip(hipMemcpyAsync(trn_.g, gb.data(), (size_t)B * 8, hipMemcpyHostToDevice, stream_), "upload");
+        if (rc) return rc;
+        forward(trn_, fin, fout, nullptr, nullptr);
+        loss(trn_, B, nv);
+        backward(trn_, fin);
+        ++adam_t_;
+        const double frac = c.steps > 1 ? (double)step / (c.steps - 1) : 1.0;
+        const float lr = (float)(c.lr_final + 0.5 * (c.lr - c.lr_final) * (1.0 + std::cos(M_PI * frac)));
+        const float bc1 = 1.0f - std::pow(b1, (float)adam_t_), bc2 = 1.0f - std::pow(b2, (float)adam_t_);
+        hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, stream_, params_, grads_, adam_m_, adam_v_,
+                           kParams, lr, b1, b2, eps, bc1, bc2);
+        pack_all();
+        if (step == 0 || step == c.steps - 1 || (step % 50) == 0) {
+            rc = hip(hipMemcpyAsync(lv.data(), trn_.lossv, (size_t)B * 8, hipMemcpyDeviceToHost, stream_), "download");
+            if (!rc) rc = hip(hipStreamSynchronize(stream_), "sync");
+            if (rc) return rc;
+            const double l = std::accumulate(lv.begin(), lv.end(), 0.0) / B;
+            if (first < 0) first = l;
+            last = l;
+            if (!std::isfinite(l)) return fail("training diverged at step %d (loss %g)", step, l);
+        }
+    }
+    rc = hip(hipStreamSynchronize(stream_), "sync");
+    if (rc) return rc;
+    if (st) {
+        st->loss_first = first;
+        st->loss_last = last;
+        st->steps = c.steps;
+        st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return 0;
+}
+
+}  // namespace nn
+}  // namespace pps

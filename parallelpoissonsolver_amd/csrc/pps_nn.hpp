@@ -1,122 +1,161 @@
-// pps_nn.hpp -- learned multilevel preconditioner: network layout, the MFMA
-// convolution kernels' launchers and the preconditioner interface.
+// pps_nn.hpp -- learned multilevel preconditioner (BF16 MFMA convolutions).
 //
-// Network (3 levels, 16 channels, 3^3 kernels, ReLU; BF16 weights and
-// activations, FP32 accumulation):
-//   L0  A0 = relu(lift(r))            1 -> 16
-//       A1 = relu(a1(A0))             16 -> 16   (skip S0)
-//   L1  B0 = relu(b0(pool(A1)))       16 -> 16
-//       B1 = relu(b1(B0))             16 -> 16   (skip S1)
-//   L2  C0 = relu(c0(pool(B1)))       16 -> 16
-//       C1..C5 = relu(ci(C_{i-1}))    16 -> 16
-//   L1  E0 = relu(e0(up(C5) + S1))    16 -> 16
-//       E1 = relu(e1(E0))             16 -> 16
-//   L0  F0 = relu(f0(up(E1) + S0))    16 -> 16
-//       z  = out(F0)                  16 -> 1   (linear)
-// pool = 2x2x2 average, up = nearest-neighbour copy; both are fused into the
-// input stage of the convolution that consumes them.  Activations are stored
-// channels-last (16 BF16 = 32 bytes per voxel), dense per level.
+// Network: 3 levels, 16 channels, 3^3 kernels, ReLU; BF16 weights and
+// activations, FP32 accumulation, FP32 master weights (Adam).
+//   layer  name  level  input                      out
+//     0    lift    0    field r (1 channel)        act0
+//     1    a1      0    act0                       act1   (skip of level 0)
+//     2    b0      1    pool(act1)                 act2
+//     3    b1      1    act2                       act3   (skip of level 1)
+//     4    c0      2    pool(act3)                 act4
+//   5..9   c1..c5  2    act(i-1)                   act5..act9
+//    10    e0      1    up(act9) + act3            act10
+//    11    e1      1    act10                      act11
+//    12    f0      0    up(act11) + act1           act12
+//    13    out     0    act12 (linear, 16 -> 1)    field z
+// pool = 2x2x2 average, up = nearest neighbour; both run inside the input
+// stage of the convolution that consumes them.  Activations are dense,
+// channels-last (16 BF16 = 32 bytes per voxel), [b][z][y][x][c].
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
+#include <vector>
 
 namespace pps {
 
-struct Ctx;
+// solver region of a pitched field: element (i,j,k) of the region at
+// off + i + sy*j + sz*k, region extent n0 x n1 x n2
+struct NNFieldGeom {
+    long long off, sy, sz;
+    int n0, n1, n2;
+};
 
+// learned preconditioner as seen by the Krylov stages
 struct NNPrecond {
     virtual ~NNPrecond() = default;
-    // out = M(in) on the solver region (zero elsewhere); kernels gated on *gate
-    virtual int apply(Ctx& C, void* out, int out_prec, void* in, int in_prec, const int* gate) = 0;
-    virtual int fallback_switches() const { return 0; }
+    // out = M(in) on the region (other points untouched).  The input is scaled
+    // by 1/RMS with RMS^2 = *rr / count; every kernel returns at once when
+    // *gate != 0 or *gate2 != 0.
+    virtual int apply(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
+                      const int* gate, const int* gate2, const double* rr) = 0;
+    virtual const char* error() const = 0;
 };
 
 namespace nn {
 
-constexpr int kC = 16;        // channels
+constexpr int kC = 16;
 constexpr int kTaps = 27;
-constexpr int kSteps = 14;    // K = 28 taps x 16 channels in steps of 32 (tap 27 is zero)
-constexpr int kTileX = 16, kTileY = 4, kTileZ = 4;  // output tile of one conv block
-constexpr int kNumConv = 14;  // lift, a1, b0, b1, c0..c5, e0, e1, f0, out
+constexpr int kSteps = 14;        // K = 28 taps x 16 channels in MFMA steps of 32 (tap 27 is zero)
+constexpr int kLayers = 14;
+constexpr int kActs = 13;         // activations act0..act12
+constexpr int kLayerParams = kTaps * kC * kC + kC;  // W[tap][ci][co] then bias[co]
+constexpr int kParams = kLayers * kLayerParams;
+constexpr int kPackedHalves = kSteps * 64 * 8;      // lane-arranged BF16 weights of one layer
 
-// where a convolution reads its input from
-enum InMode {
-    IN_PLAIN = 0,   // act at this level
-    IN_POOL = 1,    // 2x2x2 average of act at the finer level
-    IN_UPADD = 2,   // act at the coarser level (nearest) + skip at this level
-    IN_MASK = 3,    // act * (mask > 0): ReLU gradient of the consumer's output
-    IN_FIELD = 4    // one-channel FP32/FP64 field (strided), times a device scale
-};
-enum OutMode {
-    OUT_RELU = 0,   // bf16 act, + bias, relu
-    OUT_LINEAR = 1, // bf16 act, no bias, no relu (backward data)
-    OUT_FIELD = 2   // channel 0 (+ bias) into a strided FP32/FP64 field
-};
+enum InMode { IN_PLAIN = 0, IN_POOL = 1, IN_UPADD = 2, IN_MASK = 3, IN_FIELD = 4 };
+enum OutMode { OUT_RELU = 0, OUT_LINEAR = 1, OUT_FIELD = 2 };
 
-// a strided one-channel field (solver field or dense training sample)
+struct LayerDef {
+    int cin, cout, level, mode_in;
+    int in_act;    // activation read (finer one for POOL, coarser one for UPADD); -1 = field
+    int skip_act;  // UPADD: the skip at this level
+    int relu;
+};
+extern const LayerDef kNet[kLayers];
+
+// one-channel strided field (solver field or dense sample batch)
 struct FieldRef {
-    void* p;           // float* or double*
-    int is64;
-    long long off;     // element offset of voxel (0,0,0) of sample 0
-    long long sx, sy, sz, sb;  // strides in elements
-    const double* scale_sum;   // if set: scale = 1/sqrt(*scale_sum / scale_count)
-    double scale_count;
-    float scale;       // used when scale_sum is null
+    void* p = nullptr;
+    int is64 = 0;
+    long long off = 0, sx = 1, sy = 0, sz = 0, sb = 0;
+    const double* scale_sum = nullptr;  // scale = 1/sqrt(*scale_sum / scale_count) when set
+    double scale_count = 1.0;
+    float scale = 1.0f;
 };
 
 struct ActRef {
-    uint16_t* p;       // bf16 bits, [b][z][y][x][16]
-    int B, D, H, W;
+    uint16_t* p = nullptr;
+    int B = 0, D = 0, H = 0, W = 0;
 };
 
-struct ConvArgs {
-    int mode_in, mode_out;
-    ActRef in, in2;     // in2: skip (UPADD) or ReLU mask (MASK); fine level for POOL is `in`
-    FieldRef fin, fout;
-    ActRef out;          // output activation (D,H,W of this level)
-    const uint16_t* wl;  // lane-arranged bf16 weights [kSteps][64][8]
-    const float* bias;   // [16] or null
-    const int* gate;
-    const int* gate2;
+struct BufSet {
+    int B = 0, n[3] = {0, 0, 0};  // batch, level-0 extent (x, y, z)
+    bool train = false;
+    int dims[3][3] = {};          // per level: D, H, W
+    std::vector<uint16_t*> act, dact;
+    uint16_t* dx[3] = {nullptr, nullptr, nullptr};  // backward-data scratch per level
+    float *in = nullptr, *y = nullptr, *dy = nullptr, *ky = nullptr;
+    double *g = nullptr, *lpart = nullptr, *coef = nullptr, *lossv = nullptr;
+    float* part = nullptr;
+    long long part_blocks = 0;
+    int lblocks = 0;
+    std::vector<void*> owned;
 };
 
-void launch_conv(const ConvArgs& a, hipStream_t s);
-
-struct DwArgs {
-    int mode_in;        // layer input as in the forward pass
-    ActRef in, in2;
-    FieldRef fin;
-    int dz_mode;        // 0: dZ = dY (bf16 act) masked by Y>0; 1: dZ = dY unmasked bf16; 2: dZ = FP32 field
-    ActRef dy, y;
-    FieldRef fdy;
-    int D, H, W, B;     // output geometry of the layer
-    float* part;        // [nblocks][27*16*16 + 16]
-    int tiles_per_block;
+struct TrainCfg {
+    int batch;
+    int steps;
+    float lr, lr_final;
+    unsigned long long seed;
 };
-int dw_blocks(const DwArgs& a);
-void launch_dw(const DwArgs& a, hipStream_t s);
-// sum the partials: g[27*256 + 16] = sum over blocks
-void launch_dw_reduce(const float* part, int nblocks, float* g, hipStream_t s);
 
-// gradient routing of the fused input stages
-// dfine[v] (+)= dcoarse[v>>1] / 8   (pool backward)
-void launch_pool_back(const ActRef& dcoarse, const ActRef& dfine, int accumulate, hipStream_t s);
-// dskip (+)= dx ; dcoarse = sum of the 8 children of dx   (up + skip backward)
-void launch_upadd_back(const ActRef& dx, const ActRef& dskip, int accumulate_skip, const ActRef& dcoarse, hipStream_t s);
+struct TrainStats {
+    double loss_first, loss_last;
+    int steps;
+    double seconds;
+};
 
-// lane-arranged weights from FP32 master weights W[27][16 ci][16 co] (ci < cin, co < cout)
-// transposed = 1: the flipped, transposed filter of the backward-data pass
-void launch_pack_weights(const float* W, int cin, int cout, int transposed, uint16_t* wl, hipStream_t s);
+class Net : public NNPrecond {
+public:
+    Net() = default;
+    ~Net() override;
+    int init(unsigned long long seed);  // own stream, He-initialised weights
+    int apply(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
+              const int* gate, const int* gate2, const double* rr) override;
+    const char* error() const override { return err_.c_str(); }
 
-// Adam update of n parameters
-void launch_adam(float* w, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
-                 float bc1, float bc2, hipStream_t s);
+    int set_params(const float* host, long long n);
+    int get_params(float* host, long long n);
+    static long long effective_params();
+    // dense host batches [B][n2][n1][n0] (float)
+    int forward_host(const float* in, int B, const int n[3], float* out);
+    int activation_host(int act, float* out);  // act of the last forward_host / loss_grad_host
+    int loss_grad_host(const float* in, const double* G, int B, const int n[3], double* loss, float* grad);
+    // training set of normalised residuals (device) and their energy norms G
+    int dataset_reserve(long long npairs, const int n[3]);
+    float* dataset_sample(long long i) const { return data_r_ + i * data_n_; }
+    long long sample_size() const { return data_n_; }
+    std::vector<double>& dataset_g() { return data_g_; }
+    long long dataset_size() const { return data_pairs_; }
+    int train(const TrainCfg& c, TrainStats* st);
+    int fail(const char* fmt, ...);
+    int hip(hipError_t e, const char* what);
 
-// loss = mean over samples of ||y - t||^2 / ||t||^2 ; dy = 2 (y - t) / ||t||^2 / B
-void launch_loss(const float* y, const float* t, const float* tnorm2, int B, long long n, float* dy, double* loss_partial,
-                 hipStream_t s);
+private:
+    int ensure(BufSet& bs, int B, const int n[3], bool train);
+    void release(BufSet& bs);
+    void forward(BufSet& bs, const FieldRef& fin, const FieldRef& fout, const int* gate, const int* gate2);
+    void backward(BufSet& bs, const FieldRef& fin);
+    void loss(BufSet& bs, int B, long long nvox);
+    void pack_all();
+    ActRef act(const BufSet& bs, int i) const;
+    ActRef dact(const BufSet& bs, int i) const;
+    ActRef level(const BufSet& bs, int lv, uint16_t* p) const;
+
+    hipStream_t stream_ = nullptr;  // host-driven work; apply() runs on the caller's stream
+    std::string err_;
+    float *params_ = nullptr, *grads_ = nullptr, *adam_m_ = nullptr, *adam_v_ = nullptr;
+    uint16_t *wl_ = nullptr, *wlt_ = nullptr;  // packed forward / backward-data weights, kLayers each
+    BufSet inf_, trn_;
+    float* data_r_ = nullptr;
+    long long data_n_ = 0, data_pairs_ = 0, data_cap_ = 0;
+    int data_dims_[3] = {0, 0, 0};
+    std::vector<double> data_g_;
+    int adam_t_ = 0;
+};
 
 }  // namespace nn
 }  // namespace pps

@@ -1,0 +1,164 @@
+"""GPU tests of the learned preconditioner: the BF16 MFMA forward and backward
+kernels against the CPU oracle (oracle/nn_oracle.py), on-GPU training, and
+what the trained network gains inside FCG over the plain (Jacobi) method.
+
+Tolerances: activations and outputs are stored in BF16 (8 significant bits);
+the oracle rounds at the same points, so the two differ by FP32 accumulation
+order and occasional one-ulp BF16 flips: 2 % of the largest value per
+activation, 1 % relative L2 on the output, 3 % relative L2 per layer gradient.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+import parallelpoissonsolver_amd._lib as L  # noqa: E402
+from oracle import nn_oracle as O  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    p = O.init_params(seed)
+    for l in range(O.N_LAYERS):
+        p[l * O.LAYER_PARAMS + O.TAPS * 256:(l + 1) * O.LAYER_PARAMS] = rng.normal(0, 0.05, 16)
+    return p
+
+
+@pytest.fixture(scope="module")
+def net():
+    n = L.Network(seed=3)
+    yield n
+    n.close()
+
+
+@pytest.mark.parametrize("shape", [(2, 16, 16, 16), (1, 9, 13, 20)])
+def test_forward_matches_oracle(net, shape):
+    print(f"[nn] forward {shape}", flush=True)
+    p = _params(11)
+    net.set_params(p)
+    p = net.get_params()  # entries outside the channel counts are zeroed by the library
+    x = np.random.default_rng(12).normal(size=shape).astype(np.float32)
+    y = net.forward(x)
+    yo, acts = O.forward(p, x, rounding=True)
+    assert np.isfinite(y).all()
+    assert np.linalg.norm(y - yo) / np.linalg.norm(yo) < 1e-2
+    for i in range(O.N_LAYERS - 1):
+        a = net.activation(i)
+        assert a.shape == acts[i].shape
+        assert np.max(np.abs(a - acts[i])) <= 0.02 * np.max(np.abs(acts[i])) + 1e-6, i
+
+
+def test_loss_gradient_matches_oracle(net):
+    print("[nn] loss/grad 2x16^3", flush=True)
+    p = _params(21)
+    net.set_params(p)
+    p = net.get_params()
+    rng = np.random.default_rng(22)
+    x = rng.normal(size=(2, 16, 16, 16)).astype(np.float32)
+    G = np.array([2.0, 3.0])
+    loss, g = net.loss_grad(x, G)
+    lo, go = O.loss_grad(p, x, G, rounding=True)
+    assert abs(loss - lo) < 1e-3 * max(1.0, abs(lo))
+    for l in range(O.N_LAYERS):
+        sl = slice(l * O.LAYER_PARAMS, (l + 1) * O.LAYER_PARAMS)
+        ref = go[sl]
+        assert np.linalg.norm(g[sl] - ref) <= 3e-2 * np.linalg.norm(ref) + 1e-9, l
+
+
+def test_training_set_and_training(net):
+    print("[nn] generate 16^3 + train", flush=True)
+    net.set_params(O.init_params(5))
+    pairs = net.generate(n=16, nrhs=4, snapshots=4, kmax=8, seed=1)
+    assert pairs == 16
+    for i in range(pairs):
+        r, g = net.dataset(i)
+        assert abs(np.sqrt(np.mean(r.astype(np.float64) ** 2)) - 1.0) < 1e-5
+        assert g > 0
+    st = net.train(n=16, batch=4, steps=80, lr=2e-3, lr_final=2e-4, seed=2)
+    assert st["pairs"] == 16 and st["steps"] == 80
+    assert np.isfinite(st["loss_last"]) and st["loss_last"] < st["loss_first"]
+
+
+@pytest.fixture(scope="module")
+def trained():
+    """A network trained on 32^3 CG iterates (256 pairs, 400 Adam steps)."""
+    n = L.Network(seed=7)
+    n.generate(n=32, nrhs=32, snapshots=8, kmax=16, seed=100)
+    st = n.train(n=32, batch=8, steps=400, lr=2e-3, lr_final=1e-4, seed=3)
+    print(f"[nn] trained: loss {st['loss_first']:.3f} -> {st['loss_last']:.3f} in {st['seconds']:.1f} s", flush=True)
+    yield n, st
+    n.close()
+
+
+def _ctx(n):
+    npg = n + 2
+    ds = 1.0 / (npg - 1)
+    C = L.Context((npg,) * 3, (ds,) * 3)
+    C.fill(L.FIELD_X, 0.0)
+    C.rhs_gauss(5, 16, 0.05, 0.2)
+    C.copy_field(L.FIELD_T0, L.FIELD_B)
+    return C
+
+
+def _solve(C, chain):
+    C.fill(L.FIELD_X, 0.0)
+    C.copy_field(L.FIELD_B, L.FIELD_T0)
+    return C.solve(chain)
+
+
+def test_trained_network_beats_jacobi_fcg(trained):
+    net, st = trained
+    assert st["loss_last"] < st["loss_first"]
+    C = _ctx(48)
+    net.attach(C)
+    jac = _solve(C, [dict(kind=L.FCG, tol=1e-10, maxit=5000), dict(kind=L.JACOBI)])
+    nn = _solve(C, [dict(kind=L.FCG, tol=1e-10, maxit=5000), dict(kind=L.NN)])
+    print(f"[nn] 48^3 FCG iterations: Jacobi {jac['iterations']}, learned {nn['iterations']}", flush=True)
+    for r in (jac, nn):
+        assert r["error_compute_operator"] < 1.5e-10
+    assert nn["fallback_switches"] == 0
+    assert nn["iterations"] < 0.7 * jac["iterations"]
+    C.close()
+
+
+def test_stall_switches_to_fallback(trained):
+    net, _ = trained
+    C = _ctx(32)
+    net.attach(C)
+    # a window test no residual drop can pass: the fallback takes over after 5 iterations
+    r = _solve(C, [dict(kind=L.FCG, tol=1e-10, maxit=5000), dict(kind=L.NN, stall_window=5, stall_ratio=1e-30)])
+    assert r["fallback_switches"] == 1
+    assert r["error_compute_operator"] < 1.5e-10
+    C.close()
+
+
+def test_nonfinite_output_restarts_on_fallback():
+    net = L.Network(seed=9)
+    p = net.get_params()
+    p[13 * O.LAYER_PARAMS + O.TAPS * 256] = np.nan  # output bias -> every output NaN
+    net.set_params(p)
+    C = _ctx(24)
+    net.attach(C)
+    r = _solve(C, [dict(kind=L.FCG, tol=1e-10, maxit=5000), dict(kind=L.NN)])
+    jac = _solve(C, [dict(kind=L.FCG, tol=1e-10, maxit=5000), dict(kind=L.JACOBI)])
+    assert r["fallback_switches"] == 1
+    assert r["error_compute_operator"] < 1.5e-10
+    assert r["iterations"] == jac["iterations"] + 1  # the discarded step, then the Jacobi iterates
+    C.close()
+    net.close()
+
+
+def test_mixed_precision_learned_fcg(trained):
+    net, _ = trained
+    C = _ctx(48)
+    net.attach(C)
+    r = _solve(C, [dict(kind=L.IR, tol=1e-10, maxit=50), dict(kind=L.FCG, precision=32, tol=1e-5, maxit=5000),
+                   dict(kind=L.NN, precision=32)])
+    print(f"[nn] IR + FP32 learned FCG: {r['iterations']} inner, {r['outer_iterations']} outer", flush=True)
+    assert r["error_compute_operator"] < 1e-10
+    assert r["fallback_switches"] == 0
+    C.close()
