@@ -85,10 +85,11 @@ def test_training_set_and_training(net):
 
 @pytest.fixture(scope="module")
 def trained():
-    """A network trained on 32^3 CG iterates (256 pairs, 400 Adam steps)."""
+    """A network trained on 32^3 CG iterates (256 pairs, half of the right-hand
+    sides white noise, 600 Adam steps)."""
     n = L.Network(seed=7)
-    n.generate(n=32, nrhs=32, snapshots=8, kmax=16, seed=100)
-    st = n.train(n=32, batch=8, steps=400, lr=2e-3, lr_final=1e-4, seed=3)
+    n.generate(n=32, nrhs=32, snapshots=8, kmax=16, seed=100, white_fraction=0.5)
+    st = n.train(n=32, batch=8, steps=600, lr=2e-3, lr_final=1e-4, seed=3)
     print(f"[nn] trained: loss {st['loss_first']:.3f} -> {st['loss_last']:.3f} in {st['seconds']:.1f} s", flush=True)
     yield n, st
     n.close()
@@ -121,7 +122,8 @@ def test_trained_network_beats_jacobi_fcg(trained):
     for r in (jac, nn):
         assert r["error_compute_operator"] < 1.5e-10
     assert nn["fallback_switches"] == 0
-    assert nn["iterations"] < 0.7 * jac["iterations"]
+    # measured on MI355X: 34 vs 143 iterations at 32^3, 33 vs 287 at 64^3
+    assert nn["iterations"] < 0.5 * jac["iterations"]
     C.close()
 
 
