@@ -32,19 +32,19 @@ LAYER_PARAMS = TAPS * C * C + C
 # (cin, cout, level, mode, in_act, skip_act, relu): mode 'field'|'plain'|'pool'|'upadd'
 NET = [
     (1, 16, 0, "field", -1, -1, 1),   # lift
-    (16, 16, 0, "plain", 0, -1, 1),   # a1
-    (16, 16, 1, "pool", 1, -1, 1),    # b0
-    (16, 16, 1, "plain", 2, -1, 1),   # b1
-    (16, 16, 2, "pool", 3, -1, 1),    # c0
-    (16, 16, 2, "plain", 4, -1, 1),   # c1
-    (16, 16, 2, "plain", 5, -1, 1),   # c2
-    (16, 16, 2, "plain", 6, -1, 1),   # c3
-    (16, 16, 2, "plain", 7, -1, 1),   # c4
-    (16, 16, 2, "plain", 8, -1, 1),   # c5
-    (16, 16, 1, "upadd", 9, 3, 1),    # e0
-    (16, 16, 1, "plain", 10, -1, 1),  # e1
-    (16, 16, 0, "upadd", 11, 1, 1),   # f0
-    (16, 1, 0, "plain", 12, -1, 0),   # out
+    (16, 16, 1, "pool", 0, -1, 1),    # b0
+    (16, 16, 1, "plain", 1, -1, 1),   # b1
+    (16, 16, 2, "pool", 2, -1, 1),    # c0
+    (16, 16, 2, "plain", 3, -1, 1),   # c1
+    (16, 16, 2, "plain", 4, -1, 1),   # c2
+    (16, 16, 2, "plain", 5, -1, 1),   # c3
+    (16, 16, 2, "plain", 6, -1, 1),   # c4
+    (16, 16, 2, "plain", 7, -1, 1),   # c5
+    (16, 16, 2, "plain", 8, -1, 1),   # c6
+    (16, 16, 2, "plain", 9, -1, 1),   # c7
+    (16, 16, 1, "upadd", 10, 2, 1),   # e0
+    (16, 16, 1, "plain", 11, -1, 1),  # e1
+    (16, 1, 0, "upadd", 12, 0, 0),    # out
 ]
 N_LAYERS = len(NET)
 N_PARAMS = N_LAYERS * LAYER_PARAMS

@@ -306,7 +306,7 @@ class Network:
     stages (below an FCG stage)."""
 
     N_ACTS = 13
-    LEVEL = (0, 0, 1, 1, 2, 2, 2, 2, 2, 2, 1, 1, 0)  # level of activation i
+    LEVEL = (0, 1, 1, 2, 2, 2, 2, 2, 2, 2, 2, 1, 1)  # level of activation i
 
     def __init__(self, seed=0):
         self.h = _VP()

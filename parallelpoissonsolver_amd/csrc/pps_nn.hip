@@ -28,20 +28,20 @@ namespace pps {
 namespace nn {
 
 const LayerDef kNet[kLayers] = {
-    {1, 16, 0, IN_FIELD, -1, -1, 1},   // lift
-    {16, 16, 0, IN_PLAIN, 0, -1, 1},   // a1
-    {16, 16, 1, IN_POOL, 1, -1, 1},    // b0
-    {16, 16, 1, IN_PLAIN, 2, -1, 1},   // b1
-    {16, 16, 2, IN_POOL, 3, -1, 1},    // c0
-    {16, 16, 2, IN_PLAIN, 4, -1, 1},   // c1
-    {16, 16, 2, IN_PLAIN, 5, -1, 1},   // c2
-    {16, 16, 2, IN_PLAIN, 6, -1, 1},   // c3
-    {16, 16, 2, IN_PLAIN, 7, -1, 1},   // c4
-    {16, 16, 2, IN_PLAIN, 8, -1, 1},   // c5
-    {16, 16, 1, IN_UPADD, 9, 3, 1},    // e0
-    {16, 16, 1, IN_PLAIN, 10, -1, 1},  // e1
-    {16, 16, 0, IN_UPADD, 11, 1, 1},   // f0
-    {16, 1, 0, IN_PLAIN, 12, -1, 0},   // out
+    {1, 16, 0, IN_FIELD, -1, -1, 1},   // 0  lift     r -> act0 (skip of level 0)
+    {16, 16, 1, IN_POOL, 0, -1, 1},    // 1  b0       pool(act0)
+    {16, 16, 1, IN_PLAIN, 1, -1, 1},   // 2  b1       -> act2 (skip of level 1)
+    {16, 16, 2, IN_POOL, 2, -1, 1},    // 3  c0       pool(act2)
+    {16, 16, 2, IN_PLAIN, 3, -1, 1},   // 4  c1
+    {16, 16, 2, IN_PLAIN, 4, -1, 1},   // 5  c2
+    {16, 16, 2, IN_PLAIN, 5, -1, 1},   // 6  c3
+    {16, 16, 2, IN_PLAIN, 6, -1, 1},   // 7  c4
+    {16, 16, 2, IN_PLAIN, 7, -1, 1},   // 8  c5
+    {16, 16, 2, IN_PLAIN, 8, -1, 1},   // 9  c6
+    {16, 16, 2, IN_PLAIN, 9, -1, 1},   // 10 c7
+    {16, 16, 1, IN_UPADD, 10, 2, 1},   // 11 e0       up(act10) + act2
+    {16, 16, 1, IN_PLAIN, 11, -1, 1},  // 12 e1
+    {16, 1, 0, IN_UPADD, 12, 0, 0},    // 13 out      up(act12) + act0 -> z (linear)
 };
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -169,9 +169,105 @@ __device__ __forceinline__ u32x4 load_chunk(const ActRef& in, const ActRef& in2,
 }
 
 // ------------------------------------------------------------ convolution
+//
+// Tap pairs.  The 9 (dz, dx) combinations c = 3*dz + dx are paired as
+// (0,1) (2,3) (4,5) (6,7) (8,-): one MFMA of pair p at dy covers taps
+// (dz, dy, dx) of both members, its B operand is the input row y + dy at the
+// members' (dz, dx) shifts.  That operand is the same for output row y at dy
+// and row y - 1 at dy + 1, so each LDS read feeds up to three MFMAs:
+// a wavefront computing 4 output rows reads 6 input rows x 5 pairs = 30
+// operands for 60 MFMAs.  Weight operand o = 3*p + dy.
+
+__host__ __device__ constexpr int pair_combo(int p, int m) { return 2 * p + m; }  // < 9 valid
+
+// raw staging values: global loads are issued one plane ahead into these and
+// converted to BF16 chunks only when the plane is written to LDS
+template <int MI> struct Raw { u32x4 a; };
+template <> struct Raw<IN_MASK> { u32x4 a, m; };
+template <> struct Raw<IN_UPADD> { u32x4 a, s; };
+template <> struct Raw<IN_FIELD> { double a; };
+
+template <int MI>
+__device__ __forceinline__ Raw<MI> fetch(const ActRef& in, const ActRef& in2, const FieldRef& f, float fscale, int D,
+                                         int H, int W, int b, int x, int y, int z, int ch)
+{
+    Raw<MI> r{};
+    if (x < 0 || y < 0 || z < 0 || x >= W || y >= H || z >= D) return r;
+    if constexpr (MI == IN_PLAIN) {
+        r.a = *(const u32x4*)(in.p + vox(in, b, x, y, z) + ch * 8);
+    } else if constexpr (MI == IN_MASK) {
+        const long long o = vox(in, b, x, y, z) + ch * 8;
+        r.a = *(const u32x4*)(in.p + o);
+        r.m = *(const u32x4*)(in2.p + o);
+    } else if constexpr (MI == IN_UPADD) {
+        r.a = *(const u32x4*)(in.p + vox(in, b, x >> 1, y >> 1, z >> 1) + ch * 8);
+        r.s = *(const u32x4*)(in2.p + vox(in2, b, x, y, z) + ch * 8);
+    } else if constexpr (MI == IN_POOL) {
+        r.a = load_chunk<IN_POOL>(in, in2, f, fscale, D, H, W, b, x, y, z, ch);
+    } else {
+        if (ch == 0) {
+            const long long i = fidx(f, b, x, y, z);
+            r.a = f.is64 ? ((const double*)f.p)[i] : (double)((const float*)f.p)[i];
+        }
+    }
+    return r;
+}
+
+template <int MI>
+__device__ __forceinline__ u32x4 finish(const Raw<MI>& r, const FieldRef& f, float fscale)
+{
+    if constexpr (MI == IN_PLAIN || MI == IN_POOL) {
+        return r.a;
+    } else if constexpr (MI == IN_MASK) {
+        u32x4 o;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            o[i] = (bf_pos(r.m[i] & 0xffffu) ? (r.a[i] & 0xffffu) : 0u) | (bf_pos(r.m[i] >> 16) ? (r.a[i] & 0xffff0000u) : 0u);
+        return o;
+    } else if constexpr (MI == IN_UPADD) {
+        float a[8], s[8];
+        unpack8(r.a, a);
+        unpack8(r.s, s);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a[e] += s[e];
+        return pack8(a);
+    } else {
+        u32x4 o = {0u, 0u, 0u, 0u};
+        const float v = f.is64 ? (float)(r.a * (double)fscale) : (float)r.a * fscale;
+        o[0] = f2bf(v);
+        return o;
+    }
+}
+
+template <int MO>
+__device__ __forceinline__ void store_out(const ConvArgs& a, const f32x4& acc, const float* bias, int b, int x, int y,
+                                          int z, int j)
+{
+    if constexpr (MO == OUT_FIELD) {
+        if (j == 0) {
+            const long long o = fidx(a.fout, b, x, y, z);
+            const float v = acc[0] + bias[0];
+            if (a.fout.is64) ((double*)a.fout.p)[o] = (double)v;
+            else ((float*)a.fout.p)[o] = v;
+        }
+    } else {
+        float v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            v[r] = acc[r] + bias[r];
+            if (MO == OUT_RELU) v[r] = fmaxf(v[r], 0.0f);
+        }
+        u32x2 o;
+        o[0] = f2bf(v[0]) | (f2bf(v[1]) << 16);
+        o[1] = f2bf(v[2]) | (f2bf(v[3]) << 16);
+        *(u32x2*)(a.out.p + vox(a.out, b, x, y, z) + 4 * j) = o;
+    }
+}
+
+constexpr int kChunksPerThread = (kPlane + 255) / 256;
 
 template <int MI, int MO>
-__global__ __launch_bounds__(256) void k_conv(ConvArgs a, int zc)
+__global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
     __shared__ u32x4 lds[4 * kPlane];
@@ -186,69 +282,175 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a, int zc)
     float fscale = 1.0f;
     if constexpr (MI == IN_FIELD) fscale = field_scale(a.fin);
 
-    bf16x8 wr[kSteps];
+    bf16x8 wr[kOps];
     const u32x4* wl = (const u32x4*)a.wl;
 #pragma unroll
-    for (int s = 0; s < kSteps; ++s) wr[s] = __builtin_bit_cast(bf16x8, wl[s * 64 + lane]);
+    for (int o = 0; o < kOps; ++o) wr[o] = __builtin_bit_cast(bf16x8, wl[o * 64 + lane]);
     float bias[4] = {0.f, 0.f, 0.f, 0.f};
     if (MO != OUT_LINEAR && a.bias) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) bias[r] = a.bias[4 * j + r];
     }
 
-    auto stage = [&](int zi) {
-        u32x4* dst = lds + (zi & 3) * kPlane;
-        for (int t = tid; t < kPlane; t += 256) {
-            const int yy = t / (2 * kLW);
-            const int rem = t - yy * (2 * kLW);
-            const int xx = rem >> 1, ch = rem & 1;
-            dst[(yy * 2 + ch) * kLW + xx] =
-                load_chunk<MI>(a.in, a.in2, a.fin, fscale, D, H, W, b, x0 - 1 + xx, y0 - 1 + yy, zi, ch);
-        }
+    // this thread's staging chunks: plane position and LDS slot, fixed for the march
+    int sx[kChunksPerThread], sy[kChunksPerThread], sc[kChunksPerThread], sl[kChunksPerThread];
+#pragma unroll
+    for (int k = 0; k < kChunksPerThread; ++k) {
+        const int t = tid + 256 * k;
+        const int yy = t / (2 * kLW);
+        const int rem = t - yy * (2 * kLW);
+        sx[k] = x0 - 1 + (rem >> 1);
+        sy[k] = y0 - 1 + yy;
+        sc[k] = rem & 1;
+        sl[k] = t < kPlane ? (yy * 2 + (rem & 1)) * kLW + (rem >> 1) : -1;
+    }
+    // planes in flight: preA / preB hold raw loads issued one or two steps earlier
+    Raw<MI> preA[kChunksPerThread], preB[kChunksPerThread];
+    auto fetch_plane = [&](Raw<MI>* pre, int zi) {
+#pragma unroll
+        for (int k = 0; k < kChunksPerThread; ++k)
+            if (sl[k] >= 0) pre[k] = fetch<MI>(a.in, a.in2, a.fin, fscale, D, H, W, b, sx[k], sy[k], zi, sc[k]);
     };
-    stage(z0 - 1);
-    stage(z0);
+    auto store_plane = [&](const Raw<MI>* pre, int zi) {
+        u32x4* dst = lds + (zi & 3) * kPlane;
+#pragma unroll
+        for (int k = 0; k < kChunksPerThread; ++k)
+            if (sl[k] >= 0) dst[sl[k]] = finish<MI>(pre[k], a.fin, fscale);
+    };
+
+    // lane geometry of the two pair members: member j>>1, channel half j&1
+    const int m = j >> 1;
+    fetch_plane(preA, z0 - 1);
+    store_plane(preA, z0 - 1);
+    fetch_plane(preA, z0);
+    store_plane(preA, z0);
+    // prefetch depth: two planes where the raw loads fit the register budget of
+    // three blocks per CU, one plane for the two-source input stages
+    constexpr int kDepth = (MI == IN_UPADD || MI == IN_MASK) ? 1 : 2;
+    fetch_plane(preA, z0 + 1);
+    if (kDepth == 2 && z0 + 2 <= z1) fetch_plane(preB, z0 + 2);
     const int x = x0 + n;
     for (int z = z0; z < z1; ++z) {
-        stage(z + 1);
+        if constexpr (kDepth == 1) {
+            store_plane(preA, z + 1);
+            if (z + 2 <= z1) fetch_plane(preA, z + 2);
+        } else if (((z - z0) & 1) == 0) {
+            // plane z+1 comes from the buffer of its parity; refill it with plane z+3
+            store_plane(preA, z + 1);
+            if (z + 3 <= z1) fetch_plane(preA, z + 3);
+        } else {
+            store_plane(preB, z + 1);
+            if (z + 3 <= z1) fetch_plane(preB, z + 3);
+        }
         __syncthreads();
         f32x4 acc[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int s = 0; s < kSteps; ++s) {
-            const int tap = min(2 * s + (j >> 1), 26);
-            const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
-            const u32x4* pl = lds + ((z + dz - 1) & 3) * kPlane + (j & 1) * kLW + n + dx;
+        for (int p = 0; p < 5; ++p) {
+            // member geometry (the empty member of pair 4 reuses member 0's, its weights are zero)
+            const int c = pair_combo(p, m) < 9 ? pair_combo(p, m) : pair_combo(p, 0);
+            const int dz = c / 3, dx = c % 3;
+            const u32x4* pl = lds + ((z + dz - 1) & 3) * kPlane + (j & 1) * kLW + n + dx + wave * 4 * 2 * kLW;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const u32x4 v = pl[(wave * 4 + i + dy) * 2 * kLW];
-                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[s], __builtin_bit_cast(bf16x8, v), acc[i], 0, 0, 0);
+            for (int t = 0; t < 6; ++t) {
+                const bf16x8 B = __builtin_bit_cast(bf16x8, pl[t * 2 * kLW]);
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy) {
+                    const int i = t - dy;
+                    if (i >= 0 && i < 4) acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[3 * p + dy], B, acc[i], 0, 0, 0);
+                }
             }
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int y = y0 + wave * 4 + i;
-            if (x >= W || y >= H) continue;
-            if constexpr (MO == OUT_FIELD) {
-                if (j == 0) {
-                    const long long o = fidx(a.fout, b, x, y, z);
-                    const float v = acc[i][0] + bias[0];
-                    if (a.fout.is64) ((double*)a.fout.p)[o] = (double)v;
-                    else ((float*)a.fout.p)[o] = v;
-                }
-            } else {
-                float v[4];
+            if (x < W && y < H) store_out<MO>(a, acc[i], bias, b, x, y, z, j);
+        }
+    }
+}
+
+// Lift (1 -> 16 channels): K = the 27 taps of one channel (+5 zero taps), one
+// MFMA per 16 voxels; lane (voxel n, block j) gathers taps 8j .. 8j+7 from a
+// BF16 staging ring.  Input stage as IN_FIELD (scaled to unit RMS).
+__global__ __launch_bounds__(256) void k_lift(ConvArgs a, int zc)
+{
+    if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
+    constexpr int kP = kLH * kLW;
+    __shared__ uint16_t lds[4 * kP];
+    const int D = a.out.D, H = a.out.H, W = a.out.W;
+    const int nzc = (D + zc - 1) / zc;
+    const int b = blockIdx.z / nzc;
+    const int z0 = (blockIdx.z - b * nzc) * zc;
+    const int z1 = min(z0 + zc, D);
+    const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTHt;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = lane >> 4, n = lane & 15;
+    const float fscale = field_scale(a.fin);
+    const bf16x8 w = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[lane]);
+    float bias[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    v[r] = acc[i][r] + bias[r];
-                    if (MO == OUT_RELU) v[r] = fmaxf(v[r], 0.0f);
-                }
-                u32x2 o;
-                o[0] = f2bf(v[0]) | (f2bf(v[1]) << 16);
-                o[1] = f2bf(v[2]) | (f2bf(v[3]) << 16);
-                *(u32x2*)(a.out.p + vox(a.out, b, x, y, z) + 4 * j) = o;
+    for (int r = 0; r < 4; ++r) bias[r] = a.bias ? a.bias[4 * j + r] : 0.0f;
+    // taps 8j+e of this lane: plane offset dz and in-plane offset (tap 27.. reuse tap 26; zero weights)
+    int tdz[8], toff[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int tap = min(8 * j + e, 26);
+        tdz[e] = tap / 9;
+        toff[e] = ((tap / 3) % 3) * kLW + tap % 3;
+    }
+    constexpr int kPer = (kP + 255) / 256;
+    int sxx[kPer], syy[kPer], sl[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int t = tid + 256 * k;
+        sl[k] = t < kP ? t : -1;
+        syy[k] = y0 - 1 + t / kLW;
+        sxx[k] = x0 - 1 + t % kLW;
+    }
+    Raw<IN_FIELD> preA[kPer], preB[kPer];
+    auto fetch_plane = [&](Raw<IN_FIELD>* pre, int zi) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (sl[k] >= 0) pre[k] = fetch<IN_FIELD>(a.in, a.in2, a.fin, fscale, D, H, W, b, sxx[k], syy[k], zi, 0);
+    };
+    auto store_plane = [&](const Raw<IN_FIELD>* pre, int zi) {
+        uint16_t* dst = lds + (zi & 3) * kP;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (sl[k] >= 0) dst[sl[k]] = (uint16_t)finish<IN_FIELD>(pre[k], a.fin, fscale)[0];
+    };
+    fetch_plane(preA, z0 - 1);
+    store_plane(preA, z0 - 1);
+    fetch_plane(preA, z0);
+    store_plane(preA, z0);
+    fetch_plane(preA, z0 + 1);
+    if (z0 + 2 <= z1) fetch_plane(preB, z0 + 2);
+    const int x = x0 + n;
+    for (int z = z0; z < z1; ++z) {
+        // two planes in flight, as in k_conv
+        if (((z - z0) & 1) == 0) {
+            store_plane(preA, z + 1);
+            if (z + 3 <= z1) fetch_plane(preA, z + 3);
+        } else {
+            store_plane(preB, z + 1);
+            if (z + 3 <= z1) fetch_plane(preB, z + 3);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = wave * 4 + i;
+            u32x4 bw;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const unsigned lo = lds[((z + tdz[2 * e] - 1) & 3) * kP + row * kLW + n + toff[2 * e]];
+                const unsigned hi = lds[((z + tdz[2 * e + 1] - 1) & 3) * kP + row * kLW + n + toff[2 * e + 1]];
+                bw[e] = lo | (hi << 16);
             }
+            const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, __builtin_bit_cast(bf16x8, bw),
+                                                                       f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            const int y = y0 + row;
+            if (x < W && y < H) store_out<OUT_RELU>(a, acc, bias, b, x, y, z, j);
         }
     }
 }
@@ -287,6 +489,12 @@ void launch_conv(const ConvArgs& a, hipStream_t s)
     case IN_MASK: conv_mo<IN_MASK>(a, s, grid, zc); break;
     default: conv_mo<IN_FIELD>(a, s, grid, zc); break;
     }
+}
+
+void launch_lift(const ConvArgs& a, hipStream_t s)
+{
+    const int zc = pick_zc(a.out.D, a.out.H, a.out.W, a.out.B);
+    hipLaunchKernelGGL(k_lift, conv_grid(a.out.D, a.out.H, a.out.W, a.out.B, zc), dim3(256), 0, s, a, zc);
 }
 
 // ------------------------------------------------------------ weight gradient
@@ -509,22 +717,33 @@ void launch_upadd_back(const ActRef& dx, const ActRef& dskip, int accumulate_ski
 
 // ------------------------------------------------------------ weights, Adam
 
-// lane-arranged A operands: wl[s][lane][e] = W'[tap = 2s + (lane/16)/2][k = 8((lane/16)&1) + e][m = lane%16]
+// lane-arranged A operands, wl[o][lane][e] for operand o = 3*pair + dy:
+// W'[tap][k][m], m = lane%16 (output channel), k = 8*((lane/16)&1) + e (input
+// channel), tap = 9*dz + 3*dy + dx of pair member (lane/16)/2 (zero for the
+// empty member).
 // forward:        W'[tap][k][m] = W[tap][ci = k][co = m]
 // backward data:  W'[tap][k][m] = W[26 - tap][ci = m][co = k]   (flipped, transposed)
-__global__ void k_pack(const float* __restrict__ W, int cin, int cout, int transposed, uint16_t* __restrict__ wl)
+// lift (1 input channel): one operand, k = 8*(lane/16) + e is the tap
+__global__ void k_pack(const float* __restrict__ W, int cin, int cout, int mode, uint16_t* __restrict__ wl)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= kPackedHalves) return;
-    const int e = t & 7, lane = (t >> 3) & 63, s = t >> 9;
+    const int e = t & 7, lane = (t >> 3) & 63, o = t >> 9;
     const int j = lane >> 4, m = lane & 15;
-    const int tap = 2 * s + (j >> 1), k = 8 * (j & 1) + e;
     float v = 0.0f;
-    if (tap < kTaps) {
-        if (!transposed) {
-            if (k < cin && m < cout) v = W[tap * 256 + k * 16 + m];
-        } else {
-            if (m < cin && k < cout) v = W[(kTaps - 1 - tap) * 256 + m * 16 + k];
+    if (mode == 2) {
+        if (t >= 512) return;
+        const int tap = 8 * j + e;
+        if (tap < kTaps && m < cout) v = W[tap * 256 + m];
+    } else {
+        if (t >= kPackedHalves) return;
+        const int p = o / 3, dy = o % 3, c = pair_combo(p, j >> 1), k = 8 * (j & 1) + e;
+        if (c < 9) {
+            const int tap = (c / 3) * 9 + dy * 3 + c % 3;
+            if (mode == 0) {
+                if (k < cin && m < cout) v = W[tap * 256 + k * 16 + m];
+            } else {
+                if (m < cin && k < cout) v = W[(kTaps - 1 - tap) * 256 + m * 16 + k];
+            }
         }
     }
     wl[t] = (uint16_t)f2bf(v);
@@ -694,8 +913,9 @@ void Net::pack_all()
 {
     for (int l = 0; l < kLayers; ++l) {
         const float* W = params_ + (size_t)l * kLayerParams;
-        hipLaunchKernelGGL(k_pack, dim3(kPackedHalves / 256), dim3(256), 0, stream_, W, kNet[l].cin, kNet[l].cout, 0,
-                           wl_ + (size_t)l * kPackedHalves);
+        // layer 0 (one input channel) runs on the lift kernel's operand layout
+        hipLaunchKernelGGL(k_pack, dim3(kPackedHalves / 256), dim3(256), 0, stream_, W, kNet[l].cin, kNet[l].cout,
+                           l == 0 ? 2 : 0, wl_ + (size_t)l * kPackedHalves);
         hipLaunchKernelGGL(k_pack, dim3(kPackedHalves / 256), dim3(256), 0, stream_, W, kNet[l].cin, kNet[l].cout, 1,
                            wlt_ + (size_t)l * kPackedHalves);
     }
@@ -772,14 +992,31 @@ int Net::ensure(BufSet& bs, int B, const int n[3], bool train)
     bs.dact.assign(kActs, nullptr);
     int rc = 0;
     if (!train) {
-        // inference: two buffers per level, reused as soon as an activation is dead
-        static const int slot[kActs] = {0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0};
-        uint16_t* lvbuf[3][2] = {};
-        for (int l = 0; l < 3 && !rc; ++l)
-            for (int k = 0; k < 2 && !rc; ++k) rc = alloc(vox_count(l) * kC * 2, (void**)&lvbuf[l][k]);
-        if (rc) return rc;
-        for (int i = 0; i < kActs; ++i) bs.act[i] = lvbuf[kNet[i].level][slot[i]];
-        return 0;
+        // inference: an activation takes a buffer of its level that holds no live
+        // activation (live = read by a later layer), so each level keeps only
+        // as many buffers as it has activations alive at once
+        int last_use[kActs];
+        for (int i = 0; i < kActs; ++i) last_use[i] = i;
+        for (int l = 0; l < kLayers; ++l) {
+            if (kNet[l].in_act >= 0) last_use[kNet[l].in_act] = std::max(last_use[kNet[l].in_act], l);
+            if (kNet[l].skip_act >= 0) last_use[kNet[l].skip_act] = std::max(last_use[kNet[l].skip_act], l);
+        }
+        std::vector<std::pair<int, uint16_t*>> slots[3];  // (holder, buffer) per level
+        for (int i = 0; i < kActs && !rc; ++i) {
+            auto& pool = slots[kNet[i].level];
+            int pick = -1;
+            for (size_t k = 0; k < pool.size(); ++k)
+                if (last_use[pool[k].first] < i) pick = (int)k;
+            if (pick < 0) {
+                uint16_t* p = nullptr;
+                rc = alloc(vox_count(kNet[i].level) * kC * 2, (void**)&p);
+                pool.push_back({i, p});
+                pick = (int)pool.size() - 1;
+            }
+            pool[pick].first = i;
+            bs.act[i] = pool[pick].second;
+        }
+        return rc;
     }
     for (int i = 0; i < kActs && !rc; ++i) rc = alloc(vox_count(kNet[i].level) * kC * 2, (void**)&bs.act[i]);
     for (int i = 0; i < kActs && !rc; ++i) rc = alloc(vox_count(kNet[i].level) * kC * 2, (void**)&bs.dact[i]);
@@ -825,7 +1062,8 @@ void Net::forward(BufSet& bs, const FieldRef& fin, const FieldRef& fout, const i
         a.bias = params_ + (size_t)i * kLayerParams + kTaps * 256;
         a.gate = gate;
         a.gate2 = gate2;
-        launch_conv(a, stream_);
+        if (d.mode_in == IN_FIELD) launch_lift(a, stream_);
+        else launch_conv(a, stream_);
     }
 }
 

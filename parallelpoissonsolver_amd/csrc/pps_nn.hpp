@@ -2,17 +2,17 @@
 //
 // Network: 3 levels, 16 channels, 3^3 kernels, ReLU; BF16 weights and
 // activations, FP32 accumulation, FP32 master weights (Adam).
-//   layer  name  level  input                      out
-//     0    lift    0    field r (1 channel)        act0
-//     1    a1      0    act0                       act1   (skip of level 0)
-//     2    b0      1    pool(act1)                 act2
-//     3    b1      1    act2                       act3   (skip of level 1)
-//     4    c0      2    pool(act3)                 act4
-//   5..9   c1..c5  2    act(i-1)                   act5..act9
-//    10    e0      1    up(act9) + act3            act10
-//    11    e1      1    act10                      act11
-//    12    f0      0    up(act11) + act1           act12
-//    13    out     0    act12 (linear, 16 -> 1)    field z
+//   layer   name    level  input                      out
+//     0     lift      0    field r (1 channel)        act0   (skip of level 0)
+//     1     b0        1    pool(act0)                 act1
+//     2     b1        1    act1                       act2   (skip of level 1)
+//     3     c0        2    pool(act2)                 act3
+//   4..10   c1..c7    2    act(i-1)                   act4..act10
+//    11     e0        1    up(act10) + act2           act11
+//    12     e1        1    act11                      act12
+//    13     out       0    up(act12) + act0           field z (16 -> 1, linear)
+// Level 0 (the full grid) carries only the lift and the output convolution;
+// the other twelve 16 -> 16 convolutions run at 1/8 and 1/64 of the voxels.
 // pool = 2x2x2 average, up = nearest neighbour; both run inside the input
 // stage of the convolution that consumes them.  Activations are dense,
 // channels-last (16 BF16 = 32 bytes per voxel), [b][z][y][x][c].
@@ -48,12 +48,12 @@ namespace nn {
 
 constexpr int kC = 16;
 constexpr int kTaps = 27;
-constexpr int kSteps = 14;        // K = 28 taps x 16 channels in MFMA steps of 32 (tap 27 is zero)
+constexpr int kOps = 15;          // weight operands per layer: 5 tap pairs x 3 dy (K = 2 taps x 16 channels)
 constexpr int kLayers = 14;
 constexpr int kActs = 13;         // activations act0..act12
 constexpr int kLayerParams = kTaps * kC * kC + kC;  // W[tap][ci][co] then bias[co]
 constexpr int kParams = kLayers * kLayerParams;
-constexpr int kPackedHalves = kSteps * 64 * 8;      // lane-arranged BF16 weights of one layer
+constexpr int kPackedHalves = kOps * 64 * 8;        // lane-arranged BF16 weights of one layer
 
 enum InMode { IN_PLAIN = 0, IN_POOL = 1, IN_UPADD = 2, IN_MASK = 3, IN_FIELD = 4 };
 enum OutMode { OUT_RELU = 0, OUT_LINEAR = 1, OUT_FIELD = 2 };
