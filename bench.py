@@ -1,23 +1,31 @@
 #!/usr/bin/env python3
 """Benchmark: seconds to FP64 relative residual 1e-10 for the 3-D Poisson solve
-at 512^3 interior unknowns, on one MI355X.
+at 512^3 interior unknowns on one MI355X -- learned-preconditioned FCG with
+FP32 inner vectors and FP64 outer refinement, against plain FP64 CG.
 
-Workload (the mixed-precision configuration of the Krylov path): unit cube,
-7-point FD, homogeneous Dirichlet, f = seeded sum of 64 Gaussian sources
-(centres U[0,1]^3, widths U[0.01,0.1], amplitudes N(0,1)), x0 = 0, stop when
-the recursive FP64 relative residual drops below 1e-10 (baseCG.hpp:188,224).
+Workload: unit cube, 7-point FD, homogeneous Dirichlet, f = seeded sum of 64
+Gaussian sources (centres U[0,1]^3, widths U[0.01,0.1], amplitudes N(0,1)),
+x0 = 0, stop at relative residual 1e-10 (baseCG.hpp:188,224); every form's
+true FP64 residual ||b - A x|| / ||b_adj|| is checked after the solve.
 
 A step is one complete solve with the right-hand side resident in HBM: reset
 x = 0, restore b, normalise, iterate, true FP64 residual, un-normalise.  Each
 form is timed over K steps after W warm-up steps, bracketed by device
-synchronisation; `value` is the headline form's seconds per solve:
-  - mixed FP32/FP64 iterative refinement (FP32 CG inner to 1e-5, FP64 outer
-    residual and update), compared with
-  - plain FP64 CG (the reference method), reported beside it.
-The roofline entry is the dominant kernel of the headline solve, timed live
-with HIP events on the context's stream; the CPU baseline is the reference
-CPU solver (oracle/_ref, compiled from the reference sources) under MPI on the
-host's cores, timed on a bounded sample of the same workload.
+synchronisation.  Forms:
+  - ir_nnfcg (headline): FP64 outer residual/update refinement around FCG in
+    FP32 preconditioned by the learned network (BF16 MFMA convolutions), inner
+    tolerance 1e-4;
+  - nn_fcg_fp64: the same network under FP64 FCG;
+  - plain_fp64_cg: the reference method.
+The network is the committed one (parallelpoissonsolver_amd/weights/nn_64.npy,
+trained on the GPU by train_nn.py on 64^3 CG iterates); without it the bench
+trains one first (about a minute, outside the timed region).
+
+Roofline: every network layer and Krylov kernel is timed live with HIP events
+on the context's stream; the dominant kernel of the headline solve (largest
+time per iteration) is reported against 8 TB/s HBM.  CPU baseline: the
+reference CPU solver (oracle/_ref, built from the reference sources) under MPI
+on the host's cores, on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -36,6 +44,9 @@ SEED = 3
 NSRC = 64
 TOL = 1e-10
 PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0
+WEIGHTS = os.path.join(ROOT, "parallelpoissonsolver_amd", "weights", "nn_64.npy")
+LAYER_NAMES = ["lift", "b0", "b1", "c0", "c1", "c2", "c3", "c4", "c5", "c6", "c7", "e0", "e1", "out"]
 
 
 def log(msg):
@@ -44,10 +55,14 @@ def log(msg):
 
 def chains():
     import parallelpoissonsolver_amd._lib as L
-    plain = [dict(kind=L.CG, tol=TOL, maxit=200000), dict(kind=L.NONE)]
-    mixed = [dict(kind=L.IR, tol=TOL, maxit=100), dict(kind=L.CG, precision=32, tol=1e-5, maxit=200000),
-             dict(kind=L.NONE, precision=32)]
-    return {"plain_fp64_cg": plain, "ir_fp32cg_fp64": mixed}
+    return {
+        "plain_fp64_cg": [dict(kind=L.CG, tol=TOL, maxit=200000), dict(kind=L.NONE)],
+        "ir_fp32cg_fp64": [dict(kind=L.IR, tol=TOL, maxit=100), dict(kind=L.CG, precision=32, tol=1e-5, maxit=200000),
+                           dict(kind=L.NONE, precision=32)],
+        "nn_fcg_fp64": [dict(kind=L.FCG, tol=TOL, maxit=20000), dict(kind=L.NN, stall_window=50, stall_ratio=0.9)],
+        "ir_nnfcg": [dict(kind=L.IR, tol=TOL, maxit=100), dict(kind=L.FCG, precision=32, tol=1e-4, maxit=20000),
+                     dict(kind=L.NN, precision=32, stall_window=50, stall_ratio=0.9)],
+    }
 
 
 def run_form(C, chain, steps, warmup, name):
@@ -55,13 +70,13 @@ def run_form(C, chain, steps, warmup, name):
     res = None
     for w in range(warmup):
         C.fill(L.FIELD_X, 0.0)
-        C.copy_field(L.FIELD_B, L.FIELD_T0)
+        C.copy_field(L.FIELD_B, L.FIELD_T3)
         res = C.solve(chain)
         log(f"{name}: warm-up {w + 1}/{warmup}: {res['iterations']} it, {res['seconds']:.3f} s device")
     dev, wall = [], []
     for s in range(steps):
         C.fill(L.FIELD_X, 0.0)
-        C.copy_field(L.FIELD_B, L.FIELD_T0)
+        C.copy_field(L.FIELD_B, L.FIELD_T3)
         C.synchronize()
         t0 = time.perf_counter()
         res = C.solve(chain)
@@ -69,14 +84,16 @@ def run_form(C, chain, steps, warmup, name):
         wall.append(time.perf_counter() - t0)
         dev.append(res["seconds"])
         log(f"{name}: step {s + 1}/{steps}: {res['iterations']} it (outer {res['outer_iterations']}), "
-            f"true rel residual {res['error_compute_operator']:.3e}, {wall[-1]:.3f} s")
+            f"true rel residual {res['error_compute_operator']:.3e}, fallback {res['fallback_switches']}, "
+            f"{wall[-1]:.3f} s")
         # plain CG stops on the recursive residual like the reference
         # (baseCG.hpp:224), so its true residual may sit a few per cent above
         # the tolerance; a form that is off by more than that has failed
         if not res["error_compute_operator"] < 1.5 * TOL:
             raise RuntimeError(f"{name} did not reach {TOL}: {res['error_compute_operator']}")
     return {"s": sum(wall) / len(wall), "device_s": sum(dev) / len(dev), "iterations": res["iterations"],
-            "outer_iterations": res["outer_iterations"], "true_rel_residual": res["error_compute_operator"]}
+            "outer_iterations": res["outer_iterations"], "true_rel_residual": res["error_compute_operator"],
+            "fallback_switches": res["fallback_switches"]}
 
 
 KERNELS = {"cg_stencil_f64": 0, "cg_update_f64": 1, "residual_f64": 2, "copy_f64": 3, "cheb_sweep_f64": 4,
@@ -85,28 +102,40 @@ KERNELS = {"cg_stencil_f64": 0, "cg_update_f64": 1, "residual_f64": 2, "copy_f64
 
 def profiled_traffic(kernel, n):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE
-    counter summary (scaled by a plain copy kernel), or None."""
-    path = os.path.join(ROOT, "profiles", f"krylov_{n}_hbm_traffic.md")
-    try:
-        for line in open(path):
-            parts = line.split()
-            if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == kernel:
-                return float(parts[2])
-    except OSError:
-        pass
+    counter summaries (scaled by a plain copy kernel), or None."""
+    for name in (f"krylov_{n}_hbm_traffic.md", f"nn_{n}_hbm_traffic.md"):
+        path = os.path.join(ROOT, "profiles", name)
+        try:
+            for line in open(path):
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == kernel:
+                    return float(parts[2])
+        except OSError:
+            pass
     return None
 
 
-def roofline(C, kernels, reps):
+def krylov_kernels(C, reps):
     out = {}
-    for name, kid in kernels.items():
+    for name, kid in KERNELS.items():
         ms, nbytes = C.time_kernel(kid, reps)
         out[name] = {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9}
         log(f"kernel {name}: {ms:.4f} ms/launch, {out[name]['gbs']:.0f} GB/s algorithmic")
     return out
 
 
-def cpu_baseline(gpu_iterations, seconds_budget_hint):
+def nn_layers(C, reps):
+    out = {}
+    for i, name in enumerate(LAYER_NAMES):
+        ms, nbytes, flops = C.nn_time_layer(i, reps)
+        out[name] = {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9,
+                     "tflops": flops / (ms * 1e-3) / 1e12}
+        log(f"layer {name}: {ms:.4f} ms/launch, {out[name]['gbs']:.0f} GB/s, {out[name]['tflops']:.1f} TFLOP/s "
+            f"({100 * out[name]['tflops'] / PEAK_BF16_TFLOPS:.1f}% of BF16 MFMA peak)")
+    return out
+
+
+def cpu_baseline(gpu_iterations):
     """Reference CPU solver (compiled from the reference sources into oracle/_ref)
     on the host cores: 10 CG iterations of the same 512^3 workload under MPI,
     extrapolated to the iteration count the solve needs."""
@@ -147,10 +176,24 @@ def cpu_baseline(gpu_iterations, seconds_budget_hint):
     except OSError:
         pass
     return {"value": est, "unit": "s", "cores": nproc, "kind": "reference",
-            "sample": f"reference solverPoissonMPI_CPU BaseCG+NoneSolver, 512^3 Gaussian-source rhs, {nproc} MPI ranks "
-                      f"(2x2x2), {it} iterations timed ({per_it:.3f} s/iteration) + setup, extrapolated to the "
-                      f"{gpu_iterations} iterations of the full solve",
+            "sample": f"reference solverPoissonMPI_CPU BaseCG+NoneSolver (plain CG), 512^3 Gaussian-source rhs, "
+                      f"{nproc} MPI ranks (2x2x2), {it} iterations timed ({per_it:.3f} s/iteration) + setup, "
+                      f"extrapolated to the {gpu_iterations} iterations of the full plain-CG solve",
             "cpu_model": cpu_model}
+
+
+def network():
+    import parallelpoissonsolver_amd._lib as L
+    from parallelpoissonsolver_amd import train_nn
+    if os.path.exists(WEIGHTS):
+        net = L.Network(seed=0)
+        info = train_nn.load(net, WEIGHTS)
+        log(f"network: {WEIGHTS} ({net.n_effective} parameters, trained loss {info.get('loss_last', float('nan')):.4f})")
+        return net, dict(info, source="committed weights")
+    log("network: no committed weights, training on 64^3 CG iterates first")
+    net, info = train_nn.train()
+    log(f"network: trained in {info['generate_seconds'] + info['train_seconds']:.1f} s, loss {info['loss_last']:.4f}")
+    return net, dict(info, source="trained in this run")
 
 
 def main():
@@ -159,9 +202,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=N_INTERIOR, help="interior points per direction")
-    ap.add_argument("--kz", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="plain_fp64_cg,ir_fp32cg_fp64")
+    ap.add_argument("--forms", default="ir_nnfcg,nn_fcg_fp64,plain_fp64_cg")
     ap.add_argument("--reps", type=int, default=20, help="launches per kernel timing")
     ap.add_argument("--kernels-only", action="store_true", help="only the per-kernel timings")
     a = ap.parse_args()
@@ -174,39 +216,58 @@ def main():
     npg = a.n + 2
     ds = 1.0 / (npg - 1)
     C = L.Context((npg,) * 3, (ds,) * 3, (0, 0, 0), (0,) * 6)
-    if a.kz:
-        C.set_tiling(a.kz)
     C.fill(L.FIELD_X, 0.0)
     C.fill(L.FIELD_B, 0.0)
     C.rhs_gauss(SEED, NSRC, 0.01, 0.1)
-    C.copy_field(L.FIELD_T0, L.FIELD_B)
+    C.copy_field(L.FIELD_T3, L.FIELD_B)
     log(f"workload: {a.n}^3 interior, Gaussian sources (seed {SEED}, {NSRC} sources), tol {TOL}")
+    forms = a.forms.split(",")
+    needs_nn = a.kernels_only or any(f in ("ir_nnfcg", "nn_fcg_fp64") for f in forms)
+    net, net_info = network() if needs_nn else (None, None)
+    if net is not None:
+        net.attach(C)
 
-    results = {}
-    forms = chains()
     if a.kernels_only:
         # the per-kernel timings alone (short enough to run under hardware counters)
-        roofline(C, KERNELS, reps=a.reps)
+        krylov_kernels(C, a.reps)
+        nn_layers(C, a.reps)
         C.close()
         return
-    for name in a.forms.split(","):
-        results[name] = run_form(C, forms[name], a.steps, a.warmup, name)
 
-    headline = "ir_fp32cg_fp64" if "ir_fp32cg_fp64" in results else list(results)[0]
-    kern = roofline(C, KERNELS, reps=a.reps)
-    # dominant kernel of the headline solve: per CG iteration the fused stencil pass
-    # moves 6 words per point (r, p, x in; p, Ap, x out), the update pass 3
-    dom = "cg_stencil_f32" if headline == "ir_fp32cg_fp64" else "cg_stencil_f64"
-    k = kern[dom]
+    results = {}
+    all_forms = chains()
+    for name in forms:
+        results[name] = run_form(C, all_forms[name], a.steps, a.warmup, name)
+
+    headline = forms[0]
+    kern = krylov_kernels(C, a.reps)
+    layers = nn_layers(C, a.reps) if net is not None else {}
+    # dominant kernel of the headline solve: largest device time per iteration.
+    # Per learned-FCG iteration: one network apply (each layer once) and the FP32
+    # FCG kernels (gamma + stencil pass + update: the CG stencil/update pair is
+    # their timing proxy); per plain-CG iteration: the fused stencil and update.
+    if headline in ("ir_nnfcg", "nn_fcg_fp64"):
+        per_it = {f"nn_{k}": v for k, v in layers.items()}
+        sfx = "f32" if headline == "ir_nnfcg" else "f64"
+        per_it[f"cg_stencil_{sfx}"] = kern[f"cg_stencil_{sfx}"]
+        per_it[f"cg_update_{sfx}"] = kern[f"cg_update_{sfx}"]
+    elif headline == "ir_fp32cg_fp64":
+        per_it = {k: kern[k] for k in ("cg_stencil_f32", "cg_update_f32")}
+    else:
+        per_it = {k: kern[k] for k in ("cg_stencil_f64", "cg_update_f64")}
+    dom = max(per_it, key=lambda k: per_it[k]["ms"])
+    k = per_it[dom]
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(k["gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(k["gbs"] / PEAK_HBM_GBS, 4), "traffic": profiled_traffic(dom, a.n),
-            "bytes_per_launch": k["bytes"], "ms_per_launch": k["ms"]}
+            "bytes_per_launch": k["bytes"], "ms_per_launch": round(k["ms"], 4)}
+    if "tflops" in k:
+        roof["mfma_tflops"] = round(k["tflops"], 2)
 
     cpu = None
     if not a.no_cpu and a.n == N_INTERIOR:
         try:
-            cpu = cpu_baseline(results["plain_fp64_cg"]["iterations"] if "plain_fp64_cg" in results
-                               else results[headline]["iterations"], 30)
+            ref_it = results["plain_fp64_cg"]["iterations"] if "plain_fp64_cg" in results else 2406
+            cpu = cpu_baseline(ref_it)
         except Exception as e:  # the baseline is reported, never the product
             log(f"CPU baseline error: {e}")
     C.close()
@@ -223,14 +284,18 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f32+f64" if headline != "plain_fp64_cg" else "f64",
+        "dtype": {"ir_nnfcg": "f32+f64+bf16", "nn_fcg_fp64": "f64+bf16", "ir_fp32cg_fp64": "f32+f64"}.get(headline, "f64"),
         "data": "synthetic",
         "config": {"workload": f"poisson3d_{a.n}^3_dirichlet_gauss{NSRC}", "n_interior": a.n, "rhs_seed": SEED,
-                   "solver": headline, "tol": TOL, "kz": a.kz or 32},
+                   "solver": headline, "tol": TOL},
         "forms": results,
         "ratio_headline_vs_plain": (round(hv["s"] / results["plain_fp64_cg"]["s"], 4)
                                     if "plain_fp64_cg" in results else None),
+        "network": ({k2: net_info[k2] for k2 in ("source", "n", "pairs", "steps", "loss_last", "params_effective")
+                     if k2 in net_info} if net_info else None),
         "kernels": {k2: {"gbs": round(v["gbs"], 1), "ms": round(v["ms"], 4)} for k2, v in kern.items()},
+        "nn_layers": {k2: {"gbs": round(v["gbs"], 1), "tflops": round(v["tflops"], 2), "ms": round(v["ms"], 4)}
+                      for k2, v in layers.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

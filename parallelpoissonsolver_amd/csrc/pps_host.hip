@@ -1795,6 +1795,30 @@ int pps_nn_apply_field(void* ctx, int out, int in)
     return C->hip(hipStreamSynchronize(C->stream), "sync");
 }
 
+int pps_nn_time_layer(void* ctx, int layer, int reps, double* ms, double* bytes, double* flops)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C || !ms || !bytes || !flops) return PPS_ERR_ARG;
+    // an apply on T0 <- B sets up the activations the layer reads
+    int rc = pps_nn_apply_field(ctx, PPS_FIELD_T0, PPS_FIELD_B);
+    if (rc) return rc;
+    Field fo, fi;
+    if ((rc = field_of(*C, PPS_FIELD_T0, fo)) || (rc = field_of(*C, PPS_FIELD_B, fi))) return rc;
+    Scal* S = C->scal(C->nscal - 1);
+    NNFieldGeom g;
+    g.off = C->L64.at(C->solver.lo[0], C->solver.lo[1], C->solver.lo[2]);
+    g.sy = C->L64.px;
+    g.sz = C->L64.plane;
+    g.n0 = C->solver.hi[0] - C->solver.lo[0];
+    g.n1 = C->solver.hi[1] - C->solver.lo[1];
+    g.n2 = C->solver.hi[2] - C->solver.lo[2];
+    auto* net = dynamic_cast<nn::Net*>(C->nn.get());
+    if (!net) return C->fail(PPS_ERR_STATE, "no network attached");
+    if (net->time_layer(C->stream, g, fo.p, 64, fi.p, 64, &S->v[S_RR], layer, reps, ms, bytes, flops))
+        return C->fail(PPS_ERR_ARG, "%s", net->error());
+    return PPS_OK;
+}
+
 int pps_nn_attach(void* ctx, void* nn)
 {
     Ctx* C = as_ctx(ctx);

@@ -49,17 +49,6 @@ typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-struct ConvArgs {
-    int mode_in, mode_out;
-    ActRef in, in2;   // in2: skip (UPADD) or ReLU mask (MASK); POOL reads the finer level through `in`
-    FieldRef fin, fout;
-    ActRef out;       // output activation; its extent is the level's (p unused for OUT_FIELD)
-    const uint16_t* wl;
-    const float* bias;
-    const int* gate;
-    const int* gate2;
-};
-
 struct DwArgs {
     int mode_in;      // the layer's input, as in the forward pass
     ActRef in, in2;
@@ -1046,25 +1035,34 @@ int Net::ensure(BufSet& bs, int B, const int n[3], bool train)
     return rc;
 }
 
+ConvArgs Net::layer_args(const BufSet& bs, int i, const FieldRef& fin, const FieldRef& fout, const int* gate,
+                         const int* gate2) const
+{
+    const LayerDef& d = kNet[i];
+    ConvArgs a{};
+    a.mode_in = d.mode_in;
+    a.mode_out = i == kLayers - 1 ? OUT_FIELD : OUT_RELU;
+    a.out = i < kActs ? act(bs, i) : level(bs, 0, nullptr);
+    if (d.mode_in == IN_FIELD) a.fin = fin;
+    else a.in = act(bs, d.in_act);
+    if (d.mode_in == IN_UPADD) a.in2 = act(bs, d.skip_act);
+    a.fout = fout;
+    a.wl = wl_ + (size_t)i * kPackedHalves;
+    a.bias = params_ + (size_t)i * kLayerParams + kTaps * 256;
+    a.gate = gate;
+    a.gate2 = gate2;
+    return a;
+}
+
+void Net::launch_layer(const ConvArgs& a, int i)
+{
+    if (kNet[i].mode_in == IN_FIELD) launch_lift(a, stream_);
+    else launch_conv(a, stream_);
+}
+
 void Net::forward(BufSet& bs, const FieldRef& fin, const FieldRef& fout, const int* gate, const int* gate2)
 {
-    for (int i = 0; i < kLayers; ++i) {
-        const LayerDef& d = kNet[i];
-        ConvArgs a{};
-        a.mode_in = d.mode_in;
-        a.mode_out = i == kLayers - 1 ? OUT_FIELD : OUT_RELU;
-        a.out = i < kActs ? act(bs, i) : level(bs, 0, nullptr);
-        if (d.mode_in == IN_FIELD) a.fin = fin;
-        else a.in = act(bs, d.in_act);
-        if (d.mode_in == IN_UPADD) a.in2 = act(bs, d.skip_act);
-        a.fout = fout;
-        a.wl = wl_ + (size_t)i * kPackedHalves;
-        a.bias = params_ + (size_t)i * kLayerParams + kTaps * 256;
-        a.gate = gate;
-        a.gate2 = gate2;
-        if (d.mode_in == IN_FIELD) launch_lift(a, stream_);
-        else launch_conv(a, stream_);
-    }
+    for (int i = 0; i < kLayers; ++i) launch_layer(layer_args(bs, i, fin, fout, gate, gate2), i);
 }
 
 void Net::backward(BufSet& bs, const FieldRef& fin)
@@ -1140,15 +1138,10 @@ static FieldRef dense_field(float* p, const int n[3])
     return f;
 }
 
-int Net::apply(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
-               const int* gate, const int* gate2, const double* rr)
+void Net::field_refs(const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec, const double* rr,
+                     FieldRef& fi, FieldRef& fo)
 {
-    const int n[3] = {g.n0, g.n1, g.n2};
-    int rc = ensure(inf_, 1, n, false);
-    if (rc) return rc;
-    const hipStream_t own = stream_;
-    stream_ = s;
-    FieldRef fi, fo;
+    fi = FieldRef{};
     fi.p = const_cast<void*>(in);
     fi.is64 = in_prec == 64;
     fi.off = g.off;
@@ -1162,9 +1155,69 @@ int Net::apply(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, con
     fo.p = out;
     fo.is64 = out_prec == 64;
     fo.scale_sum = nullptr;
+}
+
+int Net::apply(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
+               const int* gate, const int* gate2, const double* rr)
+{
+    const int n[3] = {g.n0, g.n1, g.n2};
+    int rc = ensure(inf_, 1, n, false);
+    if (rc) return rc;
+    const hipStream_t own = stream_;
+    stream_ = s;
+    FieldRef fi, fo;
+    field_refs(g, out, out_prec, in, in_prec, rr, fi, fo);
     forward(inf_, fi, fo, gate, gate2);
     stream_ = own;
     return hip(hipGetLastError(), "network launch");
+}
+
+int Net::time_layer(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
+                    const double* rr, int i, int reps, double* ms, double* bytes, double* flops)
+{
+    if (i < 0 || i >= kLayers || reps < 1) return fail("time_layer: layer %d / reps %d out of range", i, reps);
+    const int n[3] = {g.n0, g.n1, g.n2};
+    if (!(inf_.B == 1 && inf_.n[0] == n[0] && inf_.n[1] == n[1] && inf_.n[2] == n[2] && !inf_.train))
+        return fail("time_layer: run apply() on this region first");
+    FieldRef fi, fo;
+    field_refs(g, out, out_prec, in, in_prec, rr, fi, fo);
+    const hipStream_t own = stream_;
+    stream_ = s;
+    const ConvArgs a = layer_args(inf_, i, fi, fo, nullptr, nullptr);
+    hipEvent_t e0, e1;
+    int rc = hip(hipEventCreate(&e0), "event");
+    if (!rc) rc = hip(hipEventCreate(&e1), "event");
+    if (!rc) {
+        launch_layer(a, i);  // warm-up
+        (void)hipEventRecord(e0, s);
+        for (int k = 0; k < reps; ++k) launch_layer(a, i);
+        (void)hipEventRecord(e1, s);
+        rc = hip(hipEventSynchronize(e1), "event sync");
+    }
+    float t = 0.0f;
+    if (!rc) rc = hip(hipEventElapsedTime(&t, e0, e1), "elapsed");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    stream_ = own;
+    if (rc) return rc;
+    *ms = t / reps;
+    // algorithmic traffic: every input value read once, every output written once
+    const LayerDef& d = kNet[i];
+    double vox[3];
+    for (int l = 0; l < 3; ++l) vox[l] = (double)inf_.dims[l][0] * inf_.dims[l][1] * inf_.dims[l][2];
+    const double v = vox[d.level];
+    const double act_b = kC * 2.0;
+    double rd = 0.0, wr = 0.0;
+    switch (d.mode_in) {
+    case IN_FIELD: rd = v * (in_prec == 64 ? 8.0 : 4.0); break;
+    case IN_POOL: rd = vox[d.level - 1] * act_b; break;
+    case IN_UPADD: rd = vox[d.level + 1] * act_b + v * act_b; break;
+    default: rd = v * act_b; break;
+    }
+    wr = (i == kLayers - 1) ? v * (out_prec == 64 ? 8.0 : 4.0) : v * act_b;
+    *bytes = rd + wr;
+    *flops = 2.0 * kTaps * d.cin * d.cout * v;
+    return 0;
 }
 
 int Net::forward_host(const float* in, int B, const int n[3], float* out)

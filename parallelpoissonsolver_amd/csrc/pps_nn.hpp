@@ -81,6 +81,17 @@ struct ActRef {
     int B = 0, D = 0, H = 0, W = 0;
 };
 
+struct ConvArgs {
+    int mode_in, mode_out;
+    ActRef in, in2;   // in2: skip (UPADD) or ReLU mask (MASK); POOL reads the finer level through `in`
+    FieldRef fin, fout;
+    ActRef out;       // output activation; its extent is the level's (p unused for OUT_FIELD)
+    const uint16_t* wl;
+    const float* bias;
+    const int* gate;
+    const int* gate2;
+};
+
 struct BufSet {
     int B = 0, n[3] = {0, 0, 0};  // batch, level-0 extent (x, y, z)
     bool train = false;
@@ -116,6 +127,11 @@ public:
     int apply(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
               const int* gate, const int* gate2, const double* rr) override;
     const char* error() const override { return err_.c_str(); }
+    // average device time of `reps` launches of layer `layer` of the inference
+    // pass over the region (run apply() on the same region first); algorithmic
+    // HBM bytes and FLOPs of one launch
+    int time_layer(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
+                   const double* rr, int layer, int reps, double* ms, double* bytes, double* flops);
 
     int set_params(const float* host, long long n);
     int get_params(float* host, long long n);
@@ -138,6 +154,11 @@ private:
     int ensure(BufSet& bs, int B, const int n[3], bool train);
     void release(BufSet& bs);
     void forward(BufSet& bs, const FieldRef& fin, const FieldRef& fout, const int* gate, const int* gate2);
+    ConvArgs layer_args(const BufSet& bs, int i, const FieldRef& fin, const FieldRef& fout, const int* gate,
+                        const int* gate2) const;
+    void launch_layer(const ConvArgs& a, int i);
+    static void field_refs(const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec, const double* rr,
+                           FieldRef& fi, FieldRef& fo);
     void backward(BufSet& bs, const FieldRef& fin);
     void loss(BufSet& bs, int B, long long nvox);
     void pack_all();
