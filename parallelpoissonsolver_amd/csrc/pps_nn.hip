@@ -416,6 +416,10 @@ __global__ __launch_bounds__(256) void k_lift(ConvArgs a, int zc)
     fetch_plane(preA, z0 + 1);
     if (z0 + 2 <= z1) fetch_plane(preB, z0 + 2);
     const int x = x0 + n;
+    // pooled copy (2x2x2 average of the stored BF16 outputs) for the next level:
+    // x pairs are neighbouring lanes, y pairs neighbouring rows of this wave,
+    // z pairs consecutive planes (chunks start on even planes)
+    float ps[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
     for (int z = z0; z < z1; ++z) {
         // two planes in flight, as in k_conv
         if (((z - z0) & 1) == 0) {
@@ -426,20 +430,49 @@ __global__ __launch_bounds__(256) void k_lift(ConvArgs a, int zc)
             if (z + 3 <= z1) fetch_plane(preB, z + 3);
         }
         __syncthreads();
+        const uint16_t* slot[3] = {lds + ((z - 1) & 3) * kP, lds + (z & 3) * kP, lds + ((z + 1) & 3) * kP};
+        const uint16_t* pe[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) pe[e] = slot[tdz[e]] + toff[e] + n + wave * 4 * kLW;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int row = wave * 4 + i;
             u32x4 bw;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const unsigned lo = lds[((z + tdz[2 * e] - 1) & 3) * kP + row * kLW + n + toff[2 * e]];
-                const unsigned hi = lds[((z + tdz[2 * e + 1] - 1) & 3) * kP + row * kLW + n + toff[2 * e + 1]];
-                bw[e] = lo | (hi << 16);
-            }
+            for (int e = 0; e < 4; ++e) bw[e] = (unsigned)pe[2 * e][i * kLW] | ((unsigned)pe[2 * e + 1][i * kLW] << 16);
             const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, __builtin_bit_cast(bf16x8, bw),
                                                                        f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-            const int y = y0 + row;
-            if (x < W && y < H) store_out<OUT_RELU>(a, acc, bias, b, x, y, z, j);
+            const int y = y0 + wave * 4 + i;
+            const bool in = x < W && y < H;
+            unsigned hb[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) hb[r] = f2bf(fmaxf(acc[r] + bias[r], 0.0f));
+            if (in) {
+                u32x2 o;
+                o[0] = hb[0] | (hb[1] << 16);
+                o[1] = hb[2] | (hb[3] << 16);
+                *(u32x2*)(a.out.p + vox(a.out, b, x, y, z) + 4 * j) = o;
+            }
+            if (a.pout.p) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const float v = in ? bf2f(hb[r]) : 0.0f;
+                    ps[i >> 1][r] += v + __shfl_xor(v, 1);
+                }
+            }
+        }
+        if (a.pout.p && ((z & 1) || z == D - 1)) {
+#pragma unroll
+            for (int ip = 0; ip < 2; ++ip) {
+                const int yf = y0 + wave * 4 + 2 * ip;
+                if (!(n & 1) && x < W && yf < H) {
+                    u32x2 o;
+                    o[0] = f2bf(0.125f * ps[ip][0]) | (f2bf(0.125f * ps[ip][1]) << 16);
+                    o[1] = f2bf(0.125f * ps[ip][2]) | (f2bf(0.125f * ps[ip][3]) << 16);
+                    *(u32x2*)(a.pout.p + vox(a.pout, b, x >> 1, yf >> 1, z >> 1) + 4 * j) = o;
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) ps[ip][r] = 0.0f;
+            }
         }
     }
 }
@@ -450,6 +483,7 @@ static int pick_zc(int D, int H, int W, int B)
     const long long tiles = (long long)((W + kTW - 1) / kTW) * ((H + kTHt - 1) / kTHt) * B;
     int zc = D;
     while (zc > 16 && tiles * ((D + zc - 1) / zc) < 8192) zc = (zc + 1) / 2;
+    if ((zc & 1) && zc < D) ++zc;  // chunks start on even planes (z pairs of the pooled lift output)
     return std::max(zc, 1);
 }
 static dim3 conv_grid(int D, int H, int W, int B, int zc)
@@ -712,7 +746,7 @@ void launch_upadd_back(const ActRef& dx, const ActRef& dskip, int accumulate_ski
 // empty member).
 // forward:        W'[tap][k][m] = W[tap][ci = k][co = m]
 // backward data:  W'[tap][k][m] = W[26 - tap][ci = m][co = k]   (flipped, transposed)
-// lift (1 input channel): one operand, k = 8*(lane/16) + e is the tap
+// lift (mode 2, 1 input channel): one operand, k = 8*(lane/16) + e is the tap
 __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int mode, uint16_t* __restrict__ wl)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -902,7 +936,7 @@ void Net::pack_all()
 {
     for (int l = 0; l < kLayers; ++l) {
         const float* W = params_ + (size_t)l * kLayerParams;
-        // layer 0 (one input channel) runs on the lift kernel's operand layout
+        // the first layer (one input channel) runs on the lift kernel's operand layout
         hipLaunchKernelGGL(k_pack, dim3(kPackedHalves / 256), dim3(256), 0, stream_, W, kNet[l].cin, kNet[l].cout,
                            l == 0 ? 2 : 0, wl_ + (size_t)l * kPackedHalves);
         hipLaunchKernelGGL(k_pack, dim3(kPackedHalves / 256), dim3(256), 0, stream_, W, kNet[l].cin, kNet[l].cout, 1,
@@ -979,7 +1013,7 @@ int Net::ensure(BufSet& bs, int B, const int n[3], bool train)
     };
     bs.act.assign(kActs, nullptr);
     bs.dact.assign(kActs, nullptr);
-    int rc = 0;
+    int rc = alloc(vox_count(1) * kC * 2, (void**)&bs.pool0);
     if (!train) {
         // inference: an activation takes a buffer of its level that holds no live
         // activation (live = read by a later layer), so each level keeps only
@@ -1046,6 +1080,12 @@ ConvArgs Net::layer_args(const BufSet& bs, int i, const FieldRef& fin, const Fie
     if (d.mode_in == IN_FIELD) a.fin = fin;
     else a.in = act(bs, d.in_act);
     if (d.mode_in == IN_UPADD) a.in2 = act(bs, d.skip_act);
+    if (i == 0) a.pout = level(bs, 1, bs.pool0);
+    if (d.mode_in == IN_POOL && d.in_act == 0) {
+        // the lift kernel has written pool(act0): read it as it is
+        a.mode_in = IN_PLAIN;
+        a.in = level(bs, 1, bs.pool0);
+    }
     a.fout = fout;
     a.wl = wl_ + (size_t)i * kPackedHalves;
     a.bias = params_ + (size_t)i * kLayerParams + kTaps * 256;
@@ -1080,6 +1120,10 @@ void Net::backward(BufSet& bs, const FieldRef& fin)
         if (d.mode_in == IN_FIELD) w.fin = fin;
         else w.in = act(bs, d.in_act);
         if (d.mode_in == IN_UPADD) w.in2 = act(bs, d.skip_act);
+        if (d.mode_in == IN_POOL && d.in_act == 0) {
+            w.mode_in = IN_PLAIN;  // the forward input: pool(act0) as the lift kernel wrote it
+            w.in = level(bs, 1, bs.pool0);
+        }
         if (i == kLayers - 1) {
             w.dz_mode = 2;
             w.fdy = fdy;
@@ -1210,11 +1254,12 @@ int Net::time_layer(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec
     double rd = 0.0, wr = 0.0;
     switch (d.mode_in) {
     case IN_FIELD: rd = v * (in_prec == 64 ? 8.0 : 4.0); break;
-    case IN_POOL: rd = vox[d.level - 1] * act_b; break;
+    case IN_POOL: rd = (d.in_act == 0 ? v : vox[d.level - 1]) * act_b; break;  // b0 reads the lift's pooled copy
     case IN_UPADD: rd = vox[d.level + 1] * act_b + v * act_b; break;
     default: rd = v * act_b; break;
     }
     wr = (i == kLayers - 1) ? v * (out_prec == 64 ? 8.0 : 4.0) : v * act_b;
+    if (i == 0) wr += vox[1] * act_b;  // the pooled copy
     *bytes = rd + wr;
     *flops = 2.0 * kTaps * d.cin * d.cout * v;
     return 0;

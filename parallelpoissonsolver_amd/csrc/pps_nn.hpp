@@ -86,6 +86,7 @@ struct ConvArgs {
     ActRef in, in2;   // in2: skip (UPADD) or ReLU mask (MASK); POOL reads the finer level through `in`
     FieldRef fin, fout;
     ActRef out;       // output activation; its extent is the level's (p unused for OUT_FIELD)
+    ActRef pout;      // lift: 2x2x2-pooled copy of the output (p = null: none)
     const uint16_t* wl;
     const float* bias;
     const int* gate;
@@ -98,6 +99,7 @@ struct BufSet {
     int dims[3][3] = {};          // per level: D, H, W
     std::vector<uint16_t*> act, dact;
     uint16_t* dx[3] = {nullptr, nullptr, nullptr};  // backward-data scratch per level
+    uint16_t* pool0 = nullptr;                      // pool(act0), written by the lift kernel
     float *in = nullptr, *y = nullptr, *dy = nullptr, *ky = nullptr;
     double *g = nullptr, *lpart = nullptr, *coef = nullptr, *lossv = nullptr;
     float* part = nullptr;
