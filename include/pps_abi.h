@@ -125,6 +125,8 @@ int pps_set_neumann_flux(void* ctx, int face, const double* host_plane);
 
 /* ---- the solve (BaseCG / BiCGSTAB / ChebyshevIteration operator()) ------ */
 int pps_solve(void* ctx, const pps_stage* chain, int nstages, pps_result* res);
+/* residual history of the last solve (errorFromIterationHistory_); returns the
+ * number of entries copied (<= cap) or a negative error */
 int pps_get_history(void* ctx, double* out, int cap);
 /* wait for the context's stream */
 int pps_synchronize(void* ctx);
