@@ -67,12 +67,21 @@ constexpr int kZPlane = kTHt * 2 * kTW;      // chunks of one staged gradient pl
 
 // ------------------------------------------------------------ BF16 helpers
 
-__host__ __device__ __forceinline__ unsigned f2bf(float f)
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32): one value in the low
+// half, or two values packed (a low, b high)
+__device__ __forceinline__ unsigned f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
+__device__ __forceinline__ unsigned pack2bf(float a, float b)
 {
-    unsigned u;
-    memcpy(&u, &f, 4);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return ((u >> 16) | 0x40u) & 0xffffu;  // quiet NaN
-    return ((u + 0x7fffu + ((u >> 16) & 1u)) >> 16) & 0xffffu;                  // round to nearest even
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+}
+// value of the neighbouring lane (lane ^ 1) by a DPP quad permute
+__device__ __forceinline__ float xor1(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
 }
 __device__ __forceinline__ float bf2f(unsigned h) { return __uint_as_float(h << 16); }
 __device__ __forceinline__ void unpack8(const u32x4 c, float* f)
@@ -87,7 +96,7 @@ __device__ __forceinline__ u32x4 pack8(const float* f)
 {
     u32x4 c;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) c[i] = f2bf(f[2 * i]) | (f2bf(f[2 * i + 1]) << 16);
+    for (int i = 0; i < 4; ++i) c[i] = pack2bf(f[2 * i], f[2 * i + 1]);
     return c;
 }
 __device__ __forceinline__ bool bf_pos(unsigned h) { return (h & 0x8000u) == 0 && (h & 0x7fffu) != 0; }
@@ -247,8 +256,8 @@ __device__ __forceinline__ void store_out(const ConvArgs& a, const f32x4& acc, c
             if (MO == OUT_RELU) v[r] = fmaxf(v[r], 0.0f);
         }
         u32x2 o;
-        o[0] = f2bf(v[0]) | (f2bf(v[1]) << 16);
-        o[1] = f2bf(v[2]) | (f2bf(v[3]) << 16);
+        o[0] = pack2bf(v[0], v[1]);
+        o[1] = pack2bf(v[2], v[3]);
         *(u32x2*)(a.out.p + vox(a.out, b, x, y, z) + 4 * j) = o;
     }
 }
@@ -430,33 +439,33 @@ __global__ __launch_bounds__(256) void k_lift(ConvArgs a, int zc)
             if (z + 3 <= z1) fetch_plane(preB, z + 3);
         }
         __syncthreads();
-        const uint16_t* slot[3] = {lds + ((z - 1) & 3) * kP, lds + (z & 3) * kP, lds + ((z + 1) & 3) * kP};
-        const uint16_t* pe[8];
+        // LDS offsets of this lane's 8 taps for the plane (integer offsets keep the
+        // reads on the LDS path)
+        int pe[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) pe[e] = slot[tdz[e]] + toff[e] + n + wave * 4 * kLW;
+        for (int e = 0; e < 8; ++e) pe[e] = ((z + tdz[e] - 1) & 3) * kP + toff[e] + n + wave * 4 * kLW;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             u32x4 bw;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) bw[e] = (unsigned)pe[2 * e][i * kLW] | ((unsigned)pe[2 * e + 1][i * kLW] << 16);
+            for (int e = 0; e < 4; ++e)
+                bw[e] = (unsigned)lds[pe[2 * e] + i * kLW] | ((unsigned)lds[pe[2 * e + 1] + i * kLW] << 16);
             const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, __builtin_bit_cast(bf16x8, bw),
                                                                        f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
             const int y = y0 + wave * 4 + i;
             const bool in = x < W && y < H;
-            unsigned hb[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) hb[r] = f2bf(fmaxf(acc[r] + bias[r], 0.0f));
-            if (in) {
-                u32x2 o;
-                o[0] = hb[0] | (hb[1] << 16);
-                o[1] = hb[2] | (hb[3] << 16);
+            u32x2 o;
+            o[0] = pack2bf(fmaxf(acc[0] + bias[0], 0.0f), fmaxf(acc[1] + bias[1], 0.0f));
+            o[1] = pack2bf(fmaxf(acc[2] + bias[2], 0.0f), fmaxf(acc[3] + bias[3], 0.0f));
+            if (in && a.out.p)  // inference keeps only the pooled copy
                 *(u32x2*)(a.out.p + vox(a.out, b, x, y, z) + 4 * j) = o;
-            }
             if (a.pout.p) {
+                float v[4] = {bf2f(o[0] & 0xffffu), __uint_as_float(o[0] & 0xffff0000u), bf2f(o[1] & 0xffffu),
+                              __uint_as_float(o[1] & 0xffff0000u)};
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const float v = in ? bf2f(hb[r]) : 0.0f;
-                    ps[i >> 1][r] += v + __shfl_xor(v, 1);
+                    if (!in) v[r] = 0.0f;
+                    ps[i >> 1][r] += v[r] + xor1(v[r]);
                 }
             }
         }
@@ -466,8 +475,8 @@ __global__ __launch_bounds__(256) void k_lift(ConvArgs a, int zc)
                 const int yf = y0 + wave * 4 + 2 * ip;
                 if (!(n & 1) && x < W && yf < H) {
                     u32x2 o;
-                    o[0] = f2bf(0.125f * ps[ip][0]) | (f2bf(0.125f * ps[ip][1]) << 16);
-                    o[1] = f2bf(0.125f * ps[ip][2]) | (f2bf(0.125f * ps[ip][3]) << 16);
+                    o[0] = pack2bf(0.125f * ps[ip][0], 0.125f * ps[ip][1]);
+                    o[1] = pack2bf(0.125f * ps[ip][2], 0.125f * ps[ip][3]);
                     *(u32x2*)(a.pout.p + vox(a.pout, b, x >> 1, yf >> 1, z >> 1) + 4 * j) = o;
                 }
 #pragma unroll
@@ -518,6 +527,202 @@ void launch_lift(const ConvArgs& a, hipStream_t s)
 {
     const int zc = pick_zc(a.out.D, a.out.H, a.out.W, a.out.B);
     hipLaunchKernelGGL(k_lift, conv_grid(a.out.D, a.out.H, a.out.W, a.out.B, zc), dim3(256), 0, s, a, zc);
+}
+
+// Inference form of the output layer with the lift fused in:
+//   z = out(up(E1) + relu(lift(r)))
+// so the 16-channel level-0 activation never goes to HBM.  Per plane s of the
+// 18 x 18 region (output tile + halo), in groups of 16 voxels:
+//   1. lift: two MFMAs (K = 27 taps of the BF16 input ring) whose weight rows
+//      are permuted so that lane (n, j < 2) receives channels 8j .. 8j+7 of
+//      voxel n -- the B layout of the next step, no lane exchange needed;
+//      relu, BF16, + up(E1) (BF16 sum, as the IN_UPADD input stage): X;
+//   2. tap partials P[tap][v] = sum_ci W[tap][ci] X[v][ci]: two MFMAs (rows =
+//      taps 0-15, 16-31; k = ci padded to 32), stored tap-major in LDS;
+// then every thread adds the 27 shifted partials of its voxel into three
+// running output planes; plane s completes output plane s-1.  Values equal the
+// unfused path's up to FP32 summation order (same BF16 roundings).
+__global__ __launch_bounds__(256, 3) void k_outf(ConvArgs a, int zc)
+{
+    if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
+    constexpr int kRW = kTW + 4, kRH = kTHt + 4;  // input ring plane: tile + 2-voxel halo
+    constexpr int kRP = kRW * kRH;
+    constexpr int kQ = kLH * kLW;                 // X region: tile + 1-voxel halo
+    constexpr int kGroups = (kQ + 15) / 16;
+    constexpr int kGW = (kGroups + 3) / 4;        // groups per wavefront
+    __shared__ uint16_t lr[4 * kRP];
+    __shared__ float lp[32 * kQ];                 // P[tap][voxel]
+    const int D = a.out.D, H = a.out.H, W = a.out.W;
+    const int nzc = (D + zc - 1) / zc;
+    const int b = blockIdx.z / nzc;
+    const int z0 = (blockIdx.z - b * nzc) * zc;
+    const int z1 = min(z0 + zc, D);
+    const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTHt;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = lane >> 4, n = lane & 15;
+    const bool lo2 = j < 2;  // lanes holding channels 8j .. 8j+7
+    const float fscale = field_scale(a.fin);
+    const bf16x8 wl1 = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl2)[lane]);
+    const bf16x8 wl2 = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl2)[64 + lane]);
+    const bf16x8 w0 = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[lane]);
+    const bf16x8 w1 = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[64 + lane]);
+    float bl[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) bl[e] = (lo2 && a.bias2) ? a.bias2[8 * j + e] : 0.0f;
+    const float bout = a.bias ? a.bias[0] : 0.0f;
+    // lift taps 8j'+e of this lane (B operand k-block j' = j): plane offset and in-plane offset
+    int tdz[8], toff[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int tap = min(8 * j + e, 26);
+        tdz[e] = tap / 9;
+        toff[e] = ((tap / 3) % 3) * kRW + tap % 3;
+    }
+    // per group of this wave: ring offset of the voxel, domain test, coarse offset
+    // (inference runs one sample: the coarse level's element offsets fit in 32 bits)
+    int gbase[kGW], gq[kGW], gco[kGW];
+    unsigned gdom = 0u;
+    const int cplane = a.in.H * a.in.W * kC;  // coarse plane stride
+#pragma unroll
+    for (int k = 0; k < kGW; ++k) {
+        const int g = wave + 4 * k;
+        const int q = 16 * g + n;
+        const int qq = q < kQ ? q : kQ - 1;
+        const int yy = qq / kLW, xx = qq - yy * kLW;
+        const int gx = x0 - 1 + xx, gy = y0 - 1 + yy;
+        gq[k] = q;
+        gbase[k] = yy * kRW + xx;
+        if (g < kGroups && q < kQ && gx >= 0 && gy >= 0 && gx < W && gy < H) gdom |= 1u << k;
+        gco[k] = ((gy >> 1) * a.in.W + (gx >> 1)) * kC + 8 * j;
+    }
+    constexpr int kPer = (kRP + 255) / 256;
+    int sxx[kPer], syy[kPer], sl[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int t = tid + 256 * k;
+        sl[k] = t < kRP ? t : -1;
+        syy[k] = y0 - 2 + t / kRW;
+        sxx[k] = x0 - 2 + t % kRW;
+    }
+    Raw<IN_FIELD> preA[kPer], preB[kPer];
+    auto fetch_plane = [&](Raw<IN_FIELD>* pre, int zi) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (sl[k] >= 0) pre[k] = fetch<IN_FIELD>(a.in, a.in2, a.fin, fscale, D, H, W, b, sxx[k], syy[k], zi, 0);
+    };
+    auto store_plane = [&](const Raw<IN_FIELD>* pre, int zi) {
+        uint16_t* dst = lr + (zi & 3) * kRP;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (sl[k] >= 0) dst[sl[k]] = (uint16_t)finish<IN_FIELD>(pre[k], a.fin, fscale)[0];
+    };
+    // the coarse activation up(E1) of this wave's groups, loaded one plane ahead
+    u32x4 e1v[kGW];
+    auto fetch_e1 = [&](u32x4* e1, int s) {
+        const bool zin = s >= 0 && s < D;
+#pragma unroll
+        for (int k = 0; k < kGW; ++k) {
+            e1[k] = u32x4{0u, 0u, 0u, 0u};
+            if (zin && lo2 && ((gdom >> k) & 1u)) e1[k] = *(const u32x4*)(a.in.p + gco[k] + (s >> 1) * cplane);
+        }
+    };
+    // one plane of X -> tap partials, this wave's groups
+    auto partials = [&](const u32x4* e1, int s) {
+        const bool zin = s >= 0 && s < D;
+        int pe[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) pe[e] = ((s + tdz[e] - 1) & 3) * kRP + toff[e];
+#pragma unroll
+        for (int k = 0; k < kGW; ++k) {
+            if (wave + 4 * k >= kGroups) break;  // uniform per wavefront
+            u32x4 bw;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                bw[e] = (unsigned)lr[pe[2 * e] + gbase[k]] | ((unsigned)lr[pe[2 * e + 1] + gbase[k]] << 16);
+            const bf16x8 B = __builtin_bit_cast(bf16x8, bw);
+            const f32x4 c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl1, B, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            const f32x4 c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl2, B, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            // X = BF16(BF16(relu(lift)) + up(E1)) on the domain, 0 outside (zero padding)
+            u32x4 bx = {0u, 0u, 0u, 0u};
+            if (zin && ((gdom >> k) & 1u) && lo2) {
+                const float lv[8] = {c1[0], c1[1], c1[2], c1[3], c2[0], c2[1], c2[2], c2[3]};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const unsigned h = pack2bf(fmaxf(lv[2 * e] + bl[2 * e], 0.0f), fmaxf(lv[2 * e + 1] + bl[2 * e + 1], 0.0f));
+                    bx[e] = pack2bf(bf2f(h & 0xffffu) + bf2f(e1[k][e] & 0xffffu),
+                                    __uint_as_float(h & 0xffff0000u) + __uint_as_float(e1[k][e] & 0xffff0000u));
+                }
+            }
+            const bf16x8 X = __builtin_bit_cast(bf16x8, bx);
+            const f32x4 d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, X, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            const f32x4 d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, X, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            if (gq[k] < kQ) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    lp[(4 * j + r) * kQ + gq[k]] = d0[r];
+                    lp[(16 + 4 * j + r) * kQ + gq[k]] = d1[r];
+                }
+            }
+        }
+    };
+    // this thread's output voxel
+    const int ox = tid & 15, oy = tid >> 4;
+    const int x = x0 + ox, y = y0 + oy;
+    const bool mine = x < W && y < H;
+    const int q0 = oy * kLW + ox;
+    float am1 = 0.0f, a0 = 0.0f, ap1 = 0.0f;
+
+    // planes s = z0-1 .. z1 of X; the ring holds input planes s-1 .. s+1
+    fetch_plane(preA, z0 - 2);
+    store_plane(preA, z0 - 2);
+    fetch_plane(preA, z0 - 1);
+    store_plane(preA, z0 - 1);
+    fetch_plane(preA, z0);
+    fetch_plane(preB, z0 + 1);
+    fetch_e1(e1v, z0 - 1);
+    for (int s = z0 - 1; s <= z1; ++s) {
+        const bool even = ((s - z0 + 1) & 1) == 0;
+        if (even) {
+            store_plane(preA, s + 1);
+            if (s + 3 <= z1 + 1) fetch_plane(preA, s + 3);
+        } else {
+            store_plane(preB, s + 1);
+            if (s + 3 <= z1 + 1) fetch_plane(preB, s + 3);
+        }
+        __syncthreads();  // input plane s+1 staged; every thread is past the previous gather
+        partials(e1v, s);
+        if (s + 1 <= z1) fetch_e1(e1v, s + 1);
+        __syncthreads();  // partials of plane s complete
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const float* pq = lp + (dy * 3 + dx) * kQ + q0 + dy * kLW + dx;
+                s0 += pq[0];
+                s1 += pq[9 * kQ];
+                s2 += pq[18 * kQ];
+            }
+        am1 += s2;
+        a0 += s1;
+        ap1 += s0;
+        const int zo = s - 1;
+        if (zo >= z0 && zo < z1 && mine) {
+            const long long o = fidx(a.fout, b, x, y, zo);
+            const float v = am1 + bout;
+            if (a.fout.is64) ((double*)a.fout.p)[o] = (double)v;
+            else ((float*)a.fout.p)[o] = v;
+        }
+        am1 = a0;
+        a0 = ap1;
+        ap1 = 0.0f;
+    }
+}
+
+void launch_outf(const ConvArgs& a, hipStream_t s)
+{
+    const int zc = pick_zc(a.out.D, a.out.H, a.out.W, a.out.B);
+    hipLaunchKernelGGL(k_outf, conv_grid(a.out.D, a.out.H, a.out.W, a.out.B, zc), dim3(256), 0, s, a, zc);
 }
 
 // ------------------------------------------------------------ weight gradient
@@ -747,6 +952,8 @@ void launch_upadd_back(const ActRef& dx, const ActRef& dskip, int accumulate_ski
 // forward:        W'[tap][k][m] = W[tap][ci = k][co = m]
 // backward data:  W'[tap][k][m] = W[26 - tap][ci = m][co = k]   (flipped, transposed)
 // lift (mode 2, 1 input channel): one operand, k = 8*(lane/16) + e is the tap
+// tap partials (mode 3, 1 output channel): two operands o, row m = tap 16*o + m,
+// k = input channel (k >= 16 zero)
 __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int mode, uint16_t* __restrict__ wl)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -757,6 +964,17 @@ __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int mode,
         if (t >= 512) return;
         const int tap = 8 * j + e;
         if (tap < kTaps && m < cout) v = W[tap * 256 + m];
+    } else if (mode == 3) {
+        if (t >= 1024) return;
+        const int tap = 16 * o + m, k = 8 * j + e;
+        if (tap < kTaps && k < cin && cout > 0) v = W[tap * 256 + k * 16];
+    } else if (mode == 4) {
+        // lift as two operands o whose rows m < 8 are output channels
+        // 8*(m/4) + 4*o + m%4: lane (n, j < 2) of the results then holds
+        // channels 8j .. 8j+7 (o = 0: +0..3, o = 1: +4..7)
+        if (t >= 1024) return;
+        const int tap = 8 * j + e, co = 8 * (m >> 2) + 4 * o + (m & 3);
+        if (m < 8 && tap < kTaps && co < cout) v = W[tap * 256 + co];
     } else {
         if (t >= kPackedHalves) return;
         const int p = o / 3, dy = o % 3, c = pair_combo(p, j >> 1), k = 8 * (j & 1) + e;
@@ -896,7 +1114,8 @@ Net::~Net()
 {
     release(inf_);
     release(trn_);
-    for (void* p : {(void*)params_, (void*)grads_, (void*)adam_m_, (void*)adam_v_, (void*)wl_, (void*)wlt_, (void*)data_r_})
+    for (void* p : {(void*)params_, (void*)grads_, (void*)adam_m_, (void*)adam_v_, (void*)wl_, (void*)wlt_, (void*)wlo_,
+                    (void*)wlf_, (void*)data_r_})
         if (p) (void)hipFree(p);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -917,6 +1136,8 @@ int Net::init(unsigned long long seed)
     if (!rc) rc = hip(hipMalloc(&adam_v_, kParams * sizeof(float)), "hipMalloc adam");
     if (!rc) rc = hip(hipMalloc(&wl_, (size_t)kLayers * kPackedHalves * 2), "hipMalloc weights");
     if (!rc) rc = hip(hipMalloc(&wlt_, (size_t)kLayers * kPackedHalves * 2), "hipMalloc weights");
+    if (!rc) rc = hip(hipMalloc(&wlo_, 1024 * 2), "hipMalloc weights");
+    if (!rc) rc = hip(hipMalloc(&wlf_, 1024 * 2), "hipMalloc weights");
     if (rc) return rc;
     // He initialisation (std = sqrt(2 / fan_in)) of the ReLU layers, sqrt(1 / fan_in) for the output
     std::vector<float> h(kParams, 0.0f);
@@ -942,6 +1163,11 @@ void Net::pack_all()
         hipLaunchKernelGGL(k_pack, dim3(kPackedHalves / 256), dim3(256), 0, stream_, W, kNet[l].cin, kNet[l].cout, 1,
                            wlt_ + (size_t)l * kPackedHalves);
     }
+    // the output layer again as tap-partial operands and the lift as two
+    // row-permuted operands, for the fused inference kernel
+    hipLaunchKernelGGL(k_pack, dim3(4), dim3(256), 0, stream_, params_ + (size_t)(kLayers - 1) * kLayerParams,
+                       kNet[kLayers - 1].cin, kNet[kLayers - 1].cout, 3, wlo_);
+    hipLaunchKernelGGL(k_pack, dim3(4), dim3(256), 0, stream_, params_, kNet[0].cin, kNet[0].cout, 4, wlf_);
 }
 
 int Net::set_params(const float* host, long long n)
@@ -1026,6 +1252,7 @@ int Net::ensure(BufSet& bs, int B, const int n[3], bool train)
         }
         std::vector<std::pair<int, uint16_t*>> slots[3];  // (holder, buffer) per level
         for (int i = 0; i < kActs && !rc; ++i) {
+            if (i == 0) continue;  // the lift's output lives only in the fused output kernel (and its pooled copy)
             auto& pool = slots[kNet[i].level];
             int pick = -1;
             for (size_t k = 0; k < pool.size(); ++k)
@@ -1091,12 +1318,22 @@ ConvArgs Net::layer_args(const BufSet& bs, int i, const FieldRef& fin, const Fie
     a.bias = params_ + (size_t)i * kLayerParams + kTaps * 256;
     a.gate = gate;
     a.gate2 = gate2;
+    if (!bs.train && bs.B == 1 && i == kLayers - 1 && d.mode_in == IN_UPADD && d.skip_act == 0 &&
+        kNet[0].mode_in == IN_FIELD) {
+        // inference: the output layer recomputes the lift (act0 is never stored)
+        a.fused = 1;
+        a.fin = fin;
+        a.wl = wlo_;
+        a.wl2 = wlf_;  // layer 0's lift, row-permuted operands
+        a.bias2 = params_ + kTaps * 256;
+    }
     return a;
 }
 
 void Net::launch_layer(const ConvArgs& a, int i)
 {
     if (kNet[i].mode_in == IN_FIELD) launch_lift(a, stream_);
+    else if (a.fused) launch_outf(a, stream_);
     else launch_conv(a, stream_);
 }
 
@@ -1259,9 +1496,15 @@ int Net::time_layer(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec
     default: rd = v * act_b; break;
     }
     wr = (i == kLayers - 1) ? v * (out_prec == 64 ? 8.0 : 4.0) : v * act_b;
-    if (i == 0) wr += vox[1] * act_b;  // the pooled copy
+    if (i == 0) wr = vox[1] * act_b;  // inference: the pooled copy only
+    double fl = 2.0 * kTaps * d.cin * d.cout * v;
+    if (i == kLayers - 1) {
+        // fused: reads r and the coarse activation, recomputes the lift
+        rd = v * (in_prec == 64 ? 8.0 : 4.0) + vox[1] * act_b;
+        fl += 2.0 * kTaps * kNet[0].cin * kNet[0].cout * v;
+    }
     *bytes = rd + wr;
-    *flops = 2.0 * kTaps * d.cin * d.cout * v;
+    *flops = fl;
     return 0;
 }
 

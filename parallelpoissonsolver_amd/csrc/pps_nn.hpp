@@ -91,6 +91,9 @@ struct ConvArgs {
     const float* bias;
     const int* gate;
     const int* gate2;
+    int fused;               // output layer with the lift recomputed (inference)
+    const uint16_t* wl2;     // fused: lift operand
+    const float* bias2;      // fused: lift bias
 };
 
 struct BufSet {
@@ -172,6 +175,8 @@ private:
     std::string err_;
     float *params_ = nullptr, *grads_ = nullptr, *adam_m_ = nullptr, *adam_v_ = nullptr;
     uint16_t *wl_ = nullptr, *wlt_ = nullptr;  // packed forward / backward-data weights, kLayers each
+    uint16_t* wlo_ = nullptr;                  // output layer as tap-partial operands (fused inference)
+    uint16_t* wlf_ = nullptr;                  // lift as row-permuted operands (fused inference)
     BufSet inf_, trn_;
     float* data_r_ = nullptr;
     long long data_n_ = 0, data_pairs_ = 0, data_cap_ = 0;

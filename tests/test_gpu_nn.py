@@ -54,6 +54,33 @@ def test_forward_matches_oracle(net, shape):
         assert np.max(np.abs(a - acts[i])) <= 0.02 * np.max(np.abs(acts[i])) + 1e-6, i
 
 
+def test_inference_path_matches_training_forward(net):
+    """The solver's path (pooled-only lift, output layer with the lift fused in,
+    strided FP64 field, input scaled to unit RMS) against the dense batch
+    forward of the unfused kernels on the same scaled input."""
+    print("[nn] fused inference vs dense forward 24^3", flush=True)
+    net.set_params(_params(31))
+    n = 24
+    C = L.Context((n + 2,) * 3, (1.0 / (n + 1),) * 3)
+    net.attach(C)
+    rng = np.random.default_rng(32)
+    r = rng.normal(size=(n, n, n)) * 3.7
+    f = np.zeros(C.shape)
+    f[2:-2, 2:-2, 2:-2] = r
+    C.upload(L.FIELD_T0, f)
+    C.fill(L.FIELD_T1, 0.0)
+    C.nn_apply(L.FIELD_T1, L.FIELD_T0)
+    out = C.download(L.FIELD_T1)
+    yf = out[2:-2, 2:-2, 2:-2]
+    inside = np.zeros(C.shape, bool)
+    inside[2:-2, 2:-2, 2:-2] = True
+    assert np.all(out[~inside] == 0.0)
+    rms = np.sqrt(np.mean(r * r))
+    yd = net.forward((r / rms)[None].astype(np.float32))[0]
+    assert np.linalg.norm(yf - yd) / np.linalg.norm(yd) < 2e-3
+    C.close()
+
+
 def test_loss_gradient_matches_oracle(net):
     print("[nn] loss/grad 2x16^3", flush=True)
     p = _params(21)
