@@ -125,9 +125,10 @@ def krylov_kernels(C, reps):
 
 
 def nn_layers(C, reps):
+    """Per-layer timing on FP32 fields, as the headline (mixed-precision) solve runs the network."""
     out = {}
     for i, name in enumerate(LAYER_NAMES):
-        ms, nbytes, flops = C.nn_time_layer(i, reps)
+        ms, nbytes, flops = C.nn_time_layer(i, reps, prec=32)
         out[name] = {"ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9,
                      "tflops": flops / (ms * 1e-3) / 1e12}
         log(f"layer {name}: {ms:.4f} ms/launch, {out[name]['gbs']:.0f} GB/s, {out[name]['tflops']:.1f} TFLOP/s "

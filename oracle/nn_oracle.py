@@ -138,7 +138,7 @@ def up_back(dx, cdims):
     return xp.reshape(B, Dc, 2, Hc, 2, Wc, 2, Cc).sum(axis=(2, 4, 6))
 
 
-def _layer_input(l, r, acts, rounding):
+def _layer_input(l, r, acts, rounding, exact_skip=False):
     cin, cout, lev, mode, ia, sk, relu = NET[l]
     dt = np.float32 if rounding else np.float64
     if mode == "field":
@@ -150,7 +150,7 @@ def _layer_input(l, r, acts, rounding):
     if mode == "pool":
         return _rnd(pool(acts[ia].astype(dt)), rounding)
     skip = acts[sk]
-    return _rnd(up(acts[ia], skip.shape[1:4]).astype(dt) + skip, rounding)
+    return _rnd(up(acts[ia], skip.shape[1:4]).astype(dt) + skip, rounding and not exact_skip)
 
 
 def _weights(params, rounding):
@@ -167,15 +167,19 @@ def _weights(params, rounding):
     return out
 
 
-def forward(params, r, rounding=True):
-    """Network output y [B][n2][n1][n0] (float32/64) and activations act0..act12."""
+def forward(params, r, rounding=True, exact_out_skip=False):
+    """Network output y [B][n2][n1][n0] (float32/64) and activations act0..act12.
+
+    exact_out_skip: the output layer's input up(act12) + act0 stays unrounded
+    (the fused inference kernel adds the two inside its MFMA, libpps
+    pps_nn.hip k_outf); every other rounding as with rounding=True."""
     r = np.asarray(r)
     dt = np.float32 if rounding else np.float64
     Ws = _weights(params, rounding)
     acts = []
     y = None
     for l in range(N_LAYERS):
-        x = _layer_input(l, r, acts, rounding).astype(dt)
+        x = _layer_input(l, r, acts, rounding, exact_out_skip and l == N_LAYERS - 1).astype(dt)
         W, b = Ws[l]
         z = conv(x, W.astype(dt), np.float64) + b.astype(np.float64)
         if l == N_LAYERS - 1:
@@ -183,6 +187,16 @@ def forward(params, r, rounding=True):
         else:
             acts.append(_rnd(np.maximum(z, 0.0).astype(dt), rounding))
     return y, acts
+
+
+def output_layer(params, act0, act12, exact_skip=False):
+    """The output convolution alone, from given level-0 and level-1 activations
+    (float32, as stored): conv(up(act12) + act0) + bias, channel 0.  With
+    exact_skip the sum is not rounded to BF16 (the fused inference kernel)."""
+    W, b = _weights(params, True)[N_LAYERS - 1]
+    x = up(np.asarray(act12, np.float32), act0.shape[1:4]).astype(np.float32) + np.asarray(act0, np.float32)
+    x = x if exact_skip else bf16(x)
+    return (conv(x, W.astype(np.float32), np.float64) + b.astype(np.float64))[..., 0]
 
 
 def k_apply(y):

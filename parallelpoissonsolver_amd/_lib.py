@@ -137,7 +137,7 @@ SIGNATURES = {
     "pps_nn_dataset": ([_VP, ctypes.c_longlong, _FP, _DP], ctypes.c_int),
     "pps_nn_train": ([_VP, ctypes.POINTER(CNNTrainCfg), ctypes.POINTER(CNNTrainStats)], ctypes.c_int),
     "pps_nn_apply_field": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
-    "pps_nn_time_layer": ([_VP, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP], ctypes.c_int),
+    "pps_nn_time_layer": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP], ctypes.c_int),
     "pps_nn_attach": ([_VP, _VP], ctypes.c_int),
 }
 
@@ -290,10 +290,12 @@ class Context:
         """out = M(inp) with the attached network (FP64 fields)."""
         self._chk(lib().pps_nn_apply_field(self.h, out, inp))
 
-    def nn_time_layer(self, layer, reps=10):
-        """(ms per launch, algorithmic bytes, flops) of one layer of the attached network."""
+    def nn_time_layer(self, layer, reps=10, prec=32):
+        """(ms per launch, algorithmic bytes, flops) of one layer of the attached network,
+        on FP32 (the mixed-precision solve's) or FP64 fields."""
         ms, nb, fl = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        self._chk(lib().pps_nn_time_layer(self.h, layer, reps, ctypes.byref(ms), ctypes.byref(nb), ctypes.byref(fl)))
+        self._chk(lib().pps_nn_time_layer(self.h, layer, prec, reps, ctypes.byref(ms), ctypes.byref(nb),
+                                          ctypes.byref(fl)))
         return ms.value, nb.value, fl.value
 
     def set_tiling(self, kz_stencil=0, kz_vector=0):

@@ -698,6 +698,14 @@ static Scal* caller_of(const int* gate)
     return reinterpret_cast<Scal*>(reinterpret_cast<char*>(const_cast<int*>(gate)) - offsetof(Scal, done));
 }
 
+// FP64 -> FP32 copy of a whole field allocation (FP32 and FP64 fields share one
+// element layout)
+__global__ void k_narrow(const double* __restrict__ a, float* __restrict__ b, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
 struct NNStage : Stage {
     using Stage::Stage;
     int apply(Field out, Field in, const int* gate) override
@@ -1795,26 +1803,38 @@ int pps_nn_apply_field(void* ctx, int out, int in)
     return C->hip(hipStreamSynchronize(C->stream), "sync");
 }
 
-int pps_nn_time_layer(void* ctx, int layer, int reps, double* ms, double* bytes, double* flops)
+int pps_nn_time_layer(void* ctx, int layer, int prec, int reps, double* ms, double* bytes, double* flops)
 {
     Ctx* C = as_ctx(ctx);
     if (!C || !ms || !bytes || !flops) return PPS_ERR_ARG;
+    if (prec != 32 && prec != 64) return C->fail(PPS_ERR_ARG, "pps_nn_time_layer: precision must be 32 or 64");
     // an apply on T0 <- B sets up the activations the layer reads
     int rc = pps_nn_apply_field(ctx, PPS_FIELD_T0, PPS_FIELD_B);
     if (rc) return rc;
     Field fo, fi;
     if ((rc = field_of(*C, PPS_FIELD_T0, fo)) || (rc = field_of(*C, PPS_FIELD_B, fi))) return rc;
+    if (prec == 32) {
+        // FP32 copies of B and an FP32 output, as the mixed-precision inner solve hands them over
+        Field fi32, fo32;
+        if ((rc = C->get_work("nntime.in", 32, fi32)) || (rc = C->get_work("nntime.out", 32, fo32))) return rc;
+        const long long n = C->L64.total;
+        hipLaunchKernelGGL(k_narrow, dim3((unsigned)std::min<long long>((n + 255) / 256, 65536)), dim3(256), 0,
+                           C->stream, fi.d(), fi32.f(), n);
+        fi = fi32;
+        fo = fo32;
+    }
     Scal* S = C->scal(C->nscal - 1);
+    const Layout& L = C->lay(prec);
     NNFieldGeom g;
-    g.off = C->L64.at(C->solver.lo[0], C->solver.lo[1], C->solver.lo[2]);
-    g.sy = C->L64.px;
-    g.sz = C->L64.plane;
+    g.off = L.at(C->solver.lo[0], C->solver.lo[1], C->solver.lo[2]);
+    g.sy = L.px;
+    g.sz = L.plane;
     g.n0 = C->solver.hi[0] - C->solver.lo[0];
     g.n1 = C->solver.hi[1] - C->solver.lo[1];
     g.n2 = C->solver.hi[2] - C->solver.lo[2];
     auto* net = dynamic_cast<nn::Net*>(C->nn.get());
     if (!net) return C->fail(PPS_ERR_STATE, "no network attached");
-    if (net->time_layer(C->stream, g, fo.p, 64, fi.p, 64, &S->v[S_RR], layer, reps, ms, bytes, flops))
+    if (net->time_layer(C->stream, g, fo.p, prec, fi.p, prec, &S->v[S_RR], layer, reps, ms, bytes, flops))
         return C->fail(PPS_ERR_ARG, "%s", net->error());
     return PPS_OK;
 }

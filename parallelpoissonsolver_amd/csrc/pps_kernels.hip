@@ -31,6 +31,8 @@
 // the y-halo rows a tile reads from its neighbour are then L2 hits.
 #include "pps_kernels.hpp"
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -175,6 +177,25 @@ __device__ __forceinline__ void finish_iteration(Scal* S, double err, bool histo
 
 // --------------------------------------------------------- kernel skeletons
 
+// Stencil ops may carry a side value from the staging read of a point to its
+// output (Op::Side, returned through in(c, side)): the thread that computes a
+// point also staged it, so the value need not be read from HBM again.
+struct NoSide {};
+template <class Op, class = void> struct SideOf { using type = NoSide; };
+template <class Op> struct SideOf<Op, std::void_t<typename Op::Side>> { using type = typename Op::Side; };
+template <class Op, typename S>
+__device__ __forceinline__ auto stage_in(const Op& op, long long c, S& side)
+{
+    if constexpr (std::is_same_v<S, NoSide>) return op.in(c);
+    else return op.in(c, side);
+}
+template <class Op, typename T, typename S>
+__device__ __forceinline__ void stage_out(const Op& op, long long c, T v, T Av, const S& side, double* acc)
+{
+    if constexpr (std::is_same_v<S, NoSide>) op.out(c, v, Av, acc);
+    else op.out(c, v, Av, side, acc);
+}
+
 template <typename T, int NQ, class Op>
 __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
 {
@@ -218,9 +239,11 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
     const long long hbase = (long long)g.ox + hi + (long long)g.px * hj;
     const T cx = (T)g.cx, cy = (T)g.cy, cz = (T)g.cz;
 
+    using Side = typename SideOf<Op>::type;
     bool act[kRY], inb[kRY];
     long long base[kRY];
     T vm[kRY], vc[kRY];
+    Side sc[kRY], sp[kRY];
 #pragma unroll
     for (int q = 0; q < kRY; ++q) {
         const int j = jt + ly + q * kTY;
@@ -228,7 +251,7 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
         inb[q] = (i <= g.hi0) && (j <= g.hi1);
         base[q] = (long long)g.ox + i + (long long)g.px * j;
         vm[q] = inb[q] ? op.in(base[q] + g.plane * (k0 - 1)) : T(0);
-        vc[q] = inb[q] ? op.in(base[q] + g.plane * k0) : T(0);
+        vc[q] = inb[q] ? stage_in(op, base[q] + g.plane * k0, sc[q]) : T(0);
     }
     double acc[NQ > 0 ? NQ : 1];
 #pragma unroll
@@ -240,7 +263,7 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
         // loads for plane k+1 first: they complete while plane k is computed
         T vp[kRY];
 #pragma unroll
-        for (int q = 0; q < kRY; ++q) vp[q] = inb[q] ? op.in(base[q] + g.plane * (k + 1)) : T(0);
+        for (int q = 0; q < kRY; ++q) vp[q] = inb[q] ? stage_in(op, base[q] + g.plane * (k + 1), sp[q]) : T(0);
         const T hnext = (hin && k + 1 < k1) ? op.in(hbase + g.plane * (k + 1)) : T(0);
 #pragma unroll
         for (int q = 0; q < kRY; ++q) lds[buf][ly + q * kTY + 1][lx + 1] = vc[q];
@@ -254,10 +277,11 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
                 const T Av = cx * ((lds[buf][r][lx] - v) + (lds[buf][r][lx + 2] - v)) +
                              cy * ((lds[buf][r - 1][lx + 1] - v) + (lds[buf][r + 1][lx + 1] - v)) +
                              cz * ((vm[q] - v) + (vp[q] - v));
-                op.out(base[q] + g.plane * k, v, Av, acc);
+                stage_out(op, base[q] + g.plane * k, v, Av, sc[q], acc);
             }
             vm[q] = vc[q];
             vc[q] = vp[q];
+            sc[q] = sp[q];
         }
         hcur = hnext;
         buf ^= 1;
@@ -727,12 +751,18 @@ struct OpCgA : Gate {
         beta = (T)S->v[S_BETA];
         alpha_prev = (T)S->v[S_ALPHA];
     }
+    struct Side { T pold; };  // p_old at the point, for the folded x update
     __device__ T in(long long c) const { return a * src[c] + beta * pin[c]; }
-    __device__ void out(long long c, T vc, T Av, double* acc) const
+    __device__ T in(long long c, Side& s) const
+    {
+        s.pold = pin[c];
+        return a * src[c] + beta * s.pold;
+    }
+    __device__ void out(long long c, T vc, T Av, const Side& s, double* acc) const
     {
         put(pout + c, vc, nt);
         put(Ap + c, Av, nt);
-        put(x + c, x[c] + alpha_prev * pin[c], nt);
+        put(x + c, x[c] + alpha_prev * s.pold, nt);
         acc[0] += (double)vc * (double)Av;
     }
     __device__ void finish(const double* t)

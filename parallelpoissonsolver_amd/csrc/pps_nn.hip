@@ -344,21 +344,35 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
         f32x4 acc[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // B operands in (pair, input row) order, read kPf steps ahead of their MFMAs
+        int pbase[5];
 #pragma unroll
         for (int p = 0; p < 5; ++p) {
             // member geometry (the empty member of pair 4 reuses member 0's, its weights are zero)
             const int c = pair_combo(p, m) < 9 ? pair_combo(p, m) : pair_combo(p, 0);
             const int dz = c / 3, dx = c % 3;
-            const u32x4* pl = lds + ((z + dz - 1) & 3) * kPlane + (j & 1) * kLW + n + dx + wave * 4 * 2 * kLW;
+            pbase[p] = ((z + dz - 1) & 3) * kPlane + (j & 1) * kLW + n + dx + wave * 4 * 2 * kLW;
+        }
+        // two steps ahead where the registers allow (the pooled and up+skip
+        // input stages hold more raw staging data; with the one-channel field
+        // output they are left to the compiler's schedule)
+        constexpr bool kPipe = !((MI == IN_POOL || MI == IN_UPADD) && MO == OUT_FIELD);
+        constexpr int kSteps = 30, kPf = !kPipe ? 0 : (MI == IN_POOL || MI == IN_UPADD) ? 1 : 2;
+        u32x4 bq[kPf + 1];
 #pragma unroll
-            for (int t = 0; t < 6; ++t) {
-                const bf16x8 B = __builtin_bit_cast(bf16x8, pl[t * 2 * kLW]);
+        for (int q = 0; q < kPf; ++q) bq[q] = lds[pbase[q / 6] + (q % 6) * 2 * kLW];
 #pragma unroll
-                for (int dy = 0; dy < 3; ++dy) {
-                    const int i = t - dy;
-                    if (i >= 0 && i < 4) acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[3 * p + dy], B, acc[i], 0, 0, 0);
-                }
+        for (int q = 0; q < kSteps; ++q) {
+            if (q + kPf < kSteps)
+                bq[(q + kPf) % (kPf + 1)] = lds[pbase[(q + kPf) / 6] + ((q + kPf) % 6) * 2 * kLW];
+            const bf16x8 B = __builtin_bit_cast(bf16x8, bq[q % (kPf + 1)]);
+            const int p = q / 6, t = q % 6;
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                const int i = t - dy;
+                if (i >= 0 && i < 4) acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wr[3 * p + dy], B, acc[i], 0, 0, 0);
             }
+            if constexpr (kPipe) __builtin_amdgcn_sched_barrier(0);  // keep the reads kPf steps ahead
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -368,16 +382,33 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
     }
 }
 
+// packed BF16 helpers of the lift kernels
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+// relu of two packed BF16 values: as int16 a BF16 with the sign bit set is negative
+__device__ __forceinline__ unsigned relu_bf2(unsigned h)
+{
+    return __builtin_bit_cast(unsigned, __builtin_elementwise_max(__builtin_bit_cast(s16x2, h), s16x2{0, 0}));
+}
+__device__ __forceinline__ int mod3(int v) { return ((v % 3) + 3) % 3; }
+
 // Lift (1 -> 16 channels): K = the 27 taps of one channel (+5 zero taps), one
 // MFMA per 16 voxels; lane (voxel n, block j) gathers taps 8j .. 8j+7 from a
-// BF16 staging ring.  Input stage as IN_FIELD (scaled to unit RMS).
-__global__ __launch_bounds__(256) void k_lift(ConvArgs a, int zc)
+// BF16 staging ring and receives channels 4j .. 4j+3.  The accumulator starts
+// at the bias, or at -1e30 outside the domain, so relu leaves zeros there.
+// Input plane p is staged in ring slot p mod 4, and again in slot p mod 4 + 4
+// when that is below 6: planes z-1, z, z+1 then sit in three consecutive
+// slots (and plane z+2, which a wavefront ahead of the others may already be
+// writing before the next barrier, in none of them).  Input stage as IN_FIELD
+// (scaled to unit RMS).
+__global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
     constexpr int kP = kLH * kLW;
-    __shared__ uint16_t lds[4 * kP];
+    __shared__ uint16_t lds[6 * kP];
     const int D = a.out.D, H = a.out.H, W = a.out.W;
     const int nzc = (D + zc - 1) / zc;
+    auto mod4 = [](int v) { return v & 3; };
     const int b = blockIdx.z / nzc;
     const int z0 = (blockIdx.z - b * nzc) * zc;
     const int z1 = min(z0 + zc, D);
@@ -386,37 +417,55 @@ __global__ __launch_bounds__(256) void k_lift(ConvArgs a, int zc)
     const int j = lane >> 4, n = lane & 15;
     const float fscale = field_scale(a.fin);
     const bf16x8 w = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[lane]);
-    float bias[4];
+    const int x = x0 + n;
+    f32x4 cinit[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) bias[r] = a.bias ? a.bias[4 * j + r] : 0.0f;
-    // taps 8j+e of this lane: plane offset dz and in-plane offset (tap 27.. reuse tap 26; zero weights)
-    int tdz[8], toff[8];
+    for (int i = 0; i < 4; ++i) {
+        const int y = y0 + wave * 4 + i;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cinit[i][r] = (x < W && y < H) ? (a.bias ? a.bias[4 * j + r] : 0.0f) : -1e30f;
+    }
+    // taps 8j+e of this lane: ring offset relative to the slot of plane z-1 (taps
+    // 27.. reuse tap 26; zero weights)
+    int toff[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const int tap = min(8 * j + e, 26);
-        tdz[e] = tap / 9;
-        toff[e] = ((tap / 3) % 3) * kLW + tap % 3;
+        toff[e] = (tap / 9) * kP + ((tap / 3) % 3 + wave * 4) * kLW + tap % 3 + n;
     }
     constexpr int kPer = (kP + 255) / 256;
-    int sxx[kPer], syy[kPer], sl[kPer];
+    long long soff[kPer];  // in-plane offset of the staged voxel in the field; -1: outside
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const int t = tid + 256 * k;
-        sl[k] = t < kP ? t : -1;
-        syy[k] = y0 - 1 + t / kLW;
-        sxx[k] = x0 - 1 + t % kLW;
+        const int yy = y0 - 1 + t / kLW, xx = x0 - 1 + t % kLW;
+        soff[k] = (t < kP && xx >= 0 && yy >= 0 && xx < W && yy < H) ? xx * a.fin.sx + yy * a.fin.sy + b * a.fin.sb : -1;
     }
-    Raw<IN_FIELD> preA[kPer], preB[kPer];
-    auto fetch_plane = [&](Raw<IN_FIELD>* pre, int zi) {
+    const bool fin64 = a.fin.is64;
+    double preA[kPer], preB[kPer];
+    auto fetch_plane = [&](double* pre, int zi) {
+        const bool zin = zi >= 0 && zi < D;
+        const long long zo = a.fin.off + zi * a.fin.sz;
 #pragma unroll
-        for (int k = 0; k < kPer; ++k)
-            if (sl[k] >= 0) pre[k] = fetch<IN_FIELD>(a.in, a.in2, a.fin, fscale, D, H, W, b, sxx[k], syy[k], zi, 0);
+        for (int k = 0; k < kPer; ++k) {
+            pre[k] = 0.0;
+            if (zin && soff[k] >= 0)
+                pre[k] = fin64 ? ((const double*)a.fin.p)[zo + soff[k]] : (double)((const float*)a.fin.p)[zo + soff[k]];
+        }
     };
-    auto store_plane = [&](const Raw<IN_FIELD>* pre, int zi) {
-        uint16_t* dst = lds + (zi & 3) * kP;
+    auto store_plane = [&](const double* pre, int zi) {
+        uint16_t* dst = lds + mod4(zi) * kP;
+        const bool twice = mod4(zi) < 2;  // uniform
 #pragma unroll
-        for (int k = 0; k < kPer; ++k)
-            if (sl[k] >= 0) dst[sl[k]] = (uint16_t)finish<IN_FIELD>(pre[k], a.fin, fscale)[0];
+        for (int k = 0; k < kPer; ++k) {
+            const int t = tid + 256 * k;
+            if (t < kP) {
+                const float v = fin64 ? (float)(pre[k] * (double)fscale) : (float)pre[k] * fscale;
+                const uint16_t h = (uint16_t)f2bf(v);
+                dst[t] = h;
+                if (twice) dst[t + 4 * kP] = h;
+            }
+        }
     };
     fetch_plane(preA, z0 - 1);
     store_plane(preA, z0 - 1);
@@ -424,10 +473,10 @@ __global__ __launch_bounds__(256) void k_lift(ConvArgs a, int zc)
     store_plane(preA, z0);
     fetch_plane(preA, z0 + 1);
     if (z0 + 2 <= z1) fetch_plane(preB, z0 + 2);
-    const int x = x0 + n;
     // pooled copy (2x2x2 average of the stored BF16 outputs) for the next level:
-    // x pairs are neighbouring lanes, y pairs neighbouring rows of this wave,
-    // z pairs consecutive planes (chunks start on even planes)
+    // y pairs are neighbouring rows of this wave and z pairs consecutive planes
+    // (chunks start on even planes), summed per lane; x pairs are neighbouring
+    // lanes, added when the pooled voxel is written
     float ps[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
     for (int z = z0; z < z1; ++z) {
         // two planes in flight, as in k_conv
@@ -439,44 +488,43 @@ __global__ __launch_bounds__(256) void k_lift(ConvArgs a, int zc)
             if (z + 3 <= z1) fetch_plane(preB, z + 3);
         }
         __syncthreads();
-        // LDS offsets of this lane's 8 taps for the plane (integer offsets keep the
-        // reads on the LDS path)
         int pe[8];
+        const int sbase = mod4(z - 1) * kP;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) pe[e] = ((z + tdz[e] - 1) & 3) * kP + toff[e] + n + wave * 4 * kLW;
+        for (int e = 0; e < 8; ++e) pe[e] = sbase + toff[e];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            u32x4 bw;
+            u16x2 bw[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e)
-                bw[e] = (unsigned)lds[pe[2 * e] + i * kLW] | ((unsigned)lds[pe[2 * e + 1] + i * kLW] << 16);
-            const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, __builtin_bit_cast(bf16x8, bw),
-                                                                       f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-            const int y = y0 + wave * 4 + i;
-            const bool in = x < W && y < H;
+            for (int e = 0; e < 4; ++e) {
+                bw[e].x = lds[pe[2 * e] + i * kLW];
+                bw[e].y = lds[pe[2 * e + 1] + i * kLW];
+            }
+            const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, __builtin_bit_cast(bf16x8, bw), cinit[i], 0, 0, 0);
             u32x2 o;
-            o[0] = pack2bf(fmaxf(acc[0] + bias[0], 0.0f), fmaxf(acc[1] + bias[1], 0.0f));
-            o[1] = pack2bf(fmaxf(acc[2] + bias[2], 0.0f), fmaxf(acc[3] + bias[3], 0.0f));
-            if (in && a.out.p)  // inference keeps only the pooled copy
+            o[0] = relu_bf2(pack2bf(acc[0], acc[1]));
+            o[1] = relu_bf2(pack2bf(acc[2], acc[3]));
+            const int y = y0 + wave * 4 + i;
+            if (a.out.p && x < W && y < H)  // inference keeps only the pooled copy
                 *(u32x2*)(a.out.p + vox(a.out, b, x, y, z) + 4 * j) = o;
             if (a.pout.p) {
-                float v[4] = {bf2f(o[0] & 0xffffu), __uint_as_float(o[0] & 0xffff0000u), bf2f(o[1] & 0xffffu),
-                              __uint_as_float(o[1] & 0xffff0000u)};
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    if (!in) v[r] = 0.0f;
-                    ps[i >> 1][r] += v[r] + xor1(v[r]);
-                }
+                ps[i >> 1][0] += bf2f(o[0] & 0xffffu);
+                ps[i >> 1][1] += __uint_as_float(o[0] & 0xffff0000u);
+                ps[i >> 1][2] += bf2f(o[1] & 0xffffu);
+                ps[i >> 1][3] += __uint_as_float(o[1] & 0xffff0000u);
             }
         }
         if (a.pout.p && ((z & 1) || z == D - 1)) {
 #pragma unroll
             for (int ip = 0; ip < 2; ++ip) {
+                float v[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) v[r] = 0.125f * (ps[ip][r] + xor1(ps[ip][r]));
                 const int yf = y0 + wave * 4 + 2 * ip;
                 if (!(n & 1) && x < W && yf < H) {
                     u32x2 o;
-                    o[0] = pack2bf(0.125f * ps[ip][0], 0.125f * ps[ip][1]);
-                    o[1] = pack2bf(0.125f * ps[ip][2], 0.125f * ps[ip][3]);
+                    o[0] = pack2bf(v[0], v[1]);
+                    o[1] = pack2bf(v[2], v[3]);
                     *(u32x2*)(a.pout.p + vox(a.pout, b, x >> 1, yf >> 1, z >> 1) + 4 * j) = o;
                 }
 #pragma unroll
@@ -531,145 +579,163 @@ void launch_lift(const ConvArgs& a, hipStream_t s)
 
 // Inference form of the output layer with the lift fused in:
 //   z = out(up(E1) + relu(lift(r)))
-// so the 16-channel level-0 activation never goes to HBM.  Per plane s of the
-// 18 x 18 region (output tile + halo), in groups of 16 voxels:
-//   1. lift: two MFMAs (K = 27 taps of the BF16 input ring) whose weight rows
-//      are permuted so that lane (n, j < 2) receives channels 8j .. 8j+7 of
-//      voxel n -- the B layout of the next step, no lane exchange needed;
-//      relu, BF16, + up(E1) (BF16 sum, as the IN_UPADD input stage): X;
-//   2. tap partials P[tap][v] = sum_ci W[tap][ci] X[v][ci]: two MFMAs (rows =
-//      taps 0-15, 16-31; k = ci padded to 32), stored tap-major in LDS;
-// then every thread adds the 27 shifted partials of its voxel into three
-// running output planes; plane s completes output plane s-1.  Values equal the
-// unfused path's up to FP32 summation order (same BF16 roundings).
-__global__ __launch_bounds__(256, 3) void k_outf(ConvArgs a, int zc)
+// so the 16-channel level-0 activation never goes to HBM.  A block owns a
+// 14 x 18 output tile; its X region (tile + 1-voxel halo) is 16 x 20 voxels,
+// one 16-voxel x-row per MFMA group, 5 rows per wavefront.  Per plane s:
+//   1. lift: one MFMA (K = 27 taps of the BF16 input ring, accumulator started
+//      at the bias, or at -1e30 outside the domain so that relu gives the zero
+//      padding); lane (voxel n, block j) then holds channels 4j .. 4j+3;
+//      relu + BF16 in packed 16-bit form;
+//   2. tap partials P[v][tap] = sum_ci W[tap][ci] (a[v][ci] + e[v][ci]), with
+//      a = the BF16 lift output and e = up(E1): the A operand of lane (n, j)
+//      is (a, e) of channels 4j .. 4j+3 and the weight operand repeats
+//      W[tap][4j .. 4j+3] for both halves, so the skip sum happens inside the
+//      MFMA; two MFMAs (taps 0-15, 16-26), results stored tap-major in LDS
+//      (16-byte writes of 4 consecutive voxels);
+// then every thread adds the 27 shifted partials of its output voxel into
+// three running output planes; plane s completes output plane s-1.
+// The unfused path rounds a + e to BF16 before its MFMA; here the sum is exact,
+// so outputs differ from it by that rounding (relative ~1e-4) and FP32 order.
+constexpr int kFW = 14, kFH = 18;                 // output tile
+constexpr int kXW = kFW + 2, kXH = kFH + 2;       // X region: 16 x 20
+constexpr int kFRW = kXW + 2, kFRH = kXH + 2;     // input ring plane: 18 x 22
+constexpr int kFRP = kFRW * kFRH;
+constexpr int kXQ = kXW * kXH;
+constexpr int kXQp = kXQ + 4;                     // P row stride: 4 mod 64 banks
+constexpr int kFG = kXH / 4;                      // x-rows (groups) per wavefront
+static_assert(kXW == 16 && kXH % 4 == 0, "one 16-voxel x-row per group, equal rows per wavefront");
+
+__global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
-    constexpr int kRW = kTW + 4, kRH = kTHt + 4;  // input ring plane: tile + 2-voxel halo
-    constexpr int kRP = kRW * kRH;
-    constexpr int kQ = kLH * kLW;                 // X region: tile + 1-voxel halo
-    constexpr int kGroups = (kQ + 15) / 16;
-    constexpr int kGW = (kGroups + 3) / 4;        // groups per wavefront
-    __shared__ uint16_t lr[4 * kRP];
-    __shared__ float lp[32 * kQ];                 // P[tap][voxel]
+    __shared__ uint16_t lr[6 * kFRP];  // input planes, each in two slots
+    __shared__ __align__(16) float lp[kTaps * kXQp];  // P[tap][voxel of the X region]
     const int D = a.out.D, H = a.out.H, W = a.out.W;
-    const int nzc = (D + zc - 1) / zc;
-    const int b = blockIdx.z / nzc;
-    const int z0 = (blockIdx.z - b * nzc) * zc;
+    const int z0 = blockIdx.z * zc;     // one sample
     const int z1 = min(z0 + zc, D);
-    const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTHt;
+    const int x0 = blockIdx.x * kFW, y0 = blockIdx.y * kFH;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane >> 4, n = lane & 15;
-    const bool lo2 = j < 2;  // lanes holding channels 8j .. 8j+7
     const float fscale = field_scale(a.fin);
-    const bf16x8 wl1 = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl2)[lane]);
-    const bf16x8 wl2 = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl2)[64 + lane]);
-    const bf16x8 w0 = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[lane]);
-    const bf16x8 w1 = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[64 + lane]);
-    float bl[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) bl[e] = (lo2 && a.bias2) ? a.bias2[8 * j + e] : 0.0f;
+    const bf16x8 wlift = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl2)[lane]);
+    const bf16x8 wp0 = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[lane]);
+    const bf16x8 wp1 = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[64 + lane]);
     const float bout = a.bias ? a.bias[0] : 0.0f;
-    // lift taps 8j'+e of this lane (B operand k-block j' = j): plane offset and in-plane offset
-    int tdz[8], toff[8];
+    // X voxel of lane n: column gx; rows y0 - 1 + wave + 4k (k < kFG)
+    const int gx = x0 - 1 + n;
+    const bool xin = gx >= 0 && gx < W;
+    unsigned yin = 0u;  // rows of this wavefront inside the domain
+#pragma unroll
+    for (int k = 0; k < kFG; ++k) {
+        const int gy = y0 - 1 + wave + 4 * k;
+        if (gy >= 0 && gy < H) yin |= 1u << k;
+    }
+    yin = __builtin_amdgcn_readfirstlane(yin);
+    // lift accumulators start at the bias, or at -1e30 outside the domain
+    // (relu then gives the zero padding)
+    f32x4 cinit[kFG];
+#pragma unroll
+    for (int k = 0; k < kFG; ++k)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            cinit[k][r] = (xin && ((yin >> k) & 1u)) ? (a.bias2 ? a.bias2[4 * j + r] : 0.0f) : -1e30f;
+    // coarse activation E1: element offset of this lane's 4 channels in row k = 0
+    const int cW = a.in.W, cH = a.in.H;
+    const int cplane = cH * cW * kC;
+    const int cy0 = (y0 - 1 + wave) >> 1;  // coarse row of k = 0 (rows k step by 2)
+    const int ebase = (cy0 * cW + (gx >> 1)) * kC + 4 * j;
+    // lift taps 8j + e of this lane: ring offset for X voxel (row wave, column n)
+    // relative to the slot of plane s-1
+    int toff[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const int tap = min(8 * j + e, 26);
-        tdz[e] = tap / 9;
-        toff[e] = ((tap / 3) % 3) * kRW + tap % 3;
+        toff[e] = (tap / 9) * kFRP + ((tap / 3) % 3 + wave) * kFRW + tap % 3 + n;
     }
-    // per group of this wave: ring offset of the voxel, domain test, coarse offset
-    // (inference runs one sample: the coarse level's element offsets fit in 32 bits)
-    int gbase[kGW], gq[kGW], gco[kGW];
-    unsigned gdom = 0u;
-    const int cplane = a.in.H * a.in.W * kC;  // coarse plane stride
-#pragma unroll
-    for (int k = 0; k < kGW; ++k) {
-        const int g = wave + 4 * k;
-        const int q = 16 * g + n;
-        const int qq = q < kQ ? q : kQ - 1;
-        const int yy = qq / kLW, xx = qq - yy * kLW;
-        const int gx = x0 - 1 + xx, gy = y0 - 1 + yy;
-        gq[k] = q;
-        gbase[k] = yy * kRW + xx;
-        if (g < kGroups && q < kQ && gx >= 0 && gy >= 0 && gx < W && gy < H) gdom |= 1u << k;
-        gco[k] = ((gy >> 1) * a.in.W + (gx >> 1)) * kC + 8 * j;
-    }
-    constexpr int kPer = (kRP + 255) / 256;
-    int sxx[kPer], syy[kPer], sl[kPer];
+    // staging of the input ring: region (x0 - 2 .., y0 - 2 ..), 18 x 22.  Input
+    // plane p goes to ring slots p mod 3 and p mod 3 + 3, so that planes s-1,
+    // s, s+1 always sit in three consecutive slots.
+    constexpr int kPer = (kFRP + 255) / 256;
+    int soff[kPer];  // in-plane offset of the staged voxel in the field; -1: outside
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const int t = tid + 256 * k;
-        sl[k] = t < kRP ? t : -1;
-        syy[k] = y0 - 2 + t / kRW;
-        sxx[k] = x0 - 2 + t % kRW;
+        const int yy = y0 - 2 + t / kFRW, xx = x0 - 2 + t % kFRW;
+        soff[k] = (t < kFRP && xx >= 0 && yy >= 0 && xx < W && yy < H) ? (int)(xx * a.fin.sx + yy * a.fin.sy) : -1;
     }
-    Raw<IN_FIELD> preA[kPer], preB[kPer];
-    auto fetch_plane = [&](Raw<IN_FIELD>* pre, int zi) {
+    const bool fin64 = a.fin.is64;
+    double preA[kPer], preB[kPer];
+    auto fetch_plane = [&](double* pre, int zi) {
+        const bool zin = zi >= 0 && zi < D;
+        const long long zo = a.fin.off + zi * a.fin.sz;
 #pragma unroll
-        for (int k = 0; k < kPer; ++k)
-            if (sl[k] >= 0) pre[k] = fetch<IN_FIELD>(a.in, a.in2, a.fin, fscale, D, H, W, b, sxx[k], syy[k], zi, 0);
-    };
-    auto store_plane = [&](const Raw<IN_FIELD>* pre, int zi) {
-        uint16_t* dst = lr + (zi & 3) * kRP;
-#pragma unroll
-        for (int k = 0; k < kPer; ++k)
-            if (sl[k] >= 0) dst[sl[k]] = (uint16_t)finish<IN_FIELD>(pre[k], a.fin, fscale)[0];
-    };
-    // the coarse activation up(E1) of this wave's groups, loaded one plane ahead
-    u32x4 e1v[kGW];
-    auto fetch_e1 = [&](u32x4* e1, int s) {
-        const bool zin = s >= 0 && s < D;
-#pragma unroll
-        for (int k = 0; k < kGW; ++k) {
-            e1[k] = u32x4{0u, 0u, 0u, 0u};
-            if (zin && lo2 && ((gdom >> k) & 1u)) e1[k] = *(const u32x4*)(a.in.p + gco[k] + (s >> 1) * cplane);
+        for (int k = 0; k < kPer; ++k) {
+            pre[k] = 0.0;
+            if (zin && soff[k] >= 0)
+                pre[k] = fin64 ? ((const double*)a.fin.p)[zo + soff[k]] : (double)((const float*)a.fin.p)[zo + soff[k]];
         }
     };
-    // one plane of X -> tap partials, this wave's groups
-    auto partials = [&](const u32x4* e1, int s) {
-        const bool zin = s >= 0 && s < D;
+    auto store_plane = [&](const double* pre, int zi) {
+        uint16_t* dst = lr + mod3(zi) * kFRP;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int t = tid + 256 * k;
+            if (t < kFRP) {
+                const float v = fin64 ? (float)(pre[k] * (double)fscale) : (float)pre[k] * fscale;
+                const uint16_t h = (uint16_t)f2bf(v);
+                dst[t] = h;
+                dst[t + 3 * kFRP] = h;
+            }
+        }
+    };
+    // up(E1) of this wavefront's rows, loaded one plane ahead; lanes outside
+    // the domain never load and keep the zeros they start with
+    u32x2 ev[kFG];
+#pragma unroll
+    for (int k = 0; k < kFG; ++k) ev[k] = u32x2{0u, 0u};
+    auto fetch_e = [&](u32x2* e, int s) {
+        const char* base = (const char*)(a.in.p + (size_t)(s >> 1) * cplane);  // uniform
+        if (xin) {
+#pragma unroll
+            for (int k = 0; k < kFG; ++k)
+                if ((yin >> k) & 1u) e[k] = *(const u32x2*)(base + (unsigned)(2 * (ebase + 2 * k * cW * kC)));
+        }
+    };
+    // one plane of X -> tap partials, this wavefront's rows
+    auto partials = [&](const u32x2* e, int s) {
         int pe[8];
+        const int sbase = mod3(s - 1) * kFRP;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) pe[e] = ((s + tdz[e] - 1) & 3) * kRP + toff[e];
+        for (int e8 = 0; e8 < 8; ++e8) pe[e8] = sbase + toff[e8];
 #pragma unroll
-        for (int k = 0; k < kGW; ++k) {
-            if (wave + 4 * k >= kGroups) break;  // uniform per wavefront
-            u32x4 bw;
+        for (int k = 0; k < kFG; ++k) {
+            u16x2 bw[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e)
-                bw[e] = (unsigned)lr[pe[2 * e] + gbase[k]] | ((unsigned)lr[pe[2 * e + 1] + gbase[k]] << 16);
-            const bf16x8 B = __builtin_bit_cast(bf16x8, bw);
-            const f32x4 c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl1, B, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-            const f32x4 c2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl2, B, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-            // X = BF16(BF16(relu(lift)) + up(E1)) on the domain, 0 outside (zero padding)
-            u32x4 bx = {0u, 0u, 0u, 0u};
-            if (zin && ((gdom >> k) & 1u) && lo2) {
-                const float lv[8] = {c1[0], c1[1], c1[2], c1[3], c2[0], c2[1], c2[2], c2[3]};
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const unsigned h = pack2bf(fmaxf(lv[2 * e] + bl[2 * e], 0.0f), fmaxf(lv[2 * e + 1] + bl[2 * e + 1], 0.0f));
-                    bx[e] = pack2bf(bf2f(h & 0xffffu) + bf2f(e1[k][e] & 0xffffu),
-                                    __uint_as_float(h & 0xffff0000u) + __uint_as_float(e1[k][e] & 0xffff0000u));
-                }
+            for (int e8 = 0; e8 < 4; ++e8) {
+                bw[e8].x = lr[pe[2 * e8] + 4 * k * kFRW];
+                bw[e8].y = lr[pe[2 * e8 + 1] + 4 * k * kFRW];
             }
+            const f32x4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wlift, __builtin_bit_cast(bf16x8, bw), cinit[k],
+                                                                    0, 0, 0);
+            u32x4 bx;
+            bx[0] = relu_bf2(pack2bf(c[0], c[1]));
+            bx[1] = relu_bf2(pack2bf(c[2], c[3]));
+            bx[2] = e[k][0];
+            bx[3] = e[k][1];
             const bf16x8 X = __builtin_bit_cast(bf16x8, bx);
-            const f32x4 d0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, X, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-            const f32x4 d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, X, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-            if (gq[k] < kQ) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    lp[(4 * j + r) * kQ + gq[k]] = d0[r];
-                    lp[(16 + 4 * j + r) * kQ + gq[k]] = d1[r];
-                }
-            }
+            const f32x4 p0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X, wp0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            const f32x4 p1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X, wp1, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            // lane (tap n, j): voxels 4j .. 4j+3 of row wave + 4k
+            float* dst = lp + n * kXQp + (wave + 4 * k) * kXW + 4 * j;
+            *(f32x4*)dst = p0;
+            if (n < kTaps - 16) *(f32x4*)(dst + 16 * kXQp) = p1;
         }
     };
     // this thread's output voxel
-    const int ox = tid & 15, oy = tid >> 4;
+    const int oy = tid / kFW, ox = tid - oy * kFW;
     const int x = x0 + ox, y = y0 + oy;
-    const bool mine = x < W && y < H;
-    const int q0 = oy * kLW + ox;
+    const bool mine = tid < kFW * kFH && x < W && y < H;
+    const int q0 = oy * kXW + ox;
     float am1 = 0.0f, a0 = 0.0f, ap1 = 0.0f;
 
     // planes s = z0-1 .. z1 of X; the ring holds input planes s-1 .. s+1
@@ -679,7 +745,7 @@ __global__ __launch_bounds__(256, 3) void k_outf(ConvArgs a, int zc)
     store_plane(preA, z0 - 1);
     fetch_plane(preA, z0);
     fetch_plane(preB, z0 + 1);
-    fetch_e1(e1v, z0 - 1);
+    if (z0 >= 1) fetch_e(ev, z0 - 1);
     for (int s = z0 - 1; s <= z1; ++s) {
         const bool even = ((s - z0 + 1) & 1) == 0;
         if (even) {
@@ -689,26 +755,29 @@ __global__ __launch_bounds__(256, 3) void k_outf(ConvArgs a, int zc)
             store_plane(preB, s + 1);
             if (s + 3 <= z1 + 1) fetch_plane(preB, s + 3);
         }
+        const bool sin = s >= 0 && s < D;  // X plane inside the domain (uniform)
         __syncthreads();  // input plane s+1 staged; every thread is past the previous gather
-        partials(e1v, s);
-        if (s + 1 <= z1) fetch_e1(e1v, s + 1);
+        if (sin) partials(ev, s);
+        if (s + 1 <= z1 && s + 1 < D) fetch_e(ev, s + 1);
         __syncthreads();  // partials of plane s complete
-        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+        if (sin && tid < kFW * kFH) {
+            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
-        for (int dy = 0; dy < 3; ++dy)
+            for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-                const float* pq = lp + (dy * 3 + dx) * kQ + q0 + dy * kLW + dx;
-                s0 += pq[0];
-                s1 += pq[9 * kQ];
-                s2 += pq[18 * kQ];
-            }
-        am1 += s2;
-        a0 += s1;
-        ap1 += s0;
+                for (int dx = 0; dx < 3; ++dx) {
+                    const float* pq = lp + (dy * 3 + dx) * kXQp + q0 + dy * kXW + dx;
+                    s0 += pq[0];
+                    s1 += pq[9 * kXQp];
+                    s2 += pq[18 * kXQp];
+                }
+            am1 += s2;
+            a0 += s1;
+            ap1 += s0;
+        }
         const int zo = s - 1;
         if (zo >= z0 && zo < z1 && mine) {
-            const long long o = fidx(a.fout, b, x, y, zo);
+            const long long o = fidx(a.fout, 0, x, y, zo);
             const float v = am1 + bout;
             if (a.fout.is64) ((double*)a.fout.p)[o] = (double)v;
             else ((float*)a.fout.p)[o] = v;
@@ -721,8 +790,12 @@ __global__ __launch_bounds__(256, 3) void k_outf(ConvArgs a, int zc)
 
 void launch_outf(const ConvArgs& a, hipStream_t s)
 {
-    const int zc = pick_zc(a.out.D, a.out.H, a.out.W, a.out.B);
-    hipLaunchKernelGGL(k_outf, conv_grid(a.out.D, a.out.H, a.out.W, a.out.B, zc), dim3(256), 0, s, a, zc);
+    const int D = a.out.D, H = a.out.H, W = a.out.W;
+    const int tiles = ((W + kFW - 1) / kFW) * ((H + kFH - 1) / kFH);
+    int zc = D;
+    while (zc > 16 && (long long)tiles * ((D + zc - 1) / zc) < 8192) zc = (zc + 1) / 2;
+    const dim3 grid((unsigned)((W + kFW - 1) / kFW), (unsigned)((H + kFH - 1) / kFH), (unsigned)((D + zc - 1) / zc));
+    hipLaunchKernelGGL(k_outf, grid, dim3(256), 0, s, a, zc);
 }
 
 // ------------------------------------------------------------ weight gradient
@@ -952,8 +1025,8 @@ void launch_upadd_back(const ActRef& dx, const ActRef& dskip, int accumulate_ski
 // forward:        W'[tap][k][m] = W[tap][ci = k][co = m]
 // backward data:  W'[tap][k][m] = W[26 - tap][ci = m][co = k]   (flipped, transposed)
 // lift (mode 2, 1 input channel): one operand, k = 8*(lane/16) + e is the tap
-// tap partials (mode 3, 1 output channel): two operands o, row m = tap 16*o + m,
-// k = input channel (k >= 16 zero)
+// tap partials (mode 3, 1 output channel): two operands o, column m = tap 16*o + m,
+// k = 8*(lane/16) + e -> input channel 4*(lane/16) + e%4
 __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int mode, uint16_t* __restrict__ wl)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -965,16 +1038,12 @@ __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int mode,
         const int tap = 8 * j + e;
         if (tap < kTaps && m < cout) v = W[tap * 256 + m];
     } else if (mode == 3) {
+        // the fused output layer's tap-partial weights: two operands o, column
+        // m = tap 16*o + m, k = 8*(lane/16) + e -> input channel 4*(lane/16) + e%4
+        // (the lift half and the skip half of the A operand share the weights)
         if (t >= 1024) return;
-        const int tap = 16 * o + m, k = 8 * j + e;
-        if (tap < kTaps && k < cin && cout > 0) v = W[tap * 256 + k * 16];
-    } else if (mode == 4) {
-        // lift as two operands o whose rows m < 8 are output channels
-        // 8*(m/4) + 4*o + m%4: lane (n, j < 2) of the results then holds
-        // channels 8j .. 8j+7 (o = 0: +0..3, o = 1: +4..7)
-        if (t >= 1024) return;
-        const int tap = 8 * j + e, co = 8 * (m >> 2) + 4 * o + (m & 3);
-        if (m < 8 && tap < kTaps && co < cout) v = W[tap * 256 + co];
+        const int tap = 16 * o + m, ci = 4 * j + (e & 3);
+        if (tap < kTaps && ci < cin && cout > 0) v = W[tap * 256 + ci * 16];
     } else {
         if (t >= kPackedHalves) return;
         const int p = o / 3, dy = o % 3, c = pair_combo(p, j >> 1), k = 8 * (j & 1) + e;
@@ -1115,7 +1184,7 @@ Net::~Net()
     release(inf_);
     release(trn_);
     for (void* p : {(void*)params_, (void*)grads_, (void*)adam_m_, (void*)adam_v_, (void*)wl_, (void*)wlt_, (void*)wlo_,
-                    (void*)wlf_, (void*)data_r_})
+                    (void*)data_r_})
         if (p) (void)hipFree(p);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -1137,7 +1206,6 @@ int Net::init(unsigned long long seed)
     if (!rc) rc = hip(hipMalloc(&wl_, (size_t)kLayers * kPackedHalves * 2), "hipMalloc weights");
     if (!rc) rc = hip(hipMalloc(&wlt_, (size_t)kLayers * kPackedHalves * 2), "hipMalloc weights");
     if (!rc) rc = hip(hipMalloc(&wlo_, 1024 * 2), "hipMalloc weights");
-    if (!rc) rc = hip(hipMalloc(&wlf_, 1024 * 2), "hipMalloc weights");
     if (rc) return rc;
     // He initialisation (std = sqrt(2 / fan_in)) of the ReLU layers, sqrt(1 / fan_in) for the output
     std::vector<float> h(kParams, 0.0f);
@@ -1163,11 +1231,9 @@ void Net::pack_all()
         hipLaunchKernelGGL(k_pack, dim3(kPackedHalves / 256), dim3(256), 0, stream_, W, kNet[l].cin, kNet[l].cout, 1,
                            wlt_ + (size_t)l * kPackedHalves);
     }
-    // the output layer again as tap-partial operands and the lift as two
-    // row-permuted operands, for the fused inference kernel
+    // the output layer again as tap-partial operands, for the fused inference kernel
     hipLaunchKernelGGL(k_pack, dim3(4), dim3(256), 0, stream_, params_ + (size_t)(kLayers - 1) * kLayerParams,
                        kNet[kLayers - 1].cin, kNet[kLayers - 1].cout, 3, wlo_);
-    hipLaunchKernelGGL(k_pack, dim3(4), dim3(256), 0, stream_, params_, kNet[0].cin, kNet[0].cout, 4, wlf_);
 }
 
 int Net::set_params(const float* host, long long n)
@@ -1324,7 +1390,7 @@ ConvArgs Net::layer_args(const BufSet& bs, int i, const FieldRef& fin, const Fie
         a.fused = 1;
         a.fin = fin;
         a.wl = wlo_;
-        a.wl2 = wlf_;  // layer 0's lift, row-permuted operands
+        a.wl2 = wl_;  // layer 0's lift operand
         a.bias2 = params_ + kTaps * 256;
     }
     return a;

@@ -137,3 +137,33 @@ def test_rounded_forward_stays_close_to_exact():
     y32, _ = O.forward(p, r, rounding=True)
     y64, _ = O.forward(p.astype(np.float64), r.astype(np.float64), rounding=False)
     assert np.linalg.norm(y32 - y64) / np.linalg.norm(y64) < 2e-2
+
+
+def test_exact_out_skip_changes_only_the_output_skip_rounding():
+    """exact_out_skip leaves every activation as it is and, without rounding,
+    the output too; with rounding the output moves by about the BF16 rounding
+    of the skip sum (0.4 % per value at most), not more."""
+    rng = np.random.default_rng(10)
+    p = O.init_params(11)
+    r = rng.normal(size=(1, 11, 9, 14)).astype(np.float32)
+    y0, a0 = O.forward(p, r, rounding=True)
+    y1, a1 = O.forward(p, r, rounding=True, exact_out_skip=True)
+    for u, v in zip(a0, a1):
+        assert np.array_equal(u, v)
+    d = np.linalg.norm(y1 - y0) / np.linalg.norm(y0)
+    assert 0.0 < d < 1e-2
+    z0, _ = O.forward(p.astype(np.float64), r.astype(np.float64), rounding=False)
+    z1, _ = O.forward(p.astype(np.float64), r.astype(np.float64), rounding=False, exact_out_skip=True)
+    assert np.array_equal(z0, z1)
+
+
+def test_output_layer_matches_forward():
+    """output_layer from the forward's own activations reproduces the forward's
+    output (rounded skip sum) and its exact_out_skip variant."""
+    rng = np.random.default_rng(12)
+    p = O.init_params(13)
+    r = rng.normal(size=(1, 10, 7, 12)).astype(np.float32)
+    y, acts = O.forward(p, r, rounding=True)
+    assert np.allclose(O.output_layer(p, acts[0], acts[12]), y, rtol=0, atol=1e-6 * np.abs(y).max())
+    ye, _ = O.forward(p, r, rounding=True, exact_out_skip=True)
+    assert np.allclose(O.output_layer(p, acts[0], acts[12], exact_skip=True), ye, rtol=0, atol=1e-6 * np.abs(y).max())
