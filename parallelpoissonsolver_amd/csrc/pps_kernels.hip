@@ -177,23 +177,47 @@ __device__ __forceinline__ void finish_iteration(Scal* S, double err, bool histo
 
 // --------------------------------------------------------- kernel skeletons
 
-// Stencil ops may carry a side value from the staging read of a point to its
-// output (Op::Side, returned through in(c, side)): the thread that computes a
-// point also staged it, so the value need not be read from HBM again.
-struct NoSide {};
-template <class Op, class = void> struct SideOf { using type = NoSide; };
-template <class Op> struct SideOf<Op, std::void_t<typename Op::Side>> { using type = typename Op::Side; };
-template <class Op, typename S>
-__device__ __forceinline__ auto stage_in(const Op& op, long long c, S& side)
+// Stencil op hooks.  A plain op has in(c) (the staged value, a plain load) and
+// out(c, v, Av, acc).  An op whose staged value is computed from several
+// fields, or whose output reads more fields, declares
+//   struct Raw; Raw in_raw(c); T in_value(const Raw&)   -- loads, and their combination
+//   struct Ox;  Ox out_raw(c)                            -- the output's extra loads
+//   out(c, v, Av, const Raw& centre, const Ox&, acc)
+// (Raw defaults to T, Ox to NoRaw).  The kernel issues the raw loads planes
+// ahead and combines them only when needed, so no load is waited on as soon
+// as it is issued.
+struct NoRaw {};
+template <class Op, typename T, class = void> struct RawOf { using type = T; };
+template <class Op, typename T> struct RawOf<Op, T, std::void_t<typename Op::Raw>> { using type = typename Op::Raw; };
+template <class Op, class = void> struct OxOf { using type = NoRaw; };
+template <class Op> struct OxOf<Op, std::void_t<typename Op::Ox>> { using type = typename Op::Ox; };
+template <class Op, typename T>
+constexpr bool kHooks = !std::is_same_v<typename RawOf<Op, T>::type, T> || !std::is_same_v<typename OxOf<Op>::type, NoRaw>;
+
+template <class Op, typename T>
+__device__ __forceinline__ typename RawOf<Op, T>::type st_raw(const Op& op, long long c)
 {
-    if constexpr (std::is_same_v<S, NoSide>) return op.in(c);
-    else return op.in(c, side);
+    if constexpr (std::is_same_v<typename RawOf<Op, T>::type, T>) return op.in(c);
+    else return op.in_raw(c);
 }
-template <class Op, typename T, typename S>
-__device__ __forceinline__ void stage_out(const Op& op, long long c, T v, T Av, const S& side, double* acc)
+template <class Op, typename T>
+__device__ __forceinline__ T st_value(const Op& op, const typename RawOf<Op, T>::type& r)
 {
-    if constexpr (std::is_same_v<S, NoSide>) op.out(c, v, Av, acc);
-    else op.out(c, v, Av, side, acc);
+    if constexpr (std::is_same_v<typename RawOf<Op, T>::type, T>) return r;
+    else return op.in_value(r);
+}
+template <class Op>
+__device__ __forceinline__ typename OxOf<Op>::type st_ox(const Op& op, long long c)
+{
+    if constexpr (std::is_same_v<typename OxOf<Op>::type, NoRaw>) return NoRaw{};
+    else return op.out_raw(c);
+}
+template <class Op, typename T>
+__device__ __forceinline__ void st_out(const Op& op, long long c, T v, T Av, const typename RawOf<Op, T>::type& r,
+                                       const typename OxOf<Op>::type& o, double* acc)
+{
+    if constexpr (kHooks<Op, T>) op.out(c, v, Av, r, o, acc);
+    else op.out(c, v, Av, acc);
 }
 
 template <typename T, int NQ, class Op>
@@ -215,13 +239,13 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
     const int k1 = min(k0 + g.kz, g.hi2);
 
     // halo producers: 128 threads for the two y-halo rows, 2*kTH for the x-halo columns
-    int hr = 0, hc = 0, hi = 0, hj = 0;
+    int hr = 0, hc_ = 0, hi = 0, hj = 0;
     bool hv = false;
     if (tid < 2 * kTX) {
         const int col = tid % kTX;
         const bool top = tid >= kTX;
         hr = top ? kTH + 1 : 0;
-        hc = col + 1;
+        hc_ = col + 1;
         hi = g.lo0 + tx0 * kTX + col;
         hj = jt + (top ? kTH : -1);
         hv = true;
@@ -230,7 +254,7 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
         const int row = t % kTH;
         const bool right = t >= kTH;
         hr = row + 1;
-        hc = right ? kTX + 1 : 0;
+        hc_ = right ? kTX + 1 : 0;
         hi = g.lo0 + tx0 * kTX + (right ? kTX : -1);
         hj = jt + row;
         hv = true;
@@ -239,35 +263,51 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
     const long long hbase = (long long)g.ox + hi + (long long)g.px * hj;
     const T cx = (T)g.cx, cy = (T)g.cy, cz = (T)g.cz;
 
-    using Side = typename SideOf<Op>::type;
+    using Raw = typename RawOf<Op, T>::type;
+    using Ox = typename OxOf<Op>::type;
     bool act[kRY], inb[kRY];
     long long base[kRY];
-    T vm[kRY], vc[kRY];
-    Side sc[kRY], sp[kRY];
+    // per row q: values of planes k-1, k, k+1; raw loads of planes k (centre,
+    // for the output), k+1 and k+2; the output's extra loads for planes k, k+1
+    T vm[kRY], vc[kRY], vp[kRY];
+    Raw rc[kRY], r1[kRY], r2[kRY];
+    Ox oc[kRY], on[kRY];
 #pragma unroll
     for (int q = 0; q < kRY; ++q) {
         const int j = jt + ly + q * kTY;
         act[q] = (i < g.hi0) && (j < g.hi1);
         inb[q] = (i <= g.hi0) && (j <= g.hi1);
         base[q] = (long long)g.ox + i + (long long)g.px * j;
-        vm[q] = inb[q] ? op.in(base[q] + g.plane * (k0 - 1)) : T(0);
-        vc[q] = inb[q] ? stage_in(op, base[q] + g.plane * k0, sc[q]) : T(0);
+        vm[q] = inb[q] ? st_value<Op, T>(op, st_raw<Op, T>(op, base[q] + g.plane * (k0 - 1))) : T(0);
+        rc[q] = inb[q] ? st_raw<Op, T>(op, base[q] + g.plane * k0) : Raw{};
+        r1[q] = inb[q] ? st_raw<Op, T>(op, base[q] + g.plane * (k0 + 1)) : Raw{};
+        if (act[q]) oc[q] = st_ox(op, base[q] + g.plane * k0);
+        vc[q] = inb[q] ? st_value<Op, T>(op, rc[q]) : T(0);
     }
     double acc[NQ > 0 ? NQ : 1];
 #pragma unroll
     for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) acc[q] = 0.0;
 
-    T hcur = hin ? op.in(hbase + g.plane * k0) : T(0);
+    Raw hc = hin ? st_raw<Op, T>(op, hbase + g.plane * k0) : Raw{};
+    Raw h1 = (hin && k0 + 1 < k1) ? st_raw<Op, T>(op, hbase + g.plane * (k0 + 1)) : Raw{};
     int buf = 0;
     for (int k = k0; k < k1; ++k) {
-        // loads for plane k+1 first: they complete while plane k is computed
-        T vp[kRY];
+        // loads first: plane k+2 (used from the next step on), the output's
+        // loads of plane k+1, the halo of plane k+2
+        const bool more = k + 1 < k1;
 #pragma unroll
-        for (int q = 0; q < kRY; ++q) vp[q] = inb[q] ? stage_in(op, base[q] + g.plane * (k + 1), sp[q]) : T(0);
-        const T hnext = (hin && k + 1 < k1) ? op.in(hbase + g.plane * (k + 1)) : T(0);
+        for (int q = 0; q < kRY; ++q) {
+            if (inb[q] && more) r2[q] = st_raw<Op, T>(op, base[q] + g.plane * (k + 2));
+            if (act[q] && more) on[q] = st_ox(op, base[q] + g.plane * (k + 1));
+        }
+        Raw h2{};
+        if (hin && k + 2 < k1) h2 = st_raw<Op, T>(op, hbase + g.plane * (k + 2));
+        // plane k+1's values: its loads were issued one step ago
+#pragma unroll
+        for (int q = 0; q < kRY; ++q) vp[q] = inb[q] ? st_value<Op, T>(op, r1[q]) : T(0);
 #pragma unroll
         for (int q = 0; q < kRY; ++q) lds[buf][ly + q * kTY + 1][lx + 1] = vc[q];
-        if (hv) lds[buf][hr][hc] = hcur;
+        if (hv) lds[buf][hr][hc_] = hin ? st_value<Op, T>(op, hc) : T(0);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < kRY; ++q) {
@@ -277,13 +317,16 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
                 const T Av = cx * ((lds[buf][r][lx] - v) + (lds[buf][r][lx + 2] - v)) +
                              cy * ((lds[buf][r - 1][lx + 1] - v) + (lds[buf][r + 1][lx + 1] - v)) +
                              cz * ((vm[q] - v) + (vp[q] - v));
-                stage_out(op, base[q] + g.plane * k, v, Av, sc[q], acc);
+                st_out<Op, T>(op, base[q] + g.plane * k, v, Av, rc[q], oc[q], acc);
             }
             vm[q] = vc[q];
             vc[q] = vp[q];
-            sc[q] = sp[q];
+            rc[q] = r1[q];
+            r1[q] = r2[q];
+            oc[q] = on[q];
         }
-        hcur = hnext;
+        hc = h1;
+        h1 = h2;
         buf ^= 1;
     }
     if constexpr (NQ > 0) block_partials<NQ>(acc, rb);
@@ -528,10 +571,12 @@ struct OpResidual : Gate {
     int mode;
     double jac;
     __device__ void load() { jac = S->v[S_JAC]; }
+    struct Ox { double b; };
     __device__ double in(long long c) const { return x[c]; }
-    __device__ void out(long long c, double, double Av, double* acc) const
+    __device__ Ox out_raw(long long c) const { return {b[c]}; }
+    __device__ void out(long long c, double, double Av, const double&, const Ox& o, double* acc) const
     {
-        const double rv = b[c] - Av;
+        const double rv = o.b - Av;
         r[c] = rv;
         acc[0] += rv * rv;
         acc[1] += rv * (jac * rv);
@@ -751,18 +796,16 @@ struct OpCgA : Gate {
         beta = (T)S->v[S_BETA];
         alpha_prev = (T)S->v[S_ALPHA];
     }
-    struct Side { T pold; };  // p_old at the point, for the folded x update
-    __device__ T in(long long c) const { return a * src[c] + beta * pin[c]; }
-    __device__ T in(long long c, Side& s) const
-    {
-        s.pold = pin[c];
-        return a * src[c] + beta * s.pold;
-    }
-    __device__ void out(long long c, T vc, T Av, const Side& s, double* acc) const
+    struct Raw { T src, pold; };  // p_old at the point also feeds the folded x update
+    struct Ox { T x; };
+    __device__ Raw in_raw(long long c) const { return {src[c], pin[c]}; }
+    __device__ T in_value(const Raw& r) const { return a * r.src + beta * r.pold; }
+    __device__ Ox out_raw(long long c) const { return {x[c]}; }
+    __device__ void out(long long c, T vc, T Av, const Raw& r, const Ox& o, double* acc) const
     {
         put(pout + c, vc, nt);
         put(Ap + c, Av, nt);
-        put(x + c, x[c] + alpha_prev * s.pold, nt);
+        put(x + c, o.x + alpha_prev * r.pold, nt);
         acc[0] += (double)vc * (double)Av;
     }
     __device__ void finish(const double* t)
@@ -901,11 +944,15 @@ struct OpBiK1 : Gate {
     Scal* S;
     T amp;
     __device__ void load() {}
-    __device__ T in(long long c) const { return amp * Mp[c]; }
-    __device__ void out(long long c, T, T Av, double* acc) const
+    struct Raw { T m; };
+    struct Ox { T r0; };
+    __device__ Raw in_raw(long long c) const { return {Mp[c]}; }
+    __device__ T in_value(const Raw& r) const { return amp * r.m; }
+    __device__ Ox out_raw(long long c) const { return {r0[c]}; }
+    __device__ void out(long long c, T, T Av, const Raw&, const Ox& o, double* acc) const
     {
         AMp[c] = Av;
-        acc[0] += (double)r0[c] * (double)Av;
+        acc[0] += (double)o.r0 * (double)Av;
     }
     __device__ void finish(const double* t) { S->v[S_ALPHA] = S->v[S_RHO0] / t[0]; }
 };
@@ -956,11 +1003,15 @@ struct OpBiK3 : Gate {
     Scal* S;
     T az;
     __device__ void load() {}
-    __device__ T in(long long c) const { return az * z[c]; }
-    __device__ void out(long long c, T, T Av, double* acc) const
+    struct Raw { T z; };
+    struct Ox { T r; };
+    __device__ Raw in_raw(long long c) const { return {z[c]}; }
+    __device__ T in_value(const Raw& v) const { return az * v.z; }
+    __device__ Ox out_raw(long long c) const { return {r[c]}; }
+    __device__ void out(long long c, T, T Av, const Raw&, const Ox& o, double* acc) const
     {
         Az[c] = Av;
-        acc[0] += (double)r[c] * (double)Av;
+        acc[0] += (double)o.r * (double)Av;
         acc[1] += (double)Av * (double)Av;
     }
     __device__ void finish(const double* t) { S->v[S_OMEGA] = t[0] / t[1]; }
@@ -1139,10 +1190,12 @@ struct OpChebSweep : Gate {
     T a, b, cz;
     double sign;
     __device__ void load() {}
+    struct Ox { T b, z; };
     __device__ T in(long long c) const { return Y[c]; }
-    __device__ void out(long long c, T vc, T Av, double*) const
+    __device__ Ox out_raw(long long c) const { return {B[c], Z[c]}; }
+    __device__ void out(long long c, T vc, T Av, const T&, const Ox& o, double*) const
     {
-        const T w = a * vc + b * (B[c] + Av) - cz * Z[c];
+        const T w = a * vc + b * (o.b + Av) - cz * o.z;
         if constexpr (sizeof(TO) == sizeof(T)) W[c] = sign < 0 ? -w : w;
         else W[c] = (TO)(sign * (double)w);
     }
@@ -1207,14 +1260,19 @@ struct OpFcgP : Gate {
     Scal* S;
     T beta;
     __device__ void load() { beta = (T)S->v[S_BETA]; }
-    // beta = 0 takes z as it is, so a restart never multiplies a non-finite pold
-    __device__ T in(long long c) const { return beta == (T)0 ? z[c] : z[c] - beta * pold[c]; }
-    __device__ void out(long long c, T vc, T Av, double* acc) const
+    // beta = 0 takes z as it is (pold is not read), so a restart never
+    // multiplies a non-finite pold
+    struct Raw { T z, pold; };
+    struct Ox { T r; };
+    __device__ Raw in_raw(long long c) const { return {z[c], beta == (T)0 ? (T)0 : pold[c]}; }
+    __device__ T in_value(const Raw& v) const { return beta == (T)0 ? v.z : v.z - beta * v.pold; }
+    __device__ Ox out_raw(long long c) const { return {r[c]}; }
+    __device__ void out(long long c, T vc, T Av, const Raw&, const Ox& o, double* acc) const
     {
         p[c] = vc;
         Ap[c] = Av;
         acc[0] += (double)vc * (double)Av;
-        acc[1] += (double)vc * (double)r[c];
+        acc[1] += (double)vc * (double)o.r;
     }
     __device__ void finish(const double* t)
     {
@@ -1312,10 +1370,12 @@ struct OpIrRes : Gate {
     Scal* So;
     Scal* SI;
     __device__ void load() {}
+    struct Ox { double b; };
     __device__ double in(long long c) const { return x[c]; }
-    __device__ void out(long long c, double, double Av, double* acc) const
+    __device__ Ox out_raw(long long c) const { return {b[c]}; }
+    __device__ void out(long long c, double, double Av, const double&, const Ox& o, double* acc) const
     {
-        const double rv = b[c] - Av;
+        const double rv = o.b - Av;
         const float rf = (float)rv;
         r32[c] = rf;
         acc[0] += rv * rv;

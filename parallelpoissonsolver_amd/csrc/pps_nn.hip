@@ -178,12 +178,29 @@ __device__ __forceinline__ u32x4 load_chunk(const ActRef& in, const ActRef& in2,
 
 __host__ __device__ constexpr int pair_combo(int p, int m) { return 2 * p + m; }  // < 9 valid
 
+// Block -> tile.  The hardware deals consecutive workgroups round-robin to the
+// 8 XCDs, each with its own L2; renumbered, each XCD owns a contiguous run of
+// tiles (x fastest, then y, then z-chunk), so neighbouring tiles' halo reads
+// and partial output lines meet in one L2 instead of going to HBM once per XCD.
+struct TileIdx {
+    int x, y, z;
+};
+__device__ __forceinline__ TileIdx tile_of_block()
+{
+    const int nx = gridDim.x, ny = gridDim.y;
+    const int nb = nx * ny * gridDim.z;
+    const int b = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+    const int full = nb & ~7;
+    const int t = b >= full ? b : (b & 7) * (full >> 3) + (b >> 3);
+    return {t % nx, (t / nx) % ny, t / (nx * ny)};
+}
+
 // raw staging values: global loads are issued one plane ahead into these and
 // converted to BF16 chunks only when the plane is written to LDS
 template <int MI> struct Raw { u32x4 a; };
 template <> struct Raw<IN_MASK> { u32x4 a, m; };
 template <> struct Raw<IN_UPADD> { u32x4 a, s; };
-template <> struct Raw<IN_FIELD> { double a; };
+template <> struct Raw<IN_FIELD> { u32x2 a; };  // raw bits: FP64, or FP32 in .x
 
 template <int MI>
 __device__ __forceinline__ Raw<MI> fetch(const ActRef& in, const ActRef& in2, const FieldRef& f, float fscale, int D,
@@ -205,7 +222,8 @@ __device__ __forceinline__ Raw<MI> fetch(const ActRef& in, const ActRef& in2, co
     } else {
         if (ch == 0) {
             const long long i = fidx(f, b, x, y, z);
-            r.a = f.is64 ? ((const double*)f.p)[i] : (double)((const float*)f.p)[i];
+            if (f.is64) r.a = ((const u32x2*)f.p)[i];
+            else r.a.x = ((const unsigned*)f.p)[i];
         }
     }
     return r;
@@ -231,7 +249,7 @@ __device__ __forceinline__ u32x4 finish(const Raw<MI>& r, const FieldRef& f, flo
         return pack8(a);
     } else {
         u32x4 o = {0u, 0u, 0u, 0u};
-        const float v = f.is64 ? (float)(r.a * (double)fscale) : (float)r.a * fscale;
+        const float v = f.is64 ? (float)(__builtin_bit_cast(double, r.a) * (double)fscale) : __uint_as_float(r.a.x) * fscale;
         o[0] = f2bf(v);
         return o;
     }
@@ -271,10 +289,11 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
     __shared__ u32x4 lds[4 * kPlane];
     const int D = a.out.D, H = a.out.H, W = a.out.W;
     const int nzc = (D + zc - 1) / zc;
-    const int b = blockIdx.z / nzc;
-    const int z0 = (blockIdx.z - b * nzc) * zc;
+    const TileIdx ti = tile_of_block();
+    const int b = ti.z / nzc;
+    const int z0 = (ti.z - b * nzc) * zc;
     const int z1 = min(z0 + zc, D);
-    const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTHt;
+    const int x0 = ti.x * kTW, y0 = ti.y * kTHt;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane >> 4, n = lane & 15;
     float fscale = 1.0f;
@@ -409,10 +428,11 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
     const int D = a.out.D, H = a.out.H, W = a.out.W;
     const int nzc = (D + zc - 1) / zc;
     auto mod4 = [](int v) { return v & 3; };
-    const int b = blockIdx.z / nzc;
-    const int z0 = (blockIdx.z - b * nzc) * zc;
+    const TileIdx ti = tile_of_block();
+    const int b = ti.z / nzc;
+    const int z0 = (ti.z - b * nzc) * zc;
     const int z1 = min(z0 + zc, D);
-    const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTHt;
+    const int x0 = ti.x * kTW, y0 = ti.y * kTHt;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane >> 4, n = lane & 15;
     const float fscale = field_scale(a.fin);
@@ -442,25 +462,30 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
         soff[k] = (t < kP && xx >= 0 && yy >= 0 && xx < W && yy < H) ? xx * a.fin.sx + yy * a.fin.sy + b * a.fin.sb : -1;
     }
     const bool fin64 = a.fin.is64;
-    double preA[kPer], preB[kPer];
-    auto fetch_plane = [&](double* pre, int zi) {
+    // raw bits of the staged values (FP64, or FP32 in .x): converted only when
+    // the plane is written to LDS, so the loads stay in flight meanwhile
+    u32x2 preA[kPer], preB[kPer];
+    auto fetch_plane = [&](u32x2* pre, int zi) {
         const bool zin = zi >= 0 && zi < D;
         const long long zo = a.fin.off + zi * a.fin.sz;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-            pre[k] = 0.0;
-            if (zin && soff[k] >= 0)
-                pre[k] = fin64 ? ((const double*)a.fin.p)[zo + soff[k]] : (double)((const float*)a.fin.p)[zo + soff[k]];
+            pre[k] = u32x2{0u, 0u};
+            if (zin && soff[k] >= 0) {
+                if (fin64) pre[k] = ((const u32x2*)a.fin.p)[zo + soff[k]];
+                else pre[k].x = ((const unsigned*)a.fin.p)[zo + soff[k]];
+            }
         }
     };
-    auto store_plane = [&](const double* pre, int zi) {
+    auto store_plane = [&](const u32x2* pre, int zi) {
         uint16_t* dst = lds + mod4(zi) * kP;
         const bool twice = mod4(zi) < 2;  // uniform
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int t = tid + 256 * k;
             if (t < kP) {
-                const float v = fin64 ? (float)(pre[k] * (double)fscale) : (float)pre[k] * fscale;
+                const float v = fin64 ? (float)(__builtin_bit_cast(double, pre[k]) * (double)fscale)
+                                      : __uint_as_float(pre[k].x) * fscale;
                 const uint16_t h = (uint16_t)f2bf(v);
                 dst[t] = h;
                 if (twice) dst[t + 4 * kP] = h;
@@ -611,9 +636,10 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     __shared__ uint16_t lr[6 * kFRP];  // input planes, each in two slots
     __shared__ __align__(16) float lp[kTaps * kXQp];  // P[tap][voxel of the X region]
     const int D = a.out.D, H = a.out.H, W = a.out.W;
-    const int z0 = blockIdx.z * zc;     // one sample
+    const TileIdx ti = tile_of_block();
+    const int z0 = ti.z * zc;  // one sample
     const int z1 = min(z0 + zc, D);
-    const int x0 = blockIdx.x * kFW, y0 = blockIdx.y * kFH;
+    const int x0 = ti.x * kFW, y0 = ti.y * kFH;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane >> 4, n = lane & 15;
     const float fscale = field_scale(a.fin);
@@ -664,24 +690,29 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
         soff[k] = (t < kFRP && xx >= 0 && yy >= 0 && xx < W && yy < H) ? (int)(xx * a.fin.sx + yy * a.fin.sy) : -1;
     }
     const bool fin64 = a.fin.is64;
-    double preA[kPer], preB[kPer];
-    auto fetch_plane = [&](double* pre, int zi) {
+    // raw bits of the staged values (FP64, or FP32 in .x): converted only when
+    // the plane is written to LDS, so the loads stay in flight meanwhile
+    u32x2 preA[kPer], preB[kPer];
+    auto fetch_plane = [&](u32x2* pre, int zi) {
         const bool zin = zi >= 0 && zi < D;
         const long long zo = a.fin.off + zi * a.fin.sz;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-            pre[k] = 0.0;
-            if (zin && soff[k] >= 0)
-                pre[k] = fin64 ? ((const double*)a.fin.p)[zo + soff[k]] : (double)((const float*)a.fin.p)[zo + soff[k]];
+            pre[k] = u32x2{0u, 0u};
+            if (zin && soff[k] >= 0) {
+                if (fin64) pre[k] = ((const u32x2*)a.fin.p)[zo + soff[k]];
+                else pre[k].x = ((const unsigned*)a.fin.p)[zo + soff[k]];
+            }
         }
     };
-    auto store_plane = [&](const double* pre, int zi) {
+    auto store_plane = [&](const u32x2* pre, int zi) {
         uint16_t* dst = lr + mod3(zi) * kFRP;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int t = tid + 256 * k;
             if (t < kFRP) {
-                const float v = fin64 ? (float)(pre[k] * (double)fscale) : (float)pre[k] * fscale;
+                const float v = fin64 ? (float)(__builtin_bit_cast(double, pre[k]) * (double)fscale)
+                                      : __uint_as_float(pre[k].x) * fscale;
                 const uint16_t h = (uint16_t)f2bf(v);
                 dst[t] = h;
                 dst[t + 3 * kFRP] = h;
