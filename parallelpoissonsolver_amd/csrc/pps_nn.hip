@@ -411,6 +411,51 @@ __device__ __forceinline__ unsigned relu_bf2(unsigned h)
 }
 __device__ __forceinline__ int mod3(int v) { return ((v % 3) + 3) % 3; }
 
+// Staging of the one-channel network input (an FP64 or FP32 field) into a
+// BF16 LDS plane: a thread owns kPer slots of the staged region.  Loads come
+// from clamped in-domain addresses, so no divergent control flow surrounds
+// them and they stay in flight until the plane is converted; the domain mask
+// (zero padding) is applied at the conversion.
+template <bool F64, int kPer>
+struct FieldStage {
+    using Raw = std::conditional_t<F64, u32x2, unsigned>;
+    const Raw* p;
+    long long off, sz;
+    int soff[kPer];   // clamped in-plane element offset of slot k
+    unsigned valid;   // bit k: slot k exists and lies inside the domain in x and y
+    float scale;
+    // region origin (rx, ry), row width rw, slots per plane np
+    __device__ void init(const FieldRef& f, int b, int rx, int ry, int rw, int np, int W, int H, int tid)
+    {
+        p = (const Raw*)f.p;
+        off = f.off + b * f.sb;
+        sz = f.sz;
+        scale = field_scale(f);
+        valid = 0u;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int t = tid + 256 * k;
+            const int yy = ry + t / rw, xx = rx + t % rw;
+            if (t < np && xx >= 0 && yy >= 0 && xx < W && yy < H) valid |= 1u << k;
+            soff[k] = (int)(min(max(xx, 0), W - 1) * f.sx + min(max(yy, 0), H - 1) * f.sy);
+        }
+    }
+    __device__ void fetch(Raw* pre, int zi, int D) const
+    {
+        const long long zo = off + (long long)min(max(zi, 0), D - 1) * sz;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) pre[k] = p[zo + soff[k]];
+    }
+    // BF16 bits of slot k of plane zi
+    __device__ uint16_t bf(const Raw* pre, int k, int zi, int D) const
+    {
+        float v;
+        if constexpr (F64) v = (float)(__builtin_bit_cast(double, pre[k]) * (double)scale);
+        else v = __uint_as_float(pre[k]) * scale;
+        return (zi >= 0 && zi < D && ((valid >> k) & 1u)) ? (uint16_t)f2bf(v) : (uint16_t)0;
+    }
+};
+
 // Lift (1 -> 16 channels): K = the 27 taps of one channel (+5 zero taps), one
 // MFMA per 16 voxels; lane (voxel n, block j) gathers taps 8j .. 8j+7 from a
 // BF16 staging ring and receives channels 4j .. 4j+3.  The accumulator starts
@@ -420,6 +465,7 @@ __device__ __forceinline__ int mod3(int v) { return ((v % 3) + 3) % 3; }
 // slots (and plane z+2, which a wavefront ahead of the others may already be
 // writing before the next barrier, in none of them).  Input stage as IN_FIELD
 // (scaled to unit RMS).
+template <bool F64>
 __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
@@ -435,7 +481,6 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
     const int x0 = ti.x * kTW, y0 = ti.y * kTHt;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane >> 4, n = lane & 15;
-    const float fscale = field_scale(a.fin);
     const bf16x8 w = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[lane]);
     const int x = x0 + n;
     f32x4 cinit[4];
@@ -454,64 +499,38 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
         toff[e] = (tap / 9) * kP + ((tap / 3) % 3 + wave * 4) * kLW + tap % 3 + n;
     }
     constexpr int kPer = (kP + 255) / 256;
-    long long soff[kPer];  // in-plane offset of the staged voxel in the field; -1: outside
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const int t = tid + 256 * k;
-        const int yy = y0 - 1 + t / kLW, xx = x0 - 1 + t % kLW;
-        soff[k] = (t < kP && xx >= 0 && yy >= 0 && xx < W && yy < H) ? xx * a.fin.sx + yy * a.fin.sy + b * a.fin.sb : -1;
-    }
-    const bool fin64 = a.fin.is64;
-    // raw bits of the staged values (FP64, or FP32 in .x): converted only when
-    // the plane is written to LDS, so the loads stay in flight meanwhile
-    u32x2 preA[kPer], preB[kPer];
-    auto fetch_plane = [&](u32x2* pre, int zi) {
-        const bool zin = zi >= 0 && zi < D;
-        const long long zo = a.fin.off + zi * a.fin.sz;
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            pre[k] = u32x2{0u, 0u};
-            if (zin && soff[k] >= 0) {
-                if (fin64) pre[k] = ((const u32x2*)a.fin.p)[zo + soff[k]];
-                else pre[k].x = ((const unsigned*)a.fin.p)[zo + soff[k]];
-            }
-        }
-    };
-    auto store_plane = [&](const u32x2* pre, int zi) {
+    FieldStage<F64, kPer> fs;
+    fs.init(a.fin, b, x0 - 1, y0 - 1, kLW, kP, W, H, tid);
+    using Raw = typename FieldStage<F64, kPer>::Raw;
+    Raw preA[kPer], preB[kPer];
+    auto store_plane = [&](const Raw* pre, int zi) {
         uint16_t* dst = lds + mod4(zi) * kP;
         const bool twice = mod4(zi) < 2;  // uniform
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int t = tid + 256 * k;
             if (t < kP) {
-                const float v = fin64 ? (float)(__builtin_bit_cast(double, pre[k]) * (double)fscale)
-                                      : __uint_as_float(pre[k].x) * fscale;
-                const uint16_t h = (uint16_t)f2bf(v);
+                const uint16_t h = fs.bf(pre, k, zi, D);
                 dst[t] = h;
                 if (twice) dst[t + 4 * kP] = h;
             }
         }
     };
-    fetch_plane(preA, z0 - 1);
+    fs.fetch(preA, z0 - 1, D);
     store_plane(preA, z0 - 1);
-    fetch_plane(preA, z0);
+    fs.fetch(preA, z0, D);
     store_plane(preA, z0);
-    fetch_plane(preA, z0 + 1);
-    if (z0 + 2 <= z1) fetch_plane(preB, z0 + 2);
+    fs.fetch(preA, z0 + 1, D);
+    fs.fetch(preB, z0 + 2, D);
     // pooled copy (2x2x2 average of the stored BF16 outputs) for the next level:
     // y pairs are neighbouring rows of this wave and z pairs consecutive planes
     // (chunks start on even planes), summed per lane; x pairs are neighbouring
     // lanes, added when the pooled voxel is written
     float ps[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-    for (int z = z0; z < z1; ++z) {
-        // two planes in flight, as in k_conv
-        if (((z - z0) & 1) == 0) {
-            store_plane(preA, z + 1);
-            if (z + 3 <= z1) fetch_plane(preA, z + 3);
-        } else {
-            store_plane(preB, z + 1);
-            if (z + 3 <= z1) fetch_plane(preB, z + 3);
-        }
+    // one plane: two planes of loads in flight, each buffer refilled once written
+    auto step = [&](int z, Raw* buf) {
+        store_plane(buf, z + 1);
+        fs.fetch(buf, z + 3, D);  // clamped beyond the chunk; unused there
         __syncthreads();
         int pe[8];
         const int sbase = mod4(z - 1) * kP;
@@ -556,6 +575,10 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
                 for (int r = 0; r < 4; ++r) ps[ip][r] = 0.0f;
             }
         }
+    };
+    for (int z = z0; z < z1; z += 2) {
+        step(z, preA);
+        if (z + 1 < z1) step(z + 1, preB);
     }
 }
 
@@ -599,7 +622,9 @@ void launch_conv(const ConvArgs& a, hipStream_t s)
 void launch_lift(const ConvArgs& a, hipStream_t s)
 {
     const int zc = pick_zc(a.out.D, a.out.H, a.out.W, a.out.B);
-    hipLaunchKernelGGL(k_lift, conv_grid(a.out.D, a.out.H, a.out.W, a.out.B, zc), dim3(256), 0, s, a, zc);
+    const dim3 grid = conv_grid(a.out.D, a.out.H, a.out.W, a.out.B, zc);
+    if (a.fin.is64) hipLaunchKernelGGL(k_lift<true>, grid, dim3(256), 0, s, a, zc);
+    else hipLaunchKernelGGL(k_lift<false>, grid, dim3(256), 0, s, a, zc);
 }
 
 // Inference form of the output layer with the lift fused in:
@@ -630,6 +655,7 @@ constexpr int kXQp = kXQ + 4;                     // P row stride: 4 mod 64 bank
 constexpr int kFG = kXH / 4;                      // x-rows (groups) per wavefront
 static_assert(kXW == 16 && kXH % 4 == 0, "one 16-voxel x-row per group, equal rows per wavefront");
 
+template <bool F64>
 __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
@@ -642,7 +668,6 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     const int x0 = ti.x * kFW, y0 = ti.y * kFH;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane >> 4, n = lane & 15;
-    const float fscale = field_scale(a.fin);
     const bf16x8 wlift = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl2)[lane]);
     const bf16x8 wp0 = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[lane]);
     const bf16x8 wp1 = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[64 + lane]);
@@ -682,38 +707,17 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     // plane p goes to ring slots p mod 3 and p mod 3 + 3, so that planes s-1,
     // s, s+1 always sit in three consecutive slots.
     constexpr int kPer = (kFRP + 255) / 256;
-    int soff[kPer];  // in-plane offset of the staged voxel in the field; -1: outside
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const int t = tid + 256 * k;
-        const int yy = y0 - 2 + t / kFRW, xx = x0 - 2 + t % kFRW;
-        soff[k] = (t < kFRP && xx >= 0 && yy >= 0 && xx < W && yy < H) ? (int)(xx * a.fin.sx + yy * a.fin.sy) : -1;
-    }
-    const bool fin64 = a.fin.is64;
-    // raw bits of the staged values (FP64, or FP32 in .x): converted only when
-    // the plane is written to LDS, so the loads stay in flight meanwhile
-    u32x2 preA[kPer], preB[kPer];
-    auto fetch_plane = [&](u32x2* pre, int zi) {
-        const bool zin = zi >= 0 && zi < D;
-        const long long zo = a.fin.off + zi * a.fin.sz;
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            pre[k] = u32x2{0u, 0u};
-            if (zin && soff[k] >= 0) {
-                if (fin64) pre[k] = ((const u32x2*)a.fin.p)[zo + soff[k]];
-                else pre[k].x = ((const unsigned*)a.fin.p)[zo + soff[k]];
-            }
-        }
-    };
-    auto store_plane = [&](const u32x2* pre, int zi) {
+    FieldStage<F64, kPer> fs;
+    fs.init(a.fin, 0, x0 - 2, y0 - 2, kFRW, kFRP, W, H, tid);
+    using Raw = typename FieldStage<F64, kPer>::Raw;
+    Raw preA[kPer], preB[kPer];
+    auto store_plane = [&](const Raw* pre, int zi) {
         uint16_t* dst = lr + mod3(zi) * kFRP;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int t = tid + 256 * k;
             if (t < kFRP) {
-                const float v = fin64 ? (float)(__builtin_bit_cast(double, pre[k]) * (double)fscale)
-                                      : __uint_as_float(pre[k].x) * fscale;
-                const uint16_t h = (uint16_t)f2bf(v);
+                const uint16_t h = fs.bf(pre, k, zi, D);
                 dst[t] = h;
                 dst[t + 3 * kFRP] = h;
             }
@@ -770,22 +774,17 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     float am1 = 0.0f, a0 = 0.0f, ap1 = 0.0f;
 
     // planes s = z0-1 .. z1 of X; the ring holds input planes s-1 .. s+1
-    fetch_plane(preA, z0 - 2);
+    fs.fetch(preA, z0 - 2, D);
     store_plane(preA, z0 - 2);
-    fetch_plane(preA, z0 - 1);
+    fs.fetch(preA, z0 - 1, D);
     store_plane(preA, z0 - 1);
-    fetch_plane(preA, z0);
-    fetch_plane(preB, z0 + 1);
+    fs.fetch(preA, z0, D);
+    fs.fetch(preB, z0 + 1, D);
     if (z0 >= 1) fetch_e(ev, z0 - 1);
-    for (int s = z0 - 1; s <= z1; ++s) {
-        const bool even = ((s - z0 + 1) & 1) == 0;
-        if (even) {
-            store_plane(preA, s + 1);
-            if (s + 3 <= z1 + 1) fetch_plane(preA, s + 3);
-        } else {
-            store_plane(preB, s + 1);
-            if (s + 3 <= z1 + 1) fetch_plane(preB, s + 3);
-        }
+    // one plane: two planes of input loads in flight, each buffer refilled once written
+    auto step = [&](int s, Raw* buf) {
+        store_plane(buf, s + 1);
+        fs.fetch(buf, s + 3, D);  // clamped beyond the chunk; unused there
         const bool sin = s >= 0 && s < D;  // X plane inside the domain (uniform)
         __syncthreads();  // input plane s+1 staged; every thread is past the previous gather
         if (sin) partials(ev, s);
@@ -816,6 +815,10 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
         am1 = a0;
         a0 = ap1;
         ap1 = 0.0f;
+    };
+    for (int s = z0 - 1; s <= z1; s += 2) {
+        step(s, preA);
+        if (s + 1 <= z1) step(s + 1, preB);
     }
 }
 
@@ -826,7 +829,8 @@ void launch_outf(const ConvArgs& a, hipStream_t s)
     int zc = D;
     while (zc > 16 && (long long)tiles * ((D + zc - 1) / zc) < 8192) zc = (zc + 1) / 2;
     const dim3 grid((unsigned)((W + kFW - 1) / kFW), (unsigned)((H + kFH - 1) / kFH), (unsigned)((D + zc - 1) / zc));
-    hipLaunchKernelGGL(k_outf, grid, dim3(256), 0, s, a, zc);
+    if (a.fin.is64) hipLaunchKernelGGL(k_outf<true>, grid, dim3(256), 0, s, a, zc);
+    else hipLaunchKernelGGL(k_outf<false>, grid, dim3(256), 0, s, a, zc);
 }
 
 // ------------------------------------------------------------ weight gradient
