@@ -44,6 +44,15 @@ const LayerDef kNet[kLayers] = {
     {16, 1, 0, IN_UPADD, 12, 0, 0},    // 13 out      up(act12) + act0 -> z (linear)
 };
 
+// activation i is read through a 2x2x2 average pool by a later layer: its
+// producer then also writes the pooled copy, which that layer reads as it is
+static bool pooled_act(int i)
+{
+    for (int l = 0; l < kLayers; ++l)
+        if (kNet[l].mode_in == IN_POOL && kNet[l].in_act == i) return true;
+    return false;
+}
+
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -229,6 +238,32 @@ __device__ __forceinline__ Raw<MI> fetch(const ActRef& in, const ActRef& in2, co
     return r;
 }
 
+// fetch without bounds checks: the caller passes a voxel inside the level
+// (clamped) and masks the result when it is outside
+template <int MI>
+__device__ __forceinline__ Raw<MI> fetch_raw(const ActRef& in, const ActRef& in2, const FieldRef& f, float fscale, int D,
+                                             int H, int W, int b, int x, int y, int z, int ch)
+{
+    Raw<MI> r{};
+    if constexpr (MI == IN_PLAIN) {
+        r.a = *(const u32x4*)(in.p + vox(in, b, x, y, z) + ch * 8);
+    } else if constexpr (MI == IN_MASK) {
+        const long long o = vox(in, b, x, y, z) + ch * 8;
+        r.a = *(const u32x4*)(in.p + o);
+        r.m = *(const u32x4*)(in2.p + o);
+    } else if constexpr (MI == IN_UPADD) {
+        r.a = *(const u32x4*)(in.p + vox(in, b, x >> 1, y >> 1, z >> 1) + ch * 8);
+        r.s = *(const u32x4*)(in2.p + vox(in2, b, x, y, z) + ch * 8);
+    } else if constexpr (MI == IN_POOL) {
+        r.a = load_chunk<IN_POOL>(in, in2, f, fscale, D, H, W, b, x, y, z, ch);
+    } else {
+        const long long i = fidx(f, b, x, y, z);
+        if (f.is64) r.a = ((const u32x2*)f.p)[i];
+        else r.a.x = ((const unsigned*)f.p)[i];
+    }
+    return r;
+}
+
 template <int MI>
 __device__ __forceinline__ u32x4 finish(const Raw<MI>& r, const FieldRef& f, float fscale)
 {
@@ -282,11 +317,12 @@ __device__ __forceinline__ void store_out(const ConvArgs& a, const f32x4& acc, c
 
 constexpr int kChunksPerThread = (kPlane + 255) / 256;
 
-template <int MI, int MO>
+// PO: also write the 2x2x2-pooled copy of the (RELU) output to a.pout
+template <int MI, int MO, bool PO = false>
 __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
-    __shared__ u32x4 lds[4 * kPlane];
+    __shared__ u32x4 lds[4 * kPlane + 1];  // + a dummy slot for threads without a chunk
     const int D = a.out.D, H = a.out.H, W = a.out.W;
     const int nzc = (D + zc - 1) / zc;
     const TileIdx ti = tile_of_block();
@@ -309,32 +345,43 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
         for (int r = 0; r < 4; ++r) bias[r] = a.bias[4 * j + r];
     }
 
-    // this thread's staging chunks: plane position and LDS slot, fixed for the march
+    // this thread's staging chunks: source voxel clamped into the level, LDS
+    // slot (the dummy slot for threads without one) and whether the voxel is
+    // inside the level in x and y (else the chunk is zero padding).  The loads
+    // run without bounds branches, so they stay in flight until the plane is
+    // written.
     int sx[kChunksPerThread], sy[kChunksPerThread], sc[kChunksPerThread], sl[kChunksPerThread];
+    unsigned sval = 0u;
 #pragma unroll
     for (int k = 0; k < kChunksPerThread; ++k) {
         const int t = tid + 256 * k;
         const int yy = t / (2 * kLW);
         const int rem = t - yy * (2 * kLW);
-        sx[k] = x0 - 1 + (rem >> 1);
-        sy[k] = y0 - 1 + yy;
+        const int xs = x0 - 1 + (rem >> 1), ys = y0 - 1 + yy;
+        sx[k] = min(max(xs, 0), W - 1);
+        sy[k] = min(max(ys, 0), H - 1);
         sc[k] = rem & 1;
-        sl[k] = t < kPlane ? (yy * 2 + (rem & 1)) * kLW + (rem >> 1) : -1;
+        sl[k] = t < kPlane ? (yy * 2 + (rem & 1)) * kLW + (rem >> 1) : 4 * kPlane;
+        if (t < kPlane && xs >= 0 && ys >= 0 && xs < W && ys < H && (MI != IN_FIELD || sc[k] == 0)) sval |= 1u << k;
     }
     // planes in flight: preA / preB hold raw loads issued one or two steps earlier
     Raw<MI> preA[kChunksPerThread], preB[kChunksPerThread];
     auto fetch_plane = [&](Raw<MI>* pre, int zi) {
+        const int zc_ = min(max(zi, 0), D - 1);
 #pragma unroll
         for (int k = 0; k < kChunksPerThread; ++k)
-            if (sl[k] >= 0) pre[k] = fetch<MI>(a.in, a.in2, a.fin, fscale, D, H, W, b, sx[k], sy[k], zi, sc[k]);
+            pre[k] = fetch_raw<MI>(a.in, a.in2, a.fin, fscale, D, H, W, b, sx[k], sy[k], zc_, sc[k]);
     };
     auto store_plane = [&](const Raw<MI>* pre, int zi) {
         u32x4* dst = lds + (zi & 3) * kPlane;
+        const bool zin = zi >= 0 && zi < D;
 #pragma unroll
-        for (int k = 0; k < kChunksPerThread; ++k)
-            if (sl[k] >= 0) dst[sl[k]] = finish<MI>(pre[k], a.fin, fscale);
+        for (int k = 0; k < kChunksPerThread; ++k) {
+            const u32x4 v = finish<MI>(pre[k], a.fin, fscale);
+            (sl[k] < 4 * kPlane ? dst[sl[k]] : lds[4 * kPlane]) =
+                (zin && ((sval >> k) & 1u)) ? v : u32x4{0u, 0u, 0u, 0u};
+        }
     };
-
     // lane geometry of the two pair members: member j>>1, channel half j&1
     const int m = j >> 1;
     fetch_plane(preA, z0 - 1);
@@ -342,23 +389,17 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
     fetch_plane(preA, z0);
     store_plane(preA, z0);
     // prefetch depth: two planes where the raw loads fit the register budget of
-    // three blocks per CU, one plane for the two-source input stages
-    constexpr int kDepth = (MI == IN_UPADD || MI == IN_MASK) ? 1 : 2;
+    // three blocks per CU, one plane for the two-source and pooled input stages
+    constexpr int kDepth = (MI == IN_UPADD || MI == IN_MASK || MI == IN_POOL || PO) ? 1 : 2;
     fetch_plane(preA, z0 + 1);
-    if (kDepth == 2 && z0 + 2 <= z1) fetch_plane(preB, z0 + 2);
+    if constexpr (kDepth == 2) fetch_plane(preB, z0 + 2);
     const int x = x0 + n;
-    for (int z = z0; z < z1; ++z) {
-        if constexpr (kDepth == 1) {
-            store_plane(preA, z + 1);
-            if (z + 2 <= z1) fetch_plane(preA, z + 2);
-        } else if (((z - z0) & 1) == 0) {
-            // plane z+1 comes from the buffer of its parity; refill it with plane z+3
-            store_plane(preA, z + 1);
-            if (z + 3 <= z1) fetch_plane(preA, z + 3);
-        } else {
-            store_plane(preB, z + 1);
-            if (z + 3 <= z1) fetch_plane(preB, z + 3);
-        }
+    float ps[PO ? 2 : 1][4] = {};  // pooled-copy sums over two planes
+    // one plane; the buffer just written is refilled kDepth planes ahead
+    // (clamped beyond the chunk, unused there)
+    auto step = [&](int z, Raw<MI>* buf) {
+        store_plane(buf, z + 1);
+        fetch_plane(buf, z + 1 + kDepth);
         __syncthreads();
         f32x4 acc[4];
 #pragma unroll
@@ -393,10 +434,57 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
             }
             if constexpr (kPipe) __builtin_amdgcn_sched_barrier(0);  // keep the reads kPf steps ahead
         }
+        if constexpr (MO == OUT_RELU) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int y = y0 + wave * 4 + i;
-            if (x < W && y < H) store_out<MO>(a, acc[i], bias, b, x, y, z, j);
+            for (int i = 0; i < 4; ++i) {
+                const int y = y0 + wave * 4 + i;
+                u32x2 o;
+                o[0] = pack2bf(fmaxf(acc[i][0] + bias[0], 0.0f), fmaxf(acc[i][1] + bias[1], 0.0f));
+                o[1] = pack2bf(fmaxf(acc[i][2] + bias[2], 0.0f), fmaxf(acc[i][3] + bias[3], 0.0f));
+                if (x < W && y < H) {
+                    *(u32x2*)(a.out.p + vox(a.out, b, x, y, z) + 4 * j) = o;
+                    if constexpr (PO) {
+                        ps[i >> 1][0] += bf2f(o[0] & 0xffffu);
+                        ps[i >> 1][1] += __uint_as_float(o[0] & 0xffff0000u);
+                        ps[i >> 1][2] += bf2f(o[1] & 0xffffu);
+                        ps[i >> 1][3] += __uint_as_float(o[1] & 0xffff0000u);
+                    }
+                }
+            }
+            // pooled copy for a layer that reads this output through a 2x2x2
+            // average: x pairs are neighbouring lanes, y pairs rows of this
+            // wavefront, z pairs consecutive planes (chunks start on even planes)
+            if (PO && ((z & 1) || z == D - 1)) {
+#pragma unroll
+                for (int ip = 0; ip < 2; ++ip) {
+                    float v[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) v[r] = 0.125f * (ps[ip][r] + xor1(ps[ip][r]));
+                    const int yf = y0 + wave * 4 + 2 * ip;
+                    if (!(n & 1) && x < W && yf < H) {
+                        u32x2 o;
+                        o[0] = pack2bf(v[0], v[1]);
+                        o[1] = pack2bf(v[2], v[3]);
+                        *(u32x2*)(a.pout.p + vox(a.pout, b, x >> 1, yf >> 1, z >> 1) + 4 * j) = o;
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) ps[ip][r] = 0.0f;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int y = y0 + wave * 4 + i;
+                if (x < W && y < H) store_out<MO>(a, acc[i], bias, b, x, y, z, j);
+            }
+        }
+    };
+    if constexpr (kDepth == 1) {
+        for (int z = z0; z < z1; ++z) step(z, preA);
+    } else {
+        for (int z = z0; z < z1; z += 2) {
+            step(z, preA);
+            if (z + 1 < z1) step(z + 1, preB);
         }
     }
 }
@@ -600,7 +688,16 @@ template <int MI>
 static void conv_mo(const ConvArgs& a, hipStream_t s, dim3 grid, int zc)
 {
     switch (a.mode_out) {
-    case OUT_RELU: hipLaunchKernelGGL((k_conv<MI, OUT_RELU>), grid, dim3(256), 0, s, a, zc); break;
+    case OUT_RELU:
+        // pooled copies come from plain-input layers only (checked by Net::init)
+        if constexpr (MI == IN_PLAIN) {
+            if (a.pout.p) {
+                hipLaunchKernelGGL((k_conv<MI, OUT_RELU, true>), grid, dim3(256), 0, s, a, zc);
+                break;
+            }
+        }
+        hipLaunchKernelGGL((k_conv<MI, OUT_RELU>), grid, dim3(256), 0, s, a, zc);
+        break;
     case OUT_LINEAR: hipLaunchKernelGGL((k_conv<MI, OUT_LINEAR>), grid, dim3(256), 0, s, a, zc); break;
     default: hipLaunchKernelGGL((k_conv<MI, OUT_FIELD>), grid, dim3(256), 0, s, a, zc); break;
     }
@@ -1233,6 +1330,11 @@ long long Net::effective_params()
 
 int Net::init(unsigned long long seed)
 {
+    // a pooled activation's producer writes its pooled copy: the lift, or a
+    // plain-input convolution with a ReLU output
+    for (int i = 0; i < kActs; ++i)
+        if (pooled_act(i) && !(i == 0 || (kNet[i].mode_in == IN_PLAIN && i < kLayers - 1)))
+            return fail("network table: activation %d is pooled but its producer cannot write the pooled copy", i);
     int rc = hip(hipStreamCreate(&stream_), "hipStreamCreate");
     if (!rc) rc = hip(hipMalloc(&params_, kParams * sizeof(float)), "hipMalloc params");
     if (!rc) rc = hip(hipMalloc(&grads_, kParams * sizeof(float)), "hipMalloc grads");
@@ -1340,7 +1442,10 @@ int Net::ensure(BufSet& bs, int B, const int n[3], bool train)
     };
     bs.act.assign(kActs, nullptr);
     bs.dact.assign(kActs, nullptr);
-    int rc = alloc(vox_count(1) * kC * 2, (void**)&bs.pool0);
+    bs.pooled.assign(kActs, nullptr);
+    int rc = 0;
+    for (int i = 0; i < kActs && !rc; ++i)
+        if (pooled_act(i)) rc = alloc(vox_count(kNet[i].level + 1) * kC * 2, (void**)&bs.pooled[i]);
     if (!train) {
         // inference: an activation takes a buffer of its level that holds no live
         // activation (live = read by a later layer), so each level keeps only
@@ -1408,11 +1513,11 @@ ConvArgs Net::layer_args(const BufSet& bs, int i, const FieldRef& fin, const Fie
     if (d.mode_in == IN_FIELD) a.fin = fin;
     else a.in = act(bs, d.in_act);
     if (d.mode_in == IN_UPADD) a.in2 = act(bs, d.skip_act);
-    if (i == 0) a.pout = level(bs, 1, bs.pool0);
-    if (d.mode_in == IN_POOL && d.in_act == 0) {
-        // the lift kernel has written pool(act0): read it as it is
+    if (i < kActs && bs.pooled[i]) a.pout = level(bs, d.level + 1, bs.pooled[i]);
+    if (d.mode_in == IN_POOL) {
+        // the producer has written pool(act): read it as it is
         a.mode_in = IN_PLAIN;
-        a.in = level(bs, 1, bs.pool0);
+        a.in = level(bs, d.level, bs.pooled[d.in_act]);
     }
     a.fout = fout;
     a.wl = wl_ + (size_t)i * kPackedHalves;
@@ -1458,9 +1563,9 @@ void Net::backward(BufSet& bs, const FieldRef& fin)
         if (d.mode_in == IN_FIELD) w.fin = fin;
         else w.in = act(bs, d.in_act);
         if (d.mode_in == IN_UPADD) w.in2 = act(bs, d.skip_act);
-        if (d.mode_in == IN_POOL && d.in_act == 0) {
-            w.mode_in = IN_PLAIN;  // the forward input: pool(act0) as the lift kernel wrote it
-            w.in = level(bs, 1, bs.pool0);
+        if (d.mode_in == IN_POOL) {
+            w.mode_in = IN_PLAIN;  // the forward input: pool(act) as its producer wrote it
+            w.in = level(bs, d.level, bs.pooled[d.in_act]);
         }
         if (i == kLayers - 1) {
             w.dz_mode = 2;
@@ -1592,12 +1697,13 @@ int Net::time_layer(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec
     double rd = 0.0, wr = 0.0;
     switch (d.mode_in) {
     case IN_FIELD: rd = v * (in_prec == 64 ? 8.0 : 4.0); break;
-    case IN_POOL: rd = (d.in_act == 0 ? v : vox[d.level - 1]) * act_b; break;  // b0 reads the lift's pooled copy
+    case IN_POOL: rd = v * act_b; break;  // the producer's pooled copy
     case IN_UPADD: rd = vox[d.level + 1] * act_b + v * act_b; break;
     default: rd = v * act_b; break;
     }
     wr = (i == kLayers - 1) ? v * (out_prec == 64 ? 8.0 : 4.0) : v * act_b;
-    if (i == 0) wr = vox[1] * act_b;  // inference: the pooled copy only
+    if (i == 0) wr = 0.0;  // inference keeps only the lift's pooled copy
+    if (i < kActs && pooled_act(i)) wr += vox[d.level + 1] * act_b;
     double fl = 2.0 * kTaps * d.cin * d.cout * v;
     if (i == kLayers - 1) {
         // fused: reads r and the coarse activation, recomputes the lift

@@ -102,7 +102,7 @@ struct BufSet {
     int dims[3][3] = {};          // per level: D, H, W
     std::vector<uint16_t*> act, dact;
     uint16_t* dx[3] = {nullptr, nullptr, nullptr};  // backward-data scratch per level
-    uint16_t* pool0 = nullptr;                      // pool(act0), written by the lift kernel
+    std::vector<uint16_t*> pooled;                  // pool(act i), written by act i's producer (or null)
     float *in = nullptr, *y = nullptr, *dy = nullptr, *ky = nullptr;
     double *g = nullptr, *lpart = nullptr, *coef = nullptr, *lossv = nullptr;
     float* part = nullptr;
