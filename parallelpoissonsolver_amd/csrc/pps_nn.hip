@@ -728,27 +728,30 @@ void launch_lift(const ConvArgs& a, hipStream_t s)
 //   z = out(up(E1) + relu(lift(r)))
 // so the 16-channel level-0 activation never goes to HBM.  A block owns a
 // 14 x 18 output tile; its X region (tile + 1-voxel halo) is 16 x 20 voxels,
-// one 16-voxel x-row per MFMA group, 5 rows per wavefront.  Per plane s:
+// one 16-voxel x-row per MFMA group, 5 rows per wavefront.  Per plane s and row:
 //   1. lift: one MFMA (K = 27 taps of the BF16 input ring, accumulator started
 //      at the bias, or at -1e30 outside the domain so that relu gives the zero
 //      padding); lane (voxel n, block j) then holds channels 4j .. 4j+3;
-//      relu + BF16 in packed 16-bit form;
-//   2. tap partials P[v][tap] = sum_ci W[tap][ci] (a[v][ci] + e[v][ci]), with
-//      a = the BF16 lift output and e = up(E1): the A operand of lane (n, j)
-//      is (a, e) of channels 4j .. 4j+3 and the weight operand repeats
-//      W[tap][4j .. 4j+3] for both halves, so the skip sum happens inside the
-//      MFMA; two MFMAs (taps 0-15, 16-26), results stored tap-major in LDS
-//      (16-byte writes of 4 consecutive voxels);
-// then every thread adds the 27 shifted partials of its output voxel into
-// three running output planes; plane s completes output plane s-1.
-// The unfused path rounds a + e to BF16 before its MFMA; here the sum is exact,
-// so outputs differ from it by that rounding (relative ~1e-4) and FP32 order.
+//      relu + BF16 in packed 16-bit form.  With e = up(E1) of the same
+//      channels, (a, e) is lane (n, j)'s A operand X;
+//   2. row partials P[dz,dy][n] = sum_dx sum_ci W[dz,dy,dx][ci] (a + e)[n+dx-1][ci]:
+//      X shifted one voxel left and right inside the 16-lane row (DPP), three
+//      MFMAs (one per dx) into columns m = 3 dz + dy; the skip sum a + e
+//      happens inside the MFMA (the weight operand repeats W[.][4j .. 4j+3]
+//      for both halves).  The 9 rows go to LDS as 16-byte writes of 4 voxels;
+// then every thread adds the 9 partials of its output voxel (rows y-1 .. y+1)
+// into three running output planes; plane s completes output plane s-1.
+// The output columns are 1 .. 14 of each row, so a shift never reaches past
+// the row for a value that is used.  The unfused path rounds a + e to BF16
+// before its MFMA; here the sum is exact, so outputs differ from it by that
+// rounding (relative ~1e-3 at most) and FP32 order.
 constexpr int kFW = 14, kFH = 18;                 // output tile
 constexpr int kXW = kFW + 2, kXH = kFH + 2;       // X region: 16 x 20
 constexpr int kFRW = kXW + 2, kFRH = kXH + 2;     // input ring plane: 18 x 22
 constexpr int kFRP = kFRW * kFRH;
 constexpr int kXQ = kXW * kXH;
 constexpr int kXQp = kXQ + 4;                     // P row stride: 4 mod 64 banks
+constexpr int kPRows = 9;                         // (dz, dy) row partials
 constexpr int kFG = kXH / 4;                      // x-rows (groups) per wavefront
 static_assert(kXW == 16 && kXH % 4 == 0, "one 16-voxel x-row per group, equal rows per wavefront");
 
@@ -757,7 +760,7 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
     __shared__ uint16_t lr[6 * kFRP];  // input planes, each in two slots
-    __shared__ __align__(16) float lp[kTaps * kXQp];  // P[tap][voxel of the X region]
+    __shared__ __align__(16) float lp[kPRows * kXQp];  // P[3 dz + dy][voxel of the X region]
     const int D = a.out.D, H = a.out.H, W = a.out.W;
     const TileIdx ti = tile_of_block();
     const int z0 = ti.z * zc;  // one sample
@@ -766,8 +769,9 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane >> 4, n = lane & 15;
     const bf16x8 wlift = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl2)[lane]);
-    const bf16x8 wp0 = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[lane]);
-    const bf16x8 wp1 = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[64 + lane]);
+    bf16x8 wp[3];  // per dx: column m = 3 dz + dy
+#pragma unroll
+    for (int dx = 0; dx < 3; ++dx) wp[dx] = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[64 * dx + lane]);
     const float bout = a.bias ? a.bias[0] : 0.0f;
     // X voxel of lane n: column gx; rows y0 - 1 + wave + 4k (k < kFG)
     const int gx = x0 - 1 + n;
@@ -854,20 +858,26 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
             bx[1] = relu_bf2(pack2bf(c[2], c[3]));
             bx[2] = e[k][0];
             bx[3] = e[k][1];
-            const bf16x8 X = __builtin_bit_cast(bf16x8, bx);
-            const f32x4 p0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X, wp0, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-            const f32x4 p1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(X, wp1, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-            // lane (tap n, j): voxels 4j .. 4j+3 of row wave + 4k
-            float* dst = lp + n * kXQp + (wave + 4 * k) * kXW + 4 * j;
-            *(f32x4*)dst = p0;
-            if (n < kTaps - 16) *(f32x4*)(dst + 16 * kXQp) = p1;
+            // the row's neighbours: voxel n-1 (row_shr:1) and n+1 (row_shl:1)
+            u32x4 bm, bp;
+#pragma unroll
+            for (int e8 = 0; e8 < 4; ++e8) {
+                bm[e8] = (unsigned)__builtin_amdgcn_mov_dpp((int)bx[e8], 0x111, 0xF, 0xF, true);
+                bp[e8] = (unsigned)__builtin_amdgcn_mov_dpp((int)bx[e8], 0x101, 0xF, 0xF, true);
+            }
+            f32x4 pr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, bm), wp[0],
+                                                               f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+            pr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, bx), wp[1], pr, 0, 0, 0);
+            pr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, bp), wp[2], pr, 0, 0, 0);
+            // lane (column m, j): voxels 4j .. 4j+3 of row wave + 4k
+            if (n < kPRows) *(f32x4*)(lp + n * kXQp + (wave + 4 * k) * kXW + 4 * j) = pr;
         }
     };
     // this thread's output voxel
     const int oy = tid / kFW, ox = tid - oy * kFW;
     const int x = x0 + ox, y = y0 + oy;
     const bool mine = tid < kFW * kFH && x < W && y < H;
-    const int q0 = oy * kXW + ox;
+    const int q0 = oy * kXW + ox + 1;  // X region voxel of the output's (x, y - 1) neighbour
     float am1 = 0.0f, a0 = 0.0f, ap1 = 0.0f;
 
     // planes s = z0-1 .. z1 of X; the ring holds input planes s-1 .. s+1
@@ -890,14 +900,12 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
         if (sin && tid < kFW * kFH) {
             float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
-            for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-                for (int dx = 0; dx < 3; ++dx) {
-                    const float* pq = lp + (dy * 3 + dx) * kXQp + q0 + dy * kXW + dx;
-                    s0 += pq[0];
-                    s1 += pq[9 * kXQp];
-                    s2 += pq[18 * kXQp];
-                }
+            for (int dy = 0; dy < 3; ++dy) {
+                const float* pq = lp + dy * kXQp + q0 + dy * kXW;
+                s0 += pq[0];
+                s1 += pq[3 * kXQp];
+                s2 += pq[6 * kXQp];
+            }
             am1 += s2;
             a0 += s1;
             ap1 += s0;
@@ -1157,8 +1165,8 @@ void launch_upadd_back(const ActRef& dx, const ActRef& dskip, int accumulate_ski
 // forward:        W'[tap][k][m] = W[tap][ci = k][co = m]
 // backward data:  W'[tap][k][m] = W[26 - tap][ci = m][co = k]   (flipped, transposed)
 // lift (mode 2, 1 input channel): one operand, k = 8*(lane/16) + e is the tap
-// tap partials (mode 3, 1 output channel): two operands o, column m = tap 16*o + m,
-// k = 8*(lane/16) + e -> input channel 4*(lane/16) + e%4
+// row partials (mode 3, 1 output channel): three operands o = dx, column
+// m = 3 dz + dy, k = 8*(lane/16) + e -> input channel 4*(lane/16) + e%4
 __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int mode, uint16_t* __restrict__ wl)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1170,12 +1178,13 @@ __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int mode,
         const int tap = 8 * j + e;
         if (tap < kTaps && m < cout) v = W[tap * 256 + m];
     } else if (mode == 3) {
-        // the fused output layer's tap-partial weights: two operands o, column
-        // m = tap 16*o + m, k = 8*(lane/16) + e -> input channel 4*(lane/16) + e%4
-        // (the lift half and the skip half of the A operand share the weights)
-        if (t >= 1024) return;
-        const int tap = 16 * o + m, ci = 4 * j + (e & 3);
-        if (tap < kTaps && ci < cin && cout > 0) v = W[tap * 256 + ci * 16];
+        // the fused output layer's row-partial weights: operand o = dx, column
+        // m = 3 dz + dy (m < 9), k = 8*(lane/16) + e -> input channel
+        // 4*(lane/16) + e%4 (the lift half and the skip half of the A operand
+        // share the weights)
+        if (t >= 3 * 512) return;
+        const int ci = 4 * j + (e & 3);
+        if (m < 9 && ci < cin && cout > 0) v = W[(9 * (m / 3) + 3 * (m % 3) + o) * 256 + ci * 16];
     } else {
         if (t >= kPackedHalves) return;
         const int p = o / 3, dy = o % 3, c = pair_combo(p, j >> 1), k = 8 * (j & 1) + e;
@@ -1342,7 +1351,7 @@ int Net::init(unsigned long long seed)
     if (!rc) rc = hip(hipMalloc(&adam_v_, kParams * sizeof(float)), "hipMalloc adam");
     if (!rc) rc = hip(hipMalloc(&wl_, (size_t)kLayers * kPackedHalves * 2), "hipMalloc weights");
     if (!rc) rc = hip(hipMalloc(&wlt_, (size_t)kLayers * kPackedHalves * 2), "hipMalloc weights");
-    if (!rc) rc = hip(hipMalloc(&wlo_, 1024 * 2), "hipMalloc weights");
+    if (!rc) rc = hip(hipMalloc(&wlo_, 3 * 512 * 2), "hipMalloc weights");
     if (rc) return rc;
     // He initialisation (std = sqrt(2 / fan_in)) of the ReLU layers, sqrt(1 / fan_in) for the output
     std::vector<float> h(kParams, 0.0f);
@@ -1368,8 +1377,8 @@ void Net::pack_all()
         hipLaunchKernelGGL(k_pack, dim3(kPackedHalves / 256), dim3(256), 0, stream_, W, kNet[l].cin, kNet[l].cout, 1,
                            wlt_ + (size_t)l * kPackedHalves);
     }
-    // the output layer again as tap-partial operands, for the fused inference kernel
-    hipLaunchKernelGGL(k_pack, dim3(4), dim3(256), 0, stream_, params_ + (size_t)(kLayers - 1) * kLayerParams,
+    // the output layer again as row-partial operands, for the fused inference kernel
+    hipLaunchKernelGGL(k_pack, dim3(6), dim3(256), 0, stream_, params_ + (size_t)(kLayers - 1) * kLayerParams,
                        kNet[kLayers - 1].cin, kNet[kLayers - 1].cout, 3, wlo_);
 }
 
