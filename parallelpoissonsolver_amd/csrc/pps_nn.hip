@@ -489,6 +489,28 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
     }
 }
 
+// Lift operand: the tap in k-slot 8j + e (lane block j, element e), or -1 for
+// the 5 slots with zero weights.  The assignment makes the 16-lane u16 reads of
+// the four blocks j hit disjoint LDS banks (64 or 32 banks) for every e, with
+// the lift ring's plane stride (18 x 18) and the output kernel's (404 = 18 x 22
+// + 8); a zero-weight slot reads the address of a real block of the same e (a
+// broadcast).  The assignment was found by a search over slot permutations.
+__constant__ signed char kLiftTap[32] = {
+    23, 2, 8, 24, 10, 15, -1, 12,  // j = 0, e = 0 .. 7
+    21, 1, 0, 22, 14, 16, 17, -1,  // j = 1
+    20, 5, 6, 25, 13, 3, 18, -1,   // j = 2
+    19, -1, 7, 26, 11, 4, -1, 9,   // j = 3
+};
+// the tap lane block j reads for element e (a real block's tap for a zero slot)
+__device__ __forceinline__ int lift_read_tap(int j, int e)
+{
+    const int t = kLiftTap[8 * j + e];
+    if (t >= 0) return t;
+    for (int jj = 0; jj < 4; ++jj)
+        if (kLiftTap[8 * jj + e] >= 0) return kLiftTap[8 * jj + e];
+    return 0;
+}
+
 // packed BF16 helpers of the lift kernels
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
@@ -578,12 +600,11 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
 #pragma unroll
         for (int r = 0; r < 4; ++r) cinit[i][r] = (x < W && y < H) ? (a.bias ? a.bias[4 * j + r] : 0.0f) : -1e30f;
     }
-    // taps 8j+e of this lane: ring offset relative to the slot of plane z-1 (taps
-    // 27.. reuse tap 26; zero weights)
+    // the taps of this lane's k-slots: ring offset relative to the slot of plane z-1
     int toff[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-        const int tap = min(8 * j + e, 26);
+        const int tap = lift_read_tap(j, e);
         toff[e] = (tap / 9) * kP + ((tap / 3) % 3 + wave * 4) * kLW + tap % 3 + n;
     }
     constexpr int kPer = (kP + 255) / 256;
@@ -749,6 +770,7 @@ constexpr int kFW = 14, kFH = 18;                 // output tile
 constexpr int kXW = kFW + 2, kXH = kFH + 2;       // X region: 16 x 20
 constexpr int kFRW = kXW + 2, kFRH = kXH + 2;     // input ring plane: 18 x 22
 constexpr int kFRP = kFRW * kFRH;
+constexpr int kFRPs = kFRP + 8;                   // ring plane stride (bank spread of the lift reads)
 constexpr int kXQ = kXW * kXH;
 constexpr int kXQp = kXQ + 4;                     // P row stride: 4 mod 64 banks
 constexpr int kPRows = 9;                         // (dz, dy) row partials
@@ -759,7 +781,7 @@ template <bool F64>
 __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
-    __shared__ uint16_t lr[6 * kFRP];  // input planes, each in two slots
+    __shared__ uint16_t lr[6 * kFRPs];  // input planes, each in two slots
     __shared__ __align__(16) float lp[kPRows * kXQp];  // P[3 dz + dy][voxel of the X region]
     const int D = a.out.D, H = a.out.H, W = a.out.W;
     const TileIdx ti = tile_of_block();
@@ -801,8 +823,8 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     int toff[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-        const int tap = min(8 * j + e, 26);
-        toff[e] = (tap / 9) * kFRP + ((tap / 3) % 3 + wave) * kFRW + tap % 3 + n;
+        const int tap = lift_read_tap(j, e);
+        toff[e] = (tap / 9) * kFRPs + ((tap / 3) % 3 + wave) * kFRW + tap % 3 + n;
     }
     // staging of the input ring: region (x0 - 2 .., y0 - 2 ..), 18 x 22.  Input
     // plane p goes to ring slots p mod 3 and p mod 3 + 3, so that planes s-1,
@@ -813,14 +835,14 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     using Raw = typename FieldStage<F64, kPer>::Raw;
     Raw preA[kPer], preB[kPer];
     auto store_plane = [&](const Raw* pre, int zi) {
-        uint16_t* dst = lr + mod3(zi) * kFRP;
+        uint16_t* dst = lr + mod3(zi) * kFRPs;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int t = tid + 256 * k;
             if (t < kFRP) {
                 const uint16_t h = fs.bf(pre, k, zi, D);
                 dst[t] = h;
-                dst[t + 3 * kFRP] = h;
+                dst[t + 3 * kFRPs] = h;
             }
         }
     };
@@ -840,7 +862,7 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     // one plane of X -> tap partials, this wavefront's rows
     auto partials = [&](const u32x2* e, int s) {
         int pe[8];
-        const int sbase = mod3(s - 1) * kFRP;
+        const int sbase = mod3(s - 1) * kFRPs;
 #pragma unroll
         for (int e8 = 0; e8 < 8; ++e8) pe[e8] = sbase + toff[e8];
 #pragma unroll
@@ -1164,7 +1186,8 @@ void launch_upadd_back(const ActRef& dx, const ActRef& dskip, int accumulate_ski
 // empty member).
 // forward:        W'[tap][k][m] = W[tap][ci = k][co = m]
 // backward data:  W'[tap][k][m] = W[26 - tap][ci = m][co = k]   (flipped, transposed)
-// lift (mode 2, 1 input channel): one operand, k = 8*(lane/16) + e is the tap
+// lift (mode 2, 1 input channel): one operand, k = 8*(lane/16) + e holds the tap
+// kLiftTap[k] (zero where it is -1)
 // row partials (mode 3, 1 output channel): three operands o = dx, column
 // m = 3 dz + dy, k = 8*(lane/16) + e -> input channel 4*(lane/16) + e%4
 __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int mode, uint16_t* __restrict__ wl)
@@ -1175,8 +1198,8 @@ __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int mode,
     float v = 0.0f;
     if (mode == 2) {
         if (t >= 512) return;
-        const int tap = 8 * j + e;
-        if (tap < kTaps && m < cout) v = W[tap * 256 + m];
+        const int tap = kLiftTap[8 * j + e];
+        if (tap >= 0 && m < cout) v = W[tap * 256 + m];
     } else if (mode == 3) {
         // the fused output layer's row-partial weights: operand o = dx, column
         // m = 3 dz + dy (m < 9), k = 8*(lane/16) + e -> input channel
