@@ -648,8 +648,8 @@ struct FCGStage : KrylovStage {
             int e = pre_apply(z, r, &S->done);
             if (e) return e;
             launch_fcg_gamma<double>(g, C.stream, C.rb, z.d(), Ap[cur ^ 1].d(), S);
-            launch_fcg_p<double>(g, C.stream, C.rb, z.d(), p[cur ^ 1].d(), p[cur].d(), Ap[cur].d(), r.d(), S);
-            launch_fcg_update<double>(g, C.stream, C.rb, x.d(), p[cur].d(), r.d(), Ap[cur].d(), S, main);
+            launch_fcg_p<double>(g, C.stream, C.rb, z.d(), p[cur ^ 1].d(), p[cur].d(), Ap[cur].d(), r.d(), x.d(), S);
+            launch_fcg_update<double>(g, C.stream, C.rb, r.d(), Ap[cur].d(), S, main);
             cur ^= 1;
             return PPS_OK;
         });
@@ -657,6 +657,8 @@ struct FCGStage : KrylovStage {
         Scal h;
         rc = scal_read(C, S, h);
         if (rc) return rc;
+        // iteration k wrote p[k & 1]; its x update waits for the next stencil pass
+        if (h.iter > 0) launch_fcg_final_x<double>(g, C.stream, x.d(), p[(h.iter - 1) & 1].d(), S);
         C.fallback_switches += h.fallback;
         epilogue(main, x, b, r);
         return PPS_OK;
@@ -788,8 +790,8 @@ struct IRStage : KrylovStage {
                 int e = applyM(&SI->done);
                 if (e) return e;
                 launch_fcg_gamma<float>(g, C.stream, C.rb, z.f(), Ap[cur ^ 1].f(), SI);
-                launch_fcg_p<float>(g, C.stream, C.rb, z.f(), p[cur ^ 1].f(), p[cur].f(), Ap[cur].f(), r32.f(), SI);
-                launch_fcg_update<float>(g, C.stream, C.rb, d.f(), p[cur].f(), r32.f(), Ap[cur].f(), SI, false);
+                launch_fcg_p<float>(g, C.stream, C.rb, z.f(), p[cur ^ 1].f(), p[cur].f(), Ap[cur].f(), r32.f(), d.f(), SI);
+                launch_fcg_update<float>(g, C.stream, C.rb, r32.f(), Ap[cur].f(), SI, false);
                 cur ^= 1;
             } else {
                 launch_cg_A2<float>(g, C.stream, C.rb, pw ? r32.f() : z.f(), p[cur].f(), p[cur ^ 1].f(), Ap[0].f(), d.f(),
@@ -809,6 +811,7 @@ struct IRStage : KrylovStage {
         rc = scal_read(C, SI, h);
         if (rc) return rc;
         if (!flexible && h.iter > 0) launch_cg_final_x<float>(g, C.stream, d.f(), p[h.iter & 1].f(), SI);
+        if (flexible && h.iter > 0) launch_fcg_final_x<float>(g, C.stream, d.f(), p[(h.iter - 1) & 1].f(), SI);
         inner_total += h.iter;
         if (flexible) C.fallback_switches += h.fallback;
         in->prec_iters += h.iter;

@@ -1249,7 +1249,9 @@ void launch_fcg_gamma(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const 
     run_vector<1>(g, s, op, rb);
 }
 
-// p = z - beta pold ; Ap = A p ; p.Ap, p.r ; alpha = p.r / p.Ap
+// p = z - beta pold ; Ap = A p ; p.Ap, p.r ; alpha = p.r / p.Ap ; and the
+// previous iteration's x += alpha_prev pold (folded in: this pass reads pold
+// anyway), unless that step was discarded (restart)
 template <typename T>
 struct OpFcgP : Gate {
     const T* __restrict__ z;
@@ -1257,20 +1259,28 @@ struct OpFcgP : Gate {
     T* __restrict__ p;
     T* __restrict__ Ap;
     const T* __restrict__ r;
+    T* __restrict__ x;
     Scal* S;
-    T beta;
-    __device__ void load() { beta = (T)S->v[S_BETA]; }
-    // beta = 0 takes z as it is (pold is not read), so a restart never
-    // multiplies a non-finite pold
+    T beta, alpha_prev;
+    bool upd;  // apply the previous step to x
+    __device__ void load()
+    {
+        beta = (T)S->v[S_BETA];
+        alpha_prev = (T)S->v[S_ALPHA];
+        upd = !S->restart && alpha_prev != (T)0;
+    }
+    // beta = 0 takes z as it is, and pold is read only when it is used, so a
+    // restart never multiplies a non-finite pold
     struct Raw { T z, pold; };
-    struct Ox { T r; };
-    __device__ Raw in_raw(long long c) const { return {z[c], beta == (T)0 ? (T)0 : pold[c]}; }
+    struct Ox { T r, x; };
+    __device__ Raw in_raw(long long c) const { return {z[c], (beta == (T)0 && !upd) ? (T)0 : pold[c]}; }
     __device__ T in_value(const Raw& v) const { return beta == (T)0 ? v.z : v.z - beta * v.pold; }
-    __device__ Ox out_raw(long long c) const { return {r[c]}; }
-    __device__ void out(long long c, T vc, T Av, const Raw&, const Ox& o, double* acc) const
+    __device__ Ox out_raw(long long c) const { return {r[c], upd ? x[c] : (T)0}; }
+    __device__ void out(long long c, T vc, T Av, const Raw& v, const Ox& o, double* acc) const
     {
         p[c] = vc;
         Ap[c] = Av;
+        if (upd) x[c] = o.x + alpha_prev * v.pold;
         acc[0] += (double)vc * (double)Av;
         acc[1] += (double)vc * (double)o.r;
     }
@@ -1292,7 +1302,8 @@ struct OpFcgP : Gate {
 };
 
 template <typename T>
-void launch_fcg_p(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* pold, T* p, T* Ap, const T* r, Scal* S)
+void launch_fcg_p(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* pold, T* p, T* Ap, const T* r, T* x,
+                  Scal* S)
 {
     OpFcgP<T> op;
     op.gate = &S->done;
@@ -1301,32 +1312,65 @@ void launch_fcg_p(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* p
     op.p = p;
     op.Ap = Ap;
     op.r = r;
+    op.x = x;
     op.S = S;
     run_stencil<T, 2>(g, s, op, rb);
 }
 
+// the last step of an FCG solve: x += alpha p unless it was discarded
 template <typename T>
-struct OpFcgUpdate : Gate {
+struct OpFcgFinalX : Gate {
     T* __restrict__ x;
     const T* __restrict__ p;
+    const Scal* S;
+    T alpha;
+    int skip;
+    struct R { T x, p; };
+    __device__ void load()
+    {
+        alpha = (T)S->v[S_ALPHA];
+        skip = S->restart;
+    }
+    __device__ R fetch(long long c) const { return {x[c], skip ? (T)0 : p[c]}; }
+    __device__ void apply(long long c, const R& v, int, int, int, double*) const
+    {
+        if (!skip) x[c] = v.x + alpha * v.p;
+    }
+    __device__ void finish(const double*) {}
+};
+
+template <typename T>
+void launch_fcg_final_x(const Geo& g, hipStream_t s, T* x, const T* p, const Scal* S)
+{
+    OpFcgFinalX<T> op;
+    op.gate = nullptr;
+    op.x = x;
+    op.p = p;
+    op.S = S;
+    run_vector<0>(g, s, op, kNoRed);
+}
+
+template <typename T>
+struct OpFcgUpdate : Gate {
     T* __restrict__ r;
     const T* __restrict__ Ap;
     Scal* S;
     bool history;
     T alpha;
     int skip;
-    struct R { T x, p, r, ap; };
+    struct R { T r, ap; };
     __device__ void load()
     {
         alpha = (T)S->v[S_ALPHA];
         skip = S->restart;
     }
-    __device__ R fetch(long long c) const { return {x[c], p[c], r[c], Ap[c]}; }
+    // r -= alpha Ap; x takes alpha p in the next stencil pass (or in the final
+    // x kernel after the last iteration)
+    __device__ R fetch(long long c) const { return {r[c], skip ? (T)0 : Ap[c]}; }
     __device__ void apply(long long c, const R& v, int, int, int, double* acc) const
     {
         T rv = v.r;
         if (!skip) {
-            x[c] = v.x + alpha * v.p;
             rv = v.r - alpha * v.ap;
             r[c] = rv;
         }
@@ -1348,12 +1392,10 @@ struct OpFcgUpdate : Gate {
 };
 
 template <typename T>
-void launch_fcg_update(const Geo& g, hipStream_t s, RedBuf rb, T* x, const T* p, T* r, const T* Ap, Scal* S, bool history)
+void launch_fcg_update(const Geo& g, hipStream_t s, RedBuf rb, T* r, const T* Ap, Scal* S, bool history)
 {
     OpFcgUpdate<T> op;
     op.gate = &S->done;
-    op.x = x;
-    op.p = p;
     op.r = r;
     op.Ap = Ap;
     op.S = S;
@@ -1506,8 +1548,9 @@ void launch_ir_inner_init(const Geo& g, hipStream_t s, RedBuf rb, float* r, floa
     template void launch_bicg_k6<T>(const Geo&, hipStream_t, T*, const T*, const T*, const Scal*);             \
     template void launch_cheb_first<T>(const Geo&, hipStream_t, const T*, T*, T*, double, double, const int*); \
     template void launch_fcg_gamma<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, Scal*);             \
-    template void launch_fcg_p<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, T*, T*, const T*, Scal*); \
-    template void launch_fcg_update<T>(const Geo&, hipStream_t, RedBuf, T*, const T*, T*, const T*, Scal*, bool);
+    template void launch_fcg_p<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, T*, T*, const T*, T*, Scal*); \
+    template void launch_fcg_final_x<T>(const Geo&, hipStream_t, T*, const T*, const Scal*);                      \
+    template void launch_fcg_update<T>(const Geo&, hipStream_t, RedBuf, T*, const T*, Scal*, bool);
 
 PPS_INST_T(double)
 PPS_INST_T(float)

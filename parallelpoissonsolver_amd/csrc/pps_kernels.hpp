@@ -191,9 +191,12 @@ void launch_cast(const Geo& gbox, hipStream_t s, const TS* src, TD* dst, const i
 template <typename T>
 void launch_fcg_gamma(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* Apold, Scal* S);
 template <typename T>
-void launch_fcg_p(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* pold, T* p, T* Ap, const T* r, Scal* S);
+void launch_fcg_p(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* pold, T* p, T* Ap, const T* r, T* x,
+                  Scal* S);
 template <typename T>
-void launch_fcg_update(const Geo& g, hipStream_t s, RedBuf rb, T* x, const T* p, T* r, const T* Ap, Scal* S, bool history);
+void launch_fcg_final_x(const Geo& g, hipStream_t s, T* x, const T* p, const Scal* S);
+template <typename T>
+void launch_fcg_update(const Geo& g, hipStream_t s, RedBuf rb, T* r, const T* Ap, Scal* S, bool history);
 
 // ---- mixed-precision iterative refinement ----
 // r = b - A x (FP64), r32 = (float) r; So: S_ERR = ||r||, history[outer];
