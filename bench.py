@@ -86,11 +86,15 @@ def run_form(C, chain, steps, warmup, name):
         log(f"{name}: step {s + 1}/{steps}: {res['iterations']} it (outer {res['outer_iterations']}), "
             f"true rel residual {res['error_compute_operator']:.3e}, fallback {res['fallback_switches']}, "
             f"{wall[-1]:.3f} s")
-        # plain CG stops on the recursive residual like the reference
-        # (baseCG.hpp:224), so its true residual may sit a few per cent above
-        # the tolerance; a form that is off by more than that has failed
-        if not res["error_compute_operator"] < 1.5 * TOL:
-            raise RuntimeError(f"{name} did not reach {TOL}: {res['error_compute_operator']}")
+        # refinement forms stop on the true FP64 residual and must reach TOL.
+        # The others stop on the recursive residual like the reference
+        # (baseCG.hpp:224); its drift from the true residual grows with the
+        # iteration count (about 1.3x at 512^3, 2x at 768^3 for plain CG), so
+        # they are held to 3 TOL. Stopping there does less work than reaching
+        # a true 1e-10, which favours those forms in the comparison.
+        bar = TOL if name.startswith("ir_") else 3 * TOL
+        if not res["error_compute_operator"] < bar:
+            raise RuntimeError(f"{name} did not reach {bar}: {res['error_compute_operator']}")
     return {"s": sum(wall) / len(wall), "device_s": sum(dev) / len(dev), "iterations": res["iterations"],
             "outer_iterations": res["outer_iterations"], "true_rel_residual": res["error_compute_operator"],
             "fallback_switches": res["fallback_switches"]}
@@ -207,6 +211,8 @@ def main():
     ap.add_argument("--forms", default="ir_nnfcg,nn_fcg_fp64,plain_fp64_cg")
     ap.add_argument("--reps", type=int, default=20, help="launches per kernel timing")
     ap.add_argument("--kernels-only", action="store_true", help="only the per-kernel timings")
+    ap.add_argument("--rhs", default="gauss", choices=("gauss", "multiscale"),
+                    help="right-hand side: 64 Gaussian sources (the metric's), or those plus a k^-2 random spectrum")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {a.gpus})", file=sys.stderr)
@@ -220,8 +226,11 @@ def main():
     C.fill(L.FIELD_X, 0.0)
     C.fill(L.FIELD_B, 0.0)
     C.rhs_gauss(SEED, NSRC, 0.01, 0.1)
+    if a.rhs == "multiscale":
+        C.rhs_spectrum(SEED, kmax=16, accumulate=True)
     C.copy_field(L.FIELD_T3, L.FIELD_B)
-    log(f"workload: {a.n}^3 interior, Gaussian sources (seed {SEED}, {NSRC} sources), tol {TOL}")
+    log(f"workload: {a.n}^3 interior, Gaussian sources (seed {SEED}, {NSRC} sources)"
+        f"{' plus a k^-2 spectrum' if a.rhs == 'multiscale' else ''}, tol {TOL}")
     forms = a.forms.split(",")
     needs_nn = a.kernels_only or any(f in ("ir_nnfcg", "nn_fcg_fp64") for f in forms)
     net, net_info = network() if needs_nn else (None, None)
@@ -265,7 +274,7 @@ def main():
         roof["mfma_tflops"] = round(k["tflops"], 2)
 
     cpu = None
-    if not a.no_cpu and a.n == N_INTERIOR:
+    if not a.no_cpu and a.n == N_INTERIOR and a.rhs == "gauss":
         try:
             ref_it = results["plain_fp64_cg"]["iterations"] if "plain_fp64_cg" in results else 2406
             cpu = cpu_baseline(ref_it)
@@ -287,7 +296,9 @@ def main():
         "vs_baseline": None,
         "dtype": {"ir_nnfcg": "f32+f64+bf16", "nn_fcg_fp64": "f64+bf16", "ir_fp32cg_fp64": "f32+f64"}.get(headline, "f64"),
         "data": "synthetic",
-        "config": {"workload": f"poisson3d_{a.n}^3_dirichlet_gauss{NSRC}", "n_interior": a.n, "rhs_seed": SEED,
+        "config": {"workload": f"poisson3d_{a.n}^3_dirichlet_" + (f"gauss{NSRC}" if a.rhs == "gauss" else
+                                                                  f"gauss{NSRC}_spectrum16"),
+                   "n_interior": a.n, "rhs_seed": SEED,
                    "solver": headline, "tol": TOL},
         "forms": results,
         "ratio_headline_vs_plain": (round(hv["s"] / results["plain_fp64_cg"]["s"], 4)
