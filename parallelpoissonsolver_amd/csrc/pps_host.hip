@@ -58,6 +58,14 @@ struct Stage {
     virtual bool host_driven() const { return false; }
     // operator()(out, in) as a preconditioner; kernels gated on *gate
     virtual int apply(Field out, Field in, const int* gate) = 0;
+    // apply() that may also sum out . dot into the context's reduction
+    // partials: *nblocks = their count, or 0 when the stage does not
+    virtual int apply_dot(Field out, Field in, const int* gate, Field dot, int* nblocks)
+    {
+        (void)dot;
+        *nblocks = 0;
+        return apply(out, in, gate);
+    }
     // operator()(x, b) as the main solver on the context's X and B
     virtual int solve_main() { return PPS_ERR_UNSUPPORTED; }
 };
@@ -436,6 +444,12 @@ struct KrylovStage : Stage {
         }
         return pre->apply(out, in, gate);
     }
+    int pre_apply_dot(Field out, Field in, const int* gate, Field dot, int* nblocks)
+    {
+        *nblocks = 0;
+        if (!pre) return pre_apply(out, in, gate);
+        return pre->apply_dot(out, in, gate, dot, nblocks);
+    }
     // common prologue of the main/inner loop: returns 1 if the loop must not run
     int prologue(bool main, Field x, Field b, Field r, int* early)
     {
@@ -645,9 +659,12 @@ struct FCGStage : KrylovStage {
         Field p[2] = {p0, p1}, Ap[2] = {A0, A1};
         int cur = 0;  // p[cur] = p_new, p[cur^1] = p_old
         rc = run_loop(C, S, cfg.max_iter, pre_host(), [&]() -> int {
-            int e = pre_apply(z, r, &S->done);
+            // the learned preconditioner sums z . Ap_old in its output kernel
+            int nb = 0;
+            int e = pre_apply_dot(z, r, &S->done, Ap[cur ^ 1], &nb);
             if (e) return e;
-            launch_fcg_gamma<double>(g, C.stream, C.rb, z.d(), Ap[cur ^ 1].d(), S);
+            if (nb > 0) launch_fcg_gamma_finish<double>(C.stream, C.rb, nb, S);
+            launch_fcg_gamma<double>(g, C.stream, C.rb, z.d(), Ap[cur ^ 1].d(), S, nb > 0);
             launch_fcg_p<double>(g, C.stream, C.rb, z.d(), p[cur ^ 1].d(), p[cur].d(), Ap[cur].d(), r.d(), x.d(), S);
             launch_fcg_update<double>(g, C.stream, C.rb, r.d(), Ap[cur].d(), S, main);
             cur ^= 1;
@@ -712,6 +729,14 @@ struct NNStage : Stage {
     using Stage::Stage;
     int apply(Field out, Field in, const int* gate) override
     {
+        int nb;
+        return apply_dot(out, in, gate, Field{}, &nb);
+    }
+    // dot.p set: the output kernel also writes per-block partials of out . dot
+    // to the context's reduction workspace (gated like the network itself)
+    int apply_dot(Field out, Field in, const int* gate, Field dot, int* nblocks) override
+    {
+        *nblocks = 0;
         if (!C.nn) return C.fail(PPS_ERR_STATE, "PPS_NN stage without a network (pps_nn_attach)");
         if (!gate) return C.fail(PPS_ERR_STATE, "PPS_NN stage called outside a Krylov stage");
         if (out.prec != in.prec) return C.fail(PPS_ERR_ARG, "PPS_NN: input and output precision differ");
@@ -726,8 +751,9 @@ struct NNStage : Stage {
         g.n0 = C.solver.hi[0] - C.solver.lo[0];
         g.n1 = C.solver.hi[1] - C.solver.lo[1];
         g.n2 = C.solver.hi[2] - C.solver.lo[2];
-        if (C.nn->apply(C.stream, g, out.p, out.prec, in.p, in.prec, &caller->done, &caller->fallback,
-                        &caller->v[S_RR]))
+        if (dot.p && dot.prec != out.prec) return C.fail(PPS_ERR_ARG, "PPS_NN: dot field precision differs");
+        if (C.nn->apply_dot(C.stream, g, out.p, out.prec, in.p, in.prec, &caller->done, &caller->fallback,
+                            &caller->v[S_RR], dot.p, C.rb.partials, C.rb.cap, nblocks))
             return C.fail(PPS_ERR_HIP, "learned preconditioner: %s", C.nn->error());
         const double* ds = C.grid.ds;
         const double jac = 1.0 / (-2.0 * (1.0 / (ds[0] * ds[0]) + 1.0 / (ds[1] * ds[1]) + 1.0 / (ds[2] * ds[2])));
@@ -780,6 +806,12 @@ struct IRStage : KrylovStage {
             launch_pointwise<float>(C.gbox(32), C.stream, r32.f(), z.f(), 1.0, gate);
             return PPS_OK;
         };
+        // the learned preconditioner sums z . Ap_old in its output kernel
+        auto applyM_dot = [&](const int* gate, Field dot, int* nb) -> int {
+            *nb = 0;
+            if (M) return M->apply_dot(z, r32, gate, dot, nb);
+            return applyM(gate);
+        };
         if (!flexible && !pw) {
             rc = applyM(&SI->done);
             if (rc) return rc;
@@ -787,9 +819,11 @@ struct IRStage : KrylovStage {
         }
         rc = run_loop(C, SI, in->cfg.max_iter, host_each, [&]() -> int {
             if (flexible) {
-                int e = applyM(&SI->done);
+                int nb = 0;
+                int e = applyM_dot(&SI->done, Ap[cur ^ 1], &nb);
                 if (e) return e;
-                launch_fcg_gamma<float>(g, C.stream, C.rb, z.f(), Ap[cur ^ 1].f(), SI);
+                if (nb > 0) launch_fcg_gamma_finish<float>(C.stream, C.rb, nb, SI);
+                launch_fcg_gamma<float>(g, C.stream, C.rb, z.f(), Ap[cur ^ 1].f(), SI, nb > 0);
                 launch_fcg_p<float>(g, C.stream, C.rb, z.f(), p[cur ^ 1].f(), p[cur].f(), Ap[cur].f(), r32.f(), d.f(), SI);
                 launch_fcg_update<float>(g, C.stream, C.rb, r32.f(), Ap[cur].f(), SI, false);
                 cur ^= 1;

@@ -1221,11 +1221,21 @@ void launch_cheb_sweep(const Geo& g, hipStream_t s, const T* Y, const T* B, cons
 
 // ------------------------------------------------------------------ FCG
 
+// gamma = z.Apold and beta.  When the preconditioner sums z.Apold itself (the
+// learned one, from its output kernel), only the finish runs, gated off once
+// the fallback preconditioner has taken over; the pass over z and Apold then
+// runs only while the fallback is on.
 template <typename T>
 struct OpFcgGamma : Gate {
     const T* __restrict__ z;
     const T* __restrict__ Apold;
     Scal* S;
+    const int* skip_if = nullptr;  // gated while *skip_if != 0
+    const int* run_if = nullptr;   // gated while *run_if == 0
+    __device__ __forceinline__ bool gated() const
+    {
+        return Gate::gated() || (skip_if && *skip_if) || (run_if && !*run_if);
+    }
     struct R { T z, a; };
     __device__ void load() {}
     __device__ R fetch(long long c) const { return {z[c], Apold[c]}; }
@@ -1239,14 +1249,28 @@ struct OpFcgGamma : Gate {
 };
 
 template <typename T>
-void launch_fcg_gamma(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* Apold, Scal* S)
+void launch_fcg_gamma(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* Apold, Scal* S, bool fallback_only)
 {
     OpFcgGamma<T> op;
     op.gate = &S->done;
     op.z = z;
     op.Apold = Apold;
     op.S = S;
+    if (fallback_only) op.run_if = &S->fallback;
     run_vector<1>(g, s, op, rb);
+}
+
+template <typename T>
+void launch_fcg_gamma_finish(hipStream_t s, RedBuf rb, int nblocks, Scal* S)
+{
+    OpFcgGamma<T> op;
+    op.gate = &S->done;
+    op.z = nullptr;
+    op.Apold = nullptr;
+    op.S = S;
+    op.skip_if = &S->fallback;
+    if (nblocks <= 0 || nblocks > rb.cap) return;
+    hipLaunchKernelGGL((k_finish<1, OpFcgGamma<T>>), dim3(1), dim3(kFinThreads), 0, s, op, rb, nblocks);
 }
 
 // p = z - beta pold ; Ap = A p ; p.Ap, p.r ; alpha = p.r / p.Ap ; and the
@@ -1547,7 +1571,8 @@ void launch_ir_inner_init(const Geo& g, hipStream_t s, RedBuf rb, float* r, floa
                                      const T*, const T*, Scal*, bool);                                         \
     template void launch_bicg_k6<T>(const Geo&, hipStream_t, T*, const T*, const T*, const Scal*);             \
     template void launch_cheb_first<T>(const Geo&, hipStream_t, const T*, T*, T*, double, double, const int*); \
-    template void launch_fcg_gamma<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, Scal*);             \
+    template void launch_fcg_gamma<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, Scal*, bool);       \
+    template void launch_fcg_gamma_finish<T>(hipStream_t, RedBuf, int, Scal*);                                  \
     template void launch_fcg_p<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, T*, T*, const T*, T*, Scal*); \
     template void launch_fcg_final_x<T>(const Geo&, hipStream_t, T*, const T*, const Scal*);                      \
     template void launch_fcg_update<T>(const Geo&, hipStream_t, RedBuf, T*, const T*, Scal*, bool);

@@ -189,7 +189,11 @@ void launch_cast(const Geo& gbox, hipStream_t s, const TS* src, TD* dst, const i
 
 // ---- FCG (truncation 1) ----
 template <typename T>
-void launch_fcg_gamma(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* Apold, Scal* S);
+void launch_fcg_gamma(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* Apold, Scal* S, bool fallback_only);
+// finish of gamma from per-block partials of z.Apold already in rb (written by
+// the preconditioner); gated off while S->fallback is set
+template <typename T>
+void launch_fcg_gamma_finish(hipStream_t s, RedBuf rb, int nblocks, Scal* S);
 template <typename T>
 void launch_fcg_p(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* pold, T* p, T* Ap, const T* r, T* x,
                   Scal* S);

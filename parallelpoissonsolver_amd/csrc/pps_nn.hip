@@ -763,7 +763,9 @@ void launch_lift(const ConvArgs& a, hipStream_t s)
 // then every thread adds the 9 partials of its output voxel (rows y-1 .. y+1)
 // into three running output planes; plane s completes output plane s-1.
 // The output columns are 1 .. 14 of each row, so a shift never reaches past
-// the row for a value that is used.  The unfused path rounds a + e to BF16
+// the row for a value that is used.  With a.dot set the kernel also sums
+// z . dot over its outputs (FP64) and writes one partial per block to a.dpart
+// (the Krylov step's gamma = z . Ap_old without another pass over z).  The unfused path rounds a + e to BF16
 // before its MFMA; here the sum is exact, so outputs differ from it by that
 // rounding (relative ~1e-3 at most) and FP32 order.
 constexpr int kFW = 14, kFH = 18;                 // output tile
@@ -901,6 +903,12 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     const bool mine = tid < kFW * kFH && x < W && y < H;
     const int q0 = oy * kXW + ox + 1;  // X region voxel of the output's (x, y - 1) neighbour
     float am1 = 0.0f, a0 = 0.0f, ap1 = 0.0f;
+    // dot operand of the next output plane, raw (loaded from a clamped voxel
+    // one plane ahead, masked when used)
+    const bool d64 = a.fout.is64;
+    const long long dxy = fidx(a.fout, 0, min(x, W - 1), min(y, H - 1), 0);
+    u32x2 dnr = {0u, 0u};
+    double dacc = 0.0;
 
     // planes s = z0-1 .. z1 of X; the ring holds input planes s-1 .. s+1
     fs.fetch(preA, z0 - 2, D);
@@ -938,6 +946,12 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
             const float v = am1 + bout;
             if (a.fout.is64) ((double*)a.fout.p)[o] = (double)v;
             else ((float*)a.fout.p)[o] = v;
+            if (a.dot) dacc += (double)v * (d64 ? __builtin_bit_cast(double, dnr) : (double)__uint_as_float(dnr[0]));
+        }
+        if (a.dot && s >= z0 && s < z1) {  // uniform
+            const long long o = dxy + (long long)s * a.fout.sz;
+            if (d64) dnr = ((const u32x2*)a.dot)[o];
+            else dnr[0] = ((const unsigned*)a.dot)[o];
         }
         am1 = a0;
         a0 = ap1;
@@ -947,15 +961,39 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
         step(s, preA);
         if (s + 1 <= z1) step(s + 1, preB);
     }
+    if (a.dot) {
+        __shared__ double red[4];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) dacc += __shfl_xor(dacc, o, 64);
+        if (lane == 0) red[wave] = dacc;
+        __syncthreads();
+        if (tid == 0)
+            a.dpart[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = (red[0] + red[1]) + (red[2] + red[3]);
+    }
+}
+
+// grid of the fused output kernel: 14 x 18 tiles, z-chunks so that the launch
+// has at least 8192 blocks (chunks of 16 planes or more)
+static dim3 outf_grid(int D, int H, int W, int* zc)
+{
+    const int tiles = ((W + kFW - 1) / kFW) * ((H + kFH - 1) / kFH);
+    int c = D;
+    while (c > 16 && (long long)tiles * ((D + c - 1) / c) < 8192) c = (c + 1) / 2;
+    *zc = c;
+    return dim3((unsigned)((W + kFW - 1) / kFW), (unsigned)((H + kFH - 1) / kFH), (unsigned)((D + c - 1) / c));
+}
+
+static long long outf_blocks(int D, int H, int W)
+{
+    int zc;
+    const dim3 g = outf_grid(D, H, W, &zc);
+    return (long long)g.x * g.y * g.z;
 }
 
 void launch_outf(const ConvArgs& a, hipStream_t s)
 {
-    const int D = a.out.D, H = a.out.H, W = a.out.W;
-    const int tiles = ((W + kFW - 1) / kFW) * ((H + kFH - 1) / kFH);
-    int zc = D;
-    while (zc > 16 && (long long)tiles * ((D + zc - 1) / zc) < 8192) zc = (zc + 1) / 2;
-    const dim3 grid((unsigned)((W + kFW - 1) / kFW), (unsigned)((H + kFH - 1) / kFH), (unsigned)((D + zc - 1) / zc));
+    int zc;
+    const dim3 grid = outf_grid(a.out.D, a.out.H, a.out.W, &zc);
     if (a.fin.is64) hipLaunchKernelGGL(k_outf<true>, grid, dim3(256), 0, s, a, zc);
     else hipLaunchKernelGGL(k_outf<false>, grid, dim3(256), 0, s, a, zc);
 }
@@ -1575,9 +1613,17 @@ void Net::launch_layer(const ConvArgs& a, int i)
     else launch_conv(a, stream_);
 }
 
-void Net::forward(BufSet& bs, const FieldRef& fin, const FieldRef& fout, const int* gate, const int* gate2)
+void Net::forward(BufSet& bs, const FieldRef& fin, const FieldRef& fout, const int* gate, const int* gate2,
+                  const void* dot, double* dpart)
 {
-    for (int i = 0; i < kLayers; ++i) launch_layer(layer_args(bs, i, fin, fout, gate, gate2), i);
+    for (int i = 0; i < kLayers; ++i) {
+        ConvArgs a = layer_args(bs, i, fin, fout, gate, gate2);
+        if (a.fused && dot) {
+            a.dot = dot;
+            a.dpart = dpart;
+        }
+        launch_layer(a, i);
+    }
 }
 
 void Net::backward(BufSet& bs, const FieldRef& fin)
@@ -1679,6 +1725,15 @@ void Net::field_refs(const NNFieldGeom& g, void* out, int out_prec, const void* 
 int Net::apply(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
                const int* gate, const int* gate2, const double* rr)
 {
+    int nb;
+    return apply_dot(s, g, out, out_prec, in, in_prec, gate, gate2, rr, nullptr, nullptr, 0, &nb);
+}
+
+int Net::apply_dot(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
+                   const int* gate, const int* gate2, const double* rr, const void* dot, double* dpart, int dcap,
+                   int* nblocks)
+{
+    *nblocks = 0;
     const int n[3] = {g.n0, g.n1, g.n2};
     int rc = ensure(inf_, 1, n, false);
     if (rc) return rc;
@@ -1686,7 +1741,12 @@ int Net::apply(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, con
     stream_ = s;
     FieldRef fi, fo;
     field_refs(g, out, out_prec, in, in_prec, rr, fi, fo);
-    forward(inf_, fi, fo, gate, gate2);
+    // the output layer runs fused at inference (one sample): it can sum out . dot
+    const ConvArgs last = layer_args(inf_, kLayers - 1, fi, fo, gate, gate2);
+    const long long nb = outf_blocks(last.out.D, last.out.H, last.out.W);
+    const bool with_dot = dot && dpart && last.fused && nb <= dcap;
+    forward(inf_, fi, fo, gate, gate2, with_dot ? dot : nullptr, with_dot ? dpart : nullptr);
+    if (with_dot) *nblocks = (int)nb;
     stream_ = own;
     return hip(hipGetLastError(), "network launch");
 }

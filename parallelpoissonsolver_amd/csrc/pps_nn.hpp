@@ -41,6 +41,12 @@ struct NNPrecond {
     // *gate != 0 or *gate2 != 0.
     virtual int apply(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
                       const int* gate, const int* gate2, const double* rr) = 0;
+    // apply() that also sums out . dot over the region (dot: a field of out's
+    // precision and layout): per-block FP64 partials go to dpart[0 .. *nblocks)
+    // when they fit in dcap; otherwise *nblocks = 0 and nothing is summed
+    virtual int apply_dot(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
+                          const int* gate, const int* gate2, const double* rr, const void* dot, double* dpart, int dcap,
+                          int* nblocks) = 0;
     virtual const char* error() const = 0;
 };
 
@@ -94,6 +100,8 @@ struct ConvArgs {
     int fused;               // output layer with the lift recomputed (inference)
     const uint16_t* wl2;     // fused: lift operand
     const float* bias2;      // fused: lift bias
+    const void* dot;         // fused: field of fout's precision and layout; z . dot per block into dpart
+    double* dpart;
 };
 
 struct BufSet {
@@ -131,6 +139,9 @@ public:
     int init(unsigned long long seed);  // own stream, He-initialised weights
     int apply(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
               const int* gate, const int* gate2, const double* rr) override;
+    int apply_dot(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
+                  const int* gate, const int* gate2, const double* rr, const void* dot, double* dpart, int dcap,
+                  int* nblocks) override;
     const char* error() const override { return err_.c_str(); }
     // average device time of `reps` launches of layer `layer` of the inference
     // pass over the region (run apply() on the same region first); algorithmic
@@ -158,7 +169,8 @@ public:
 private:
     int ensure(BufSet& bs, int B, const int n[3], bool train);
     void release(BufSet& bs);
-    void forward(BufSet& bs, const FieldRef& fin, const FieldRef& fout, const int* gate, const int* gate2);
+    void forward(BufSet& bs, const FieldRef& fin, const FieldRef& fout, const int* gate, const int* gate2,
+                 const void* dot = nullptr, double* dpart = nullptr);
     ConvArgs layer_args(const BufSet& bs, int i, const FieldRef& fin, const FieldRef& fout, const int* gate,
                         const int* gate2) const;
     void launch_layer(const ConvArgs& a, int i);
