@@ -71,7 +71,15 @@ struct DwArgs {
 
 constexpr int kTW = 16, kTHt = 16;          // output tile per plane
 constexpr int kLW = kTW + 2, kLH = kTHt + 2;  // staged input plane with halo
-constexpr int kPlane = kLH * 2 * kLW;        // 16-byte chunks per staged plane: [y][chunk][x]
+// Staged plane of 16-byte chunks (8 channels): [channel half][y][x], the
+// second half at kHalf.  A 16-byte LDS read is served 16 lanes per cycle, lanes
+// 0-7 with 24-31 and 8-15 with 16-23 (and likewise in the upper half-wave); it
+// is free of bank conflicts when those 16 lanes hit 16 different chunk
+// addresses mod 16.  The two channel halves of one operand are read by lane
+// blocks j and j+1, so kHalf is a multiple of 16 chunks.
+constexpr int kHalf = (kLH * kLW + 15) / 16 * 16;
+constexpr int kPlane = 2 * kHalf;
+static_assert(kHalf % 16 == 0, "channel halves 16 chunks apart mod 16");
 constexpr int kZPlane = kTHt * 2 * kTW;      // chunks of one staged gradient plane
 
 // ------------------------------------------------------------ BF16 helpers
@@ -315,7 +323,7 @@ __device__ __forceinline__ void store_out(const ConvArgs& a, const f32x4& acc, c
     }
 }
 
-constexpr int kChunksPerThread = (kPlane + 255) / 256;
+constexpr int kChunksPerThread = (2 * kLH * kLW + 255) / 256;  // staged chunks per thread and plane
 
 // PO: also write the 2x2x2-pooled copy of the (RELU) output to a.pout
 template <int MI, int MO, bool PO = false>
@@ -349,7 +357,9 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
     // slot (the dummy slot for threads without one) and whether the voxel is
     // inside the level in x and y (else the chunk is zero padding).  The loads
     // run without bounds branches, so they stay in flight until the plane is
-    // written.
+    // written.  Consecutive threads load the two halves of one voxel record,
+    // so a wavefront's loads are contiguous.
+    constexpr int kStaged = 2 * kLH * kLW;  // chunks staged per plane
     int sx[kChunksPerThread], sy[kChunksPerThread], sc[kChunksPerThread], sl[kChunksPerThread];
     unsigned sval = 0u;
 #pragma unroll
@@ -361,8 +371,8 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
         sx[k] = min(max(xs, 0), W - 1);
         sy[k] = min(max(ys, 0), H - 1);
         sc[k] = rem & 1;
-        sl[k] = t < kPlane ? (yy * 2 + (rem & 1)) * kLW + (rem >> 1) : 4 * kPlane;
-        if (t < kPlane && xs >= 0 && ys >= 0 && xs < W && ys < H && (MI != IN_FIELD || sc[k] == 0)) sval |= 1u << k;
+        sl[k] = t < kStaged ? (rem & 1) * kHalf + yy * kLW + (rem >> 1) : 4 * kPlane;
+        if (t < kStaged && xs >= 0 && ys >= 0 && xs < W && ys < H && (MI != IN_FIELD || sc[k] == 0)) sval |= 1u << k;
     }
     // planes in flight: preA / preB hold raw loads issued one or two steps earlier
     Raw<MI> preA[kChunksPerThread], preB[kChunksPerThread];
@@ -411,7 +421,7 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
             // member geometry (the empty member of pair 4 reuses member 0's, its weights are zero)
             const int c = pair_combo(p, m) < 9 ? pair_combo(p, m) : pair_combo(p, 0);
             const int dz = c / 3, dx = c % 3;
-            pbase[p] = ((z + dz - 1) & 3) * kPlane + (j & 1) * kLW + n + dx + wave * 4 * 2 * kLW;
+            pbase[p] = ((z + dz - 1) & 3) * kPlane + (j & 1) * kHalf + n + dx + wave * 4 * kLW;
         }
         // two steps ahead where the registers allow (the pooled and up+skip
         // input stages hold more raw staging data; with the one-channel field
@@ -420,11 +430,11 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
         constexpr int kSteps = 30, kPf = !kPipe ? 0 : (MI == IN_POOL || MI == IN_UPADD) ? 1 : 2;
         u32x4 bq[kPf + 1];
 #pragma unroll
-        for (int q = 0; q < kPf; ++q) bq[q] = lds[pbase[q / 6] + (q % 6) * 2 * kLW];
+        for (int q = 0; q < kPf; ++q) bq[q] = lds[pbase[q / 6] + (q % 6) * kLW];
 #pragma unroll
         for (int q = 0; q < kSteps; ++q) {
             if (q + kPf < kSteps)
-                bq[(q + kPf) % (kPf + 1)] = lds[pbase[(q + kPf) / 6] + ((q + kPf) % 6) * 2 * kLW];
+                bq[(q + kPf) % (kPf + 1)] = lds[pbase[(q + kPf) / 6] + ((q + kPf) % 6) * kLW];
             const bf16x8 B = __builtin_bit_cast(bf16x8, bq[q % (kPf + 1)]);
             const int p = q / 6, t = q % 6;
 #pragma unroll
@@ -1006,10 +1016,13 @@ void launch_outf(const ConvArgs& a, hipStream_t s)
 // wavefront 3's spare slot sums the bias gradient.  Each block writes its
 // partial sums; a second kernel adds the partials in a fixed order.
 
+// staged input plane of the weight-gradient kernel: [y][channel half][x]
+constexpr int kPlaneDw = kLH * 2 * kLW;
+
 template <int MI, int MZ>
 __global__ __launch_bounds__(256) void k_dw(DwArgs a, int zc)
 {
-    __shared__ u32x4 lx[4 * kPlane];
+    __shared__ u32x4 lx[4 * kPlaneDw];
     __shared__ u32x4 lz[2 * kZPlane];
     const int D = a.D, H = a.H, W = a.W;
     const int nzc = (D + zc - 1) / zc;
@@ -1023,8 +1036,8 @@ __global__ __launch_bounds__(256) void k_dw(DwArgs a, int zc)
     if constexpr (MI == IN_FIELD) fscale = field_scale(a.fin);
 
     auto stage_x = [&](int zi) {
-        u32x4* dst = lx + (zi & 3) * kPlane;
-        for (int t = tid; t < kPlane; t += 256) {
+        u32x4* dst = lx + (zi & 3) * kPlaneDw;
+        for (int t = tid; t < kPlaneDw; t += 256) {
             const int yy = t / (2 * kLW);
             const int rem = t - yy * (2 * kLW);
             const int xx = rem >> 1, ch = rem & 1;
@@ -1073,7 +1086,7 @@ __global__ __launch_bounds__(256) void k_dw(DwArgs a, int zc)
                 const int tap = 7 * wave + t;
                 const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
                 const uint16_t* xp =
-                    X + ((z + dz - 1) & 3) * kPlane * 8 + (((row + dy) * 2 + (n >> 3)) * kLW + xb + dx) * 8 + (n & 7);
+                    X + ((z + dz - 1) & 3) * kPlaneDw * 8 + (((row + dy) * 2 + (n >> 3)) * kLW + xb + dx) * 8 + (n & 7);
                 u32x4 aw;
 #pragma unroll
                 for (int e = 0; e < 4; ++e) aw[e] = (unsigned)xp[16 * e] | ((unsigned)xp[16 * e + 8] << 16);
