@@ -531,48 +531,68 @@ __device__ __forceinline__ unsigned relu_bf2(unsigned h)
 }
 __device__ __forceinline__ int mod3(int v) { return ((v % 3) + 3) % 3; }
 
+// Raw buffer access.  A buffer load whose offset lies at or beyond the
+// resource's num_records returns 0 and such a store is dropped, so voxels
+// outside the domain (zero padding, lanes without an output) need no branch
+// and no mask: their offset is kOOB, and a plane outside the domain gets
+// num_records = 0.
+constexpr unsigned kOOB = 0x80000000u;     // beyond every num_records used below
+constexpr int kInRange = 0x7fffffff;        // num_records of a resource that is in range
+// p and nrec must be uniform; they are read from the first lane so that the
+// resource lives in scalar registers (a per-lane resource costs a loop per access)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int nrec)
+{
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    void* q = (void*)(((unsigned long long)hi << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(q, (short)0, __builtin_amdgcn_readfirstlane(nrec), 0x00020000);
+}
+
 // Staging of the one-channel network input (an FP64 or FP32 field) into a
-// BF16 LDS plane: a thread owns kPer slots of the staged region.  Loads come
-// from clamped in-domain addresses, so no divergent control flow surrounds
-// them and they stay in flight until the plane is converted; the domain mask
-// (zero padding) is applied at the conversion.
+// BF16 LDS plane: a thread owns kPer slots of the staged region.  Slots
+// outside the domain (and beyond the region) load 0 from an out-of-range
+// buffer offset; the loads have no control flow around them and stay in
+// flight until the plane is converted.
 template <bool F64, int kPer>
 struct FieldStage {
     using Raw = std::conditional_t<F64, u32x2, unsigned>;
-    const Raw* p;
-    long long off, sz;
-    int soff[kPer];   // clamped in-plane element offset of slot k
-    unsigned valid;   // bit k: slot k exists and lies inside the domain in x and y
+    const char* base;  // element (0, 0, 0) of sample b
+    long long szb;     // plane stride in bytes
+    unsigned soff[kPer];  // byte offset of slot k inside its plane, or kOOB
     float scale;
     // region origin (rx, ry), row width rw, slots per plane np
     __device__ void init(const FieldRef& f, int b, int rx, int ry, int rw, int np, int W, int H, int tid)
     {
-        p = (const Raw*)f.p;
-        off = f.off + b * f.sb;
-        sz = f.sz;
+        constexpr int esz = F64 ? 8 : 4;
+        base = (const char*)f.p + (f.off + b * f.sb) * esz;
+        szb = f.sz * esz;
         scale = field_scale(f);
-        valid = 0u;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int t = tid + 256 * k;
             const int yy = ry + t / rw, xx = rx + t % rw;
-            if (t < np && xx >= 0 && yy >= 0 && xx < W && yy < H) valid |= 1u << k;
-            soff[k] = (int)(min(max(xx, 0), W - 1) * f.sx + min(max(yy, 0), H - 1) * f.sy);
+            const bool in = t < np && xx >= 0 && yy >= 0 && xx < W && yy < H;
+            soff[k] = in ? (unsigned)((xx * f.sx + yy * f.sy) * esz) : kOOB;
         }
     }
     __device__ void fetch(Raw* pre, int zi, int D) const
     {
-        const long long zo = off + (long long)min(max(zi, 0), D - 1) * sz;
+        const bool zin = zi >= 0 && zi < D;  // uniform
+        const __amdgpu_buffer_rsrc_t r = buf_rsrc(base + (zin ? zi : 0) * szb, zin ? kInRange : 0);
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) pre[k] = p[zo + soff[k]];
+        for (int k = 0; k < kPer; ++k) {
+            if constexpr (F64) pre[k] = __builtin_amdgcn_raw_buffer_load_b64(r, soff[k], 0, 0);
+            else pre[k] = __builtin_amdgcn_raw_buffer_load_b32(r, soff[k], 0, 0);
+        }
     }
-    // BF16 bits of slot k of plane zi
-    __device__ uint16_t bf(const Raw* pre, int k, int zi, int D) const
+    // BF16 bits of slot k (0 outside the domain: the load returned 0)
+    __device__ uint16_t bf(const Raw* pre, int k) const
     {
         float v;
         if constexpr (F64) v = (float)(__builtin_bit_cast(double, pre[k]) * (double)scale);
         else v = __uint_as_float(pre[k]) * scale;
-        return (zi >= 0 && zi < D && ((valid >> k) & 1u)) ? (uint16_t)f2bf(v) : (uint16_t)0;
+        return (uint16_t)f2bf(v);
     }
 };
 
@@ -629,7 +649,7 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
         for (int k = 0; k < kPer; ++k) {
             const int t = tid + 256 * k;
             if (t < kP) {
-                const uint16_t h = fs.bf(pre, k, zi, D);
+                const uint16_t h = fs.bf(pre, k);
                 dst[t] = h;
                 if (twice) dst[t + 4 * kP] = h;
             }
@@ -794,7 +814,9 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
     __shared__ uint16_t lr[6 * kFRPs];  // input planes, each in two slots
-    __shared__ __align__(16) float lp[kPRows * kXQp];  // P[3 dz + dy][voxel of the X region]
+    // P[3 dz + dy][voxel of the X region]; the tail pads the gathers of the
+    // four threads without an output voxel
+    __shared__ __align__(16) float lp[kPRows * kXQp + 32];
     const int D = a.out.D, H = a.out.H, W = a.out.W;
     const TileIdx ti = tile_of_block();
     const int z0 = ti.z * zc;  // one sample
@@ -825,11 +847,15 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
             cinit[k][r] = (xin && ((yin >> k) & 1u)) ? (a.bias2 ? a.bias2[4 * j + r] : 0.0f) : -1e30f;
-    // coarse activation E1: element offset of this lane's 4 channels in row k = 0
+    // coarse activation E1: byte offset of this lane's 4 channels in row k
+    // (kOOB outside the domain: the load returns zeros)
     const int cW = a.in.W, cH = a.in.H;
-    const int cplane = cH * cW * kC;
+    const long long cplane = (long long)cH * cW * kC;
     const int cy0 = (y0 - 1 + wave) >> 1;  // coarse row of k = 0 (rows k step by 2)
-    const int ebase = (cy0 * cW + (gx >> 1)) * kC + 4 * j;
+    unsigned eoff[kFG];
+#pragma unroll
+    for (int k = 0; k < kFG; ++k)
+        eoff[k] = (xin && ((yin >> k) & 1u)) ? (unsigned)(2 * (((cy0 + 2 * k) * cW + (gx >> 1)) * kC + 4 * j)) : kOOB;
     // lift taps 8j + e of this lane: ring offset for X voxel (row wave, column n)
     // relative to the slot of plane s-1
     int toff[8];
@@ -852,24 +878,21 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
         for (int k = 0; k < kPer; ++k) {
             const int t = tid + 256 * k;
             if (t < kFRP) {
-                const uint16_t h = fs.bf(pre, k, zi, D);
+                const uint16_t h = fs.bf(pre, k);
                 dst[t] = h;
                 dst[t + 3 * kFRPs] = h;
             }
         }
     };
-    // up(E1) of this wavefront's rows, loaded one plane ahead; lanes outside
-    // the domain never load and keep the zeros they start with
+    // up(E1) of this wavefront's rows, loaded one plane ahead (zeros outside
+    // the domain)
     u32x2 ev[kFG];
 #pragma unroll
     for (int k = 0; k < kFG; ++k) ev[k] = u32x2{0u, 0u};
     auto fetch_e = [&](u32x2* e, int s) {
-        const char* base = (const char*)(a.in.p + (size_t)(s >> 1) * cplane);  // uniform
-        if (xin) {
+        const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.in.p + (s >> 1) * cplane, kInRange);
 #pragma unroll
-            for (int k = 0; k < kFG; ++k)
-                if ((yin >> k) & 1u) e[k] = *(const u32x2*)(base + (unsigned)(2 * (ebase + 2 * k * cW * kC)));
-        }
+        for (int k = 0; k < kFG; ++k) e[k] = __builtin_amdgcn_raw_buffer_load_b64(r, eoff[k], 0, 0);
     };
     // one plane of X -> tap partials, this wavefront's rows
     auto partials = [&](const u32x2* e, int s) {
@@ -907,16 +930,20 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
             if (n < kPRows) *(f32x4*)(lp + n * kXQp + (wave + 4 * k) * kXW + 4 * j) = pr;
         }
     };
-    // this thread's output voxel
+    // this thread's output voxel: byte offset inside an output plane, kOOB
+    // for threads without one (their stores are dropped, dot loads give 0)
     const int oy = tid / kFW, ox = tid - oy * kFW;
     const int x = x0 + ox, y = y0 + oy;
     const bool mine = tid < kFW * kFH && x < W && y < H;
     const int q0 = oy * kXW + ox + 1;  // X region voxel of the output's (x, y - 1) neighbour
     float am1 = 0.0f, a0 = 0.0f, ap1 = 0.0f;
-    // dot operand of the next output plane, raw (loaded from a clamped voxel
-    // one plane ahead, masked when used)
-    const bool d64 = a.fout.is64;
-    const long long dxy = fidx(a.fout, 0, min(x, W - 1), min(y, H - 1), 0);
+    const bool o64 = a.fout.is64;
+    const int oesz = o64 ? 8 : 4;
+    const unsigned ooff = mine ? (unsigned)((x * a.fout.sx + y * a.fout.sy) * oesz) : kOOB;
+    const char* obase = (const char*)a.fout.p + a.fout.off * oesz;
+    const char* dbase = a.dot ? (const char*)a.dot + a.fout.off * oesz : nullptr;
+    const long long oszb = a.fout.sz * oesz;
+    // dot operand of the next output plane, raw (loaded one plane ahead)
     u32x2 dnr = {0u, 0u};
     double dacc = 0.0;
 
@@ -927,7 +954,7 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     store_plane(preA, z0 - 1);
     fs.fetch(preA, z0, D);
     fs.fetch(preB, z0 + 1, D);
-    if (z0 >= 1) fetch_e(ev, z0 - 1);
+    if (z0 >= 1) fetch_e(ev, z0 - 1);  // uniform
     // one plane: two planes of input loads in flight, each buffer refilled once written
     auto step = [&](int s, Raw* buf) {
         store_plane(buf, s + 1);
@@ -937,7 +964,7 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
         if (sin) partials(ev, s);
         if (s + 1 <= z1 && s + 1 < D) fetch_e(ev, s + 1);
         __syncthreads();  // partials of plane s complete
-        if (sin && tid < kFW * kFH) {
+        if (sin) {  // every thread gathers; those without an output read the padding
             float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
             for (int dy = 0; dy < 3; ++dy) {
@@ -951,17 +978,22 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
             ap1 += s0;
         }
         const int zo = s - 1;
-        if (zo >= z0 && zo < z1 && mine) {
-            const long long o = fidx(a.fout, 0, x, y, zo);
+        if (zo >= z0 && zo < z1) {  // uniform
             const float v = am1 + bout;
-            if (a.fout.is64) ((double*)a.fout.p)[o] = (double)v;
-            else ((float*)a.fout.p)[o] = v;
-            if (a.dot) dacc += (double)v * (d64 ? __builtin_bit_cast(double, dnr) : (double)__uint_as_float(dnr[0]));
+            const __amdgpu_buffer_rsrc_t r = buf_rsrc(obase + zo * oszb, kInRange);
+            if (o64) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (double)v), r, ooff, 0, 0);
+            else __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, ooff, 0, 0);
+            // threads without an output hold a dot operand of 0 but may hold a
+            // non-finite v (gathered from the padding)
+            if (a.dot) {  // uniform
+                const double pr = (double)v * (o64 ? __builtin_bit_cast(double, dnr) : (double)__uint_as_float(dnr[0]));
+                dacc += mine ? pr : 0.0;
+            }
         }
         if (a.dot && s >= z0 && s < z1) {  // uniform
-            const long long o = dxy + (long long)s * a.fout.sz;
-            if (d64) dnr = ((const u32x2*)a.dot)[o];
-            else dnr[0] = ((const unsigned*)a.dot)[o];
+            const __amdgpu_buffer_rsrc_t r = buf_rsrc(dbase + s * oszb, kInRange);
+            if (o64) dnr = __builtin_amdgcn_raw_buffer_load_b64(r, ooff, 0, 0);
+            else dnr[0] = __builtin_amdgcn_raw_buffer_load_b32(r, ooff, 0, 0);
         }
         am1 = a0;
         a0 = ap1;
