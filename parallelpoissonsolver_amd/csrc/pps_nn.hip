@@ -780,11 +780,11 @@ void launch_lift(const ConvArgs& a, hipStream_t s)
 // so the 16-channel level-0 activation never goes to HBM.  A block owns a
 // 14 x 18 output tile; its X region (tile + 1-voxel halo) is 16 x 20 voxels,
 // one 16-voxel x-row per MFMA group, 5 rows per wavefront.  Per plane s and row:
-//   1. lift: one MFMA (K = 27 taps of the BF16 input ring, accumulator started
-//      at the bias, or at -1e30 outside the domain so that relu gives the zero
-//      padding); lane (voxel n, block j) then holds channels 4j .. 4j+3;
-//      relu + BF16 in packed 16-bit form.  With e = up(E1) of the same
-//      channels, (a, e) is lane (n, j)'s A operand X;
+//   1. lift: two MFMAs (K = 32 + 16 slots for the 27 taps, accumulator
+//      started at the bias, or at -1e30 outside the domain so that relu gives
+//      the zero padding); lane (voxel n, block j) then holds channels
+//      4j .. 4j+3; relu + BF16 in packed 16-bit form.  With e = up(E1) of the
+//      same channels, (a, e) is lane (n, j)'s A operand X;
 //   2. row partials P[dz,dy][n] = sum_dx sum_ci W[dz,dy,dx][ci] (a + e)[n+dx-1][ci]:
 //      X shifted one voxel left and right inside the 16-lane row (DPP), three
 //      MFMAs (one per dx) into columns m = 3 dz + dy; the skip sum a + e
@@ -802,7 +802,21 @@ constexpr int kFW = 14, kFH = 18;                 // output tile
 constexpr int kXW = kFW + 2, kXH = kFH + 2;       // X region: 16 x 20
 constexpr int kFRW = kXW + 2, kFRH = kXH + 2;     // input ring plane: 18 x 22
 constexpr int kFRP = kFRW * kFRH;
-constexpr int kFRPs = kFRP + 8;                   // ring plane stride (bank spread of the lift reads)
+constexpr int kQE = kFRP + 4;                     // quads per ring entry (+ junk reads past the last row)
+// Lift operand reads from the quad ring (z-packed: quad = input planes s-1,
+// s, s+1 and a zero at one x-y position).  Two K = 32 MFMAs o = 0, 1: the
+// operand of lane block j is the quad pair at (dy, dx) = kQuad[o][j] and
+// (dy, dx + 1), one 2 x 8-byte read.  Together they cover the 9 (dy, dx)
+// positions; the other slots carry zero weights (q_real).  With the ring row
+// of 18 quads, the pairs of blocks 0/1 and of 2/3 lie 16 quads apart (mod 32)
+// or coincide, which keeps the 8-byte reads of each half-wave on distinct LDS
+// banks.
+__constant__ signed char kQuad[2][4][2] = {{{0, 2}, {1, 0}, {1, 2}, {2, 0}}, {{0, 0}, {0, 0}, {2, 2}, {2, 2}}};
+// whether quad h (0, 1) of MFMA o's pair for block j is a tap position
+__host__ __device__ constexpr bool q_real(int o, int j, int h)
+{
+    return o == 0 ? !(h == 1 && (j == 0 || j == 2)) : (j == 0 || (j == 2 && h == 0));
+}
 constexpr int kXQ = kXW * kXH;
 constexpr int kXQp = kXQ + 4;                     // P row stride: 4 mod 64 banks
 constexpr int kPRows = 9;                         // (dz, dy) row partials
@@ -813,7 +827,7 @@ template <bool F64>
 __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
-    __shared__ uint16_t lr[6 * kFRPs];  // input planes, each in two slots
+    __shared__ __align__(16) u32x2 lq[3 * kQE];  // input quad ring: 3 entries
     // P[3 dz + dy][voxel of the X region]; the tail pads the gathers of the
     // four threads without an output voxel
     __shared__ __align__(16) float lp[kPRows * kXQp + 32];
@@ -824,7 +838,9 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     const int x0 = ti.x * kFW, y0 = ti.y * kFH;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane >> 4, n = lane & 15;
-    const bf16x8 wlift = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl2)[lane]);
+    // lift operands of the two quad-pair MFMAs
+    const bf16x8 wqa = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl2)[lane]);
+    const bf16x8 wqb = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl2)[64 + lane]);
     bf16x8 wp[3];  // per dx: column m = 3 dz + dy
 #pragma unroll
     for (int dx = 0; dx < 3; ++dx) wp[dx] = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[64 * dx + lane]);
@@ -856,31 +872,33 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
 #pragma unroll
     for (int k = 0; k < kFG; ++k)
         eoff[k] = (xin && ((yin >> k) & 1u)) ? (unsigned)(2 * (((cy0 + 2 * k) * cW + (gx >> 1)) * kC + 4 * j)) : kOOB;
-    // lift taps 8j + e of this lane: ring offset for X voxel (row wave, column n)
-    // relative to the slot of plane s-1
-    int toff[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const int tap = lift_read_tap(j, e);
-        toff[e] = (tap / 9) * kFRPs + ((tap / 3) % 3 + wave) * kFRW + tap % 3 + n;
-    }
-    // staging of the input ring: region (x0 - 2 .., y0 - 2 ..), 18 x 22.  Input
-    // plane p goes to ring slots p mod 3 and p mod 3 + 3, so that planes s-1,
-    // s, s+1 always sit in three consecutive slots.
+    // lift operand quads of this lane for X voxel (row wave, column n)
+    const int pa = (kQuad[0][j][0] + wave) * kFRW + kQuad[0][j][1] + n;
+    const int pb = (kQuad[1][j][0] + wave) * kFRW + kQuad[1][j][1] + n;
+    // input ring over the region (x0 - 2 .., y0 - 2 ..), 18 x 22 positions:
+    // entry q mod 3 holds the quads (plane q-1, q, q+1, 0) of X plane q, so
+    // input plane p goes to element 2 of entry p-1, 1 of entry p and 0 of
+    // entry p+1.  Element 3 stays zero from this clearing.
+    for (int i = tid; i < 3 * kQE; i += 256) lq[i] = u32x2{0u, 0u};
+    __syncthreads();
+    uint16_t* const lq16 = (uint16_t*)lq;
     constexpr int kPer = (kFRP + 255) / 256;
     FieldStage<F64, kPer> fs;
     fs.init(a.fin, 0, x0 - 2, y0 - 2, kFRW, kFRP, W, H, tid);
     using Raw = typename FieldStage<F64, kPer>::Raw;
     Raw preA[kPer], preB[kPer];
     auto store_plane = [&](const Raw* pre, int zi) {
-        uint16_t* dst = lr + mod3(zi) * kFRPs;
+        uint16_t* const d2 = lq16 + 4 * mod3(zi - 1) * kQE + 2;
+        uint16_t* const d1 = lq16 + 4 * mod3(zi) * kQE + 1;
+        uint16_t* const d0 = lq16 + 4 * mod3(zi + 1) * kQE;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int t = tid + 256 * k;
             if (t < kFRP) {
                 const uint16_t h = fs.bf(pre, k);
-                dst[t] = h;
-                dst[t + 3 * kFRPs] = h;
+                d2[4 * t] = h;
+                d1[4 * t] = h;
+                d0[4 * t] = h;
             }
         }
     };
@@ -896,20 +914,14 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     };
     // one plane of X -> tap partials, this wavefront's rows
     auto partials = [&](const u32x2* e, int s) {
-        int pe[8];
-        const int sbase = mod3(s - 1) * kFRPs;
-#pragma unroll
-        for (int e8 = 0; e8 < 8; ++e8) pe[e8] = sbase + toff[e8];
+        const u32x2* const q = lq + mod3(s) * kQE;  // entry of X plane s
 #pragma unroll
         for (int k = 0; k < kFG; ++k) {
-            u16x2 bw[4];
-#pragma unroll
-            for (int e8 = 0; e8 < 4; ++e8) {
-                bw[e8].x = lr[pe[2 * e8] + 4 * k * kFRW];
-                bw[e8].y = lr[pe[2 * e8 + 1] + 4 * k * kFRW];
-            }
-            const f32x4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wlift, __builtin_bit_cast(bf16x8, bw), cinit[k],
-                                                                    0, 0, 0);
+            const u32x2 qa0 = q[pa + 4 * k * kFRW], qa1 = q[pa + 4 * k * kFRW + 1];
+            const u32x2 qb0 = q[pb + 4 * k * kFRW], qb1 = q[pb + 4 * k * kFRW + 1];
+            const u32x4 ba = {qa0[0], qa0[1], qa1[0], qa1[1]}, bb = {qb0[0], qb0[1], qb1[0], qb1[1]};
+            f32x4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wqa, __builtin_bit_cast(bf16x8, ba), cinit[k], 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wqb, __builtin_bit_cast(bf16x8, bb), c, 0, 0, 0);
             u32x4 bx;
             bx[0] = relu_bf2(pack2bf(c[0], c[1]));
             bx[1] = relu_bf2(pack2bf(c[2], c[3]));
@@ -1279,7 +1291,15 @@ __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int mode,
     const int e = t & 7, lane = (t >> 3) & 63, o = t >> 9;
     const int j = lane >> 4, m = lane & 15;
     float v = 0.0f;
-    if (mode == 2) {
+    if (mode == 4) {
+        // lift operands of the fused output kernel's quad ring: MFMA o, lane,
+        // slot e = quad h (e / 4) and plane d (e % 4); plane slot 3 and the
+        // quads without a tap position carry zeros
+        if (t >= 1024) return;
+        const int jj = j, mm = m, d = e & 3, h = e >> 2;
+        const int dy = kQuad[o][jj][0], dx = kQuad[o][jj][1] + h;
+        if (d < 3 && q_real(o, jj, h) && mm < cout) v = W[(9 * d + 3 * dy + dx) * 256 + mm];
+    } else if (mode == 2) {
         if (t >= 512) return;
         const int tap = kLiftTap[8 * j + e];
         if (tap >= 0 && m < cout) v = W[tap * 256 + m];
@@ -1431,7 +1451,7 @@ Net::~Net()
     release(inf_);
     release(trn_);
     for (void* p : {(void*)params_, (void*)grads_, (void*)adam_m_, (void*)adam_v_, (void*)wl_, (void*)wlt_, (void*)wlo_,
-                    (void*)data_r_})
+                    (void*)wlq_, (void*)data_r_})
         if (p) (void)hipFree(p);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -1458,6 +1478,7 @@ int Net::init(unsigned long long seed)
     if (!rc) rc = hip(hipMalloc(&wl_, (size_t)kLayers * kPackedHalves * 2), "hipMalloc weights");
     if (!rc) rc = hip(hipMalloc(&wlt_, (size_t)kLayers * kPackedHalves * 2), "hipMalloc weights");
     if (!rc) rc = hip(hipMalloc(&wlo_, 3 * 512 * 2), "hipMalloc weights");
+    if (!rc) rc = hip(hipMalloc(&wlq_, 1024 * 2), "hipMalloc weights");
     if (rc) return rc;
     // He initialisation (std = sqrt(2 / fan_in)) of the ReLU layers, sqrt(1 / fan_in) for the output
     std::vector<float> h(kParams, 0.0f);
@@ -1486,6 +1507,8 @@ void Net::pack_all()
     // the output layer again as row-partial operands, for the fused inference kernel
     hipLaunchKernelGGL(k_pack, dim3(6), dim3(256), 0, stream_, params_ + (size_t)(kLayers - 1) * kLayerParams,
                        kNet[kLayers - 1].cin, kNet[kLayers - 1].cout, 3, wlo_);
+    // and the lift in the fused kernel's quad layout
+    hipLaunchKernelGGL(k_pack, dim3(4), dim3(256), 0, stream_, params_, kNet[0].cin, kNet[0].cout, 4, wlq_);
 }
 
 int Net::set_params(const float* host, long long n)
@@ -1645,7 +1668,7 @@ ConvArgs Net::layer_args(const BufSet& bs, int i, const FieldRef& fin, const Fie
         a.fused = 1;
         a.fin = fin;
         a.wl = wlo_;
-        a.wl2 = wl_;  // layer 0's lift operand
+        a.wl2 = wlq_;  // layer 0's lift, quad layout
         a.bias2 = params_ + kTaps * 256;
     }
     return a;
