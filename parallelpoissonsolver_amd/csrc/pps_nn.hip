@@ -499,26 +499,19 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
     }
 }
 
-// Lift operand: the tap in k-slot 8j + e (lane block j, element e), or -1 for
-// the 5 slots with zero weights.  The assignment makes the 16-lane u16 reads of
-// the four blocks j hit disjoint LDS banks (64 or 32 banks) for every e, with
-// the lift ring's plane stride (18 x 18) and the output kernel's (404 = 18 x 22
-// + 8); a zero-weight slot reads the address of a real block of the same e (a
-// broadcast).  The assignment was found by a search over slot permutations.
-__constant__ signed char kLiftTap[32] = {
-    23, 2, 8, 24, 10, 15, -1, 12,  // j = 0, e = 0 .. 7
-    21, 1, 0, 22, 14, 16, 17, -1,  // j = 1
-    20, 5, 6, 25, 13, 3, 18, -1,   // j = 2
-    19, -1, 7, 26, 11, 4, -1, 9,   // j = 3
-};
-// the tap lane block j reads for element e (a real block's tap for a zero slot)
-__device__ __forceinline__ int lift_read_tap(int j, int e)
+// Lift operand reads from the quad ring (z-packed: quad = input planes s-1,
+// s, s+1 and a zero at one x-y position).  Two K = 32 MFMAs o = 0, 1: the
+// operand of lane block j is the quad pair at (dy, dx) = kQuad[o][j] and
+// (dy, dx + 1), one 2 x 8-byte read.  Together they cover the 9 (dy, dx)
+// positions; the other slots carry zero weights (q_real).  With the ring row
+// of 18 quads, the pairs of blocks 0/1 and of 2/3 lie 16 quads apart (mod 32)
+// or coincide, which keeps the 8-byte reads of each half-wave on distinct LDS
+// banks.
+__constant__ signed char kQuad[2][4][2] = {{{0, 2}, {1, 0}, {1, 2}, {2, 0}}, {{0, 0}, {0, 0}, {2, 2}, {2, 2}}};
+// whether quad h (0, 1) of MFMA o's pair for block j is a tap position
+__host__ __device__ constexpr bool q_real(int o, int j, int h)
 {
-    const int t = kLiftTap[8 * j + e];
-    if (t >= 0) return t;
-    for (int jj = 0; jj < 4; ++jj)
-        if (kLiftTap[8 * jj + e] >= 0) return kLiftTap[8 * jj + e];
-    return 0;
+    return o == 0 ? !(h == 1 && (j == 0 || j == 2)) : (j == 0 || (j == 2 && h == 0));
 }
 
 // packed BF16 helpers of the lift kernels
@@ -596,24 +589,24 @@ struct FieldStage {
     }
 };
 
-// Lift (1 -> 16 channels): K = the 27 taps of one channel (+5 zero taps), one
-// MFMA per 16 voxels; lane (voxel n, block j) gathers taps 8j .. 8j+7 from a
-// BF16 staging ring and receives channels 4j .. 4j+3.  The accumulator starts
-// at the bias, or at -1e30 outside the domain, so relu leaves zeros there.
-// Input plane p is staged in ring slot p mod 4, and again in slot p mod 4 + 4
-// when that is below 6: planes z-1, z, z+1 then sit in three consecutive
-// slots (and plane z+2, which a wavefront ahead of the others may already be
-// writing before the next barrier, in none of them).  Input stage as IN_FIELD
-// (scaled to unit RMS).
+// Lift (1 -> 16 channels): the 27 taps of one channel as two K = 32 MFMAs
+// per 16 voxels over the z-packed quad ring (kQuad); lane (voxel n, block j)
+// receives channels 4j .. 4j+3.  The accumulator starts at the bias, or at
+// -1e30 outside the domain, so relu leaves zeros there.  Ring entry q mod 4
+// holds the quads (plane q-1, q, q+1, 0) of output plane q; input plane p
+// goes to element 2 of entry p-1, 1 of entry p and 0 of entry p+1.  With one
+// barrier per plane a wavefront ahead of the others writes entries z+1 ..
+// z+3 while the others still read entry z: four entries keep them apart.
+// Input stage as IN_FIELD (scaled to unit RMS).
 template <bool F64>
 __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
     constexpr int kP = kLH * kLW;
-    __shared__ uint16_t lds[6 * kP];
+    constexpr int kE = kP + 4;  // quads per entry (+ junk reads past the last row)
+    __shared__ __align__(16) u32x2 lq[4 * kE];
     const int D = a.out.D, H = a.out.H, W = a.out.W;
     const int nzc = (D + zc - 1) / zc;
-    auto mod4 = [](int v) { return v & 3; };
     const TileIdx ti = tile_of_block();
     const int b = ti.z / nzc;
     const int z0 = (ti.z - b * nzc) * zc;
@@ -621,7 +614,8 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
     const int x0 = ti.x * kTW, y0 = ti.y * kTHt;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane >> 4, n = lane & 15;
-    const bf16x8 w = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[lane]);
+    const bf16x8 wqa = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[lane]);
+    const bf16x8 wqb = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[64 + lane]);
     const int x = x0 + n;
     f32x4 cinit[4];
 #pragma unroll
@@ -630,28 +624,30 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
 #pragma unroll
         for (int r = 0; r < 4; ++r) cinit[i][r] = (x < W && y < H) ? (a.bias ? a.bias[4 * j + r] : 0.0f) : -1e30f;
     }
-    // the taps of this lane's k-slots: ring offset relative to the slot of plane z-1
-    int toff[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const int tap = lift_read_tap(j, e);
-        toff[e] = (tap / 9) * kP + ((tap / 3) % 3 + wave * 4) * kLW + tap % 3 + n;
-    }
+    // this lane's quad pairs for X voxel (row 4 wave, column n)
+    const int pa = (kQuad[0][j][0] + wave * 4) * kLW + kQuad[0][j][1] + n;
+    const int pb = (kQuad[1][j][0] + wave * 4) * kLW + kQuad[1][j][1] + n;
+    // element 3 of every quad stays zero from this clearing
+    for (int i = tid; i < 4 * kE; i += 256) lq[i] = u32x2{0u, 0u};
+    __syncthreads();
+    uint16_t* const lq16 = (uint16_t*)lq;
     constexpr int kPer = (kP + 255) / 256;
     FieldStage<F64, kPer> fs;
     fs.init(a.fin, b, x0 - 1, y0 - 1, kLW, kP, W, H, tid);
     using Raw = typename FieldStage<F64, kPer>::Raw;
     Raw preA[kPer], preB[kPer];
     auto store_plane = [&](const Raw* pre, int zi) {
-        uint16_t* dst = lds + mod4(zi) * kP;
-        const bool twice = mod4(zi) < 2;  // uniform
+        uint16_t* const d2 = lq16 + 4 * ((zi - 1) & 3) * kE + 2;
+        uint16_t* const d1 = lq16 + 4 * (zi & 3) * kE + 1;
+        uint16_t* const d0 = lq16 + 4 * ((zi + 1) & 3) * kE;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int t = tid + 256 * k;
             if (t < kP) {
                 const uint16_t h = fs.bf(pre, k);
-                dst[t] = h;
-                if (twice) dst[t + 4 * kP] = h;
+                d2[4 * t] = h;
+                d1[4 * t] = h;
+                d0[4 * t] = h;
             }
         }
     };
@@ -671,19 +667,14 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
         store_plane(buf, z + 1);
         fs.fetch(buf, z + 3, D);  // clamped beyond the chunk; unused there
         __syncthreads();
-        int pe[8];
-        const int sbase = mod4(z - 1) * kP;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) pe[e] = sbase + toff[e];
+        const u32x2* const q = lq + (z & 3) * kE;  // entry of output plane z
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            u16x2 bw[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                bw[e].x = lds[pe[2 * e] + i * kLW];
-                bw[e].y = lds[pe[2 * e + 1] + i * kLW];
-            }
-            const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, __builtin_bit_cast(bf16x8, bw), cinit[i], 0, 0, 0);
+            const u32x2 qa0 = q[pa + i * kLW], qa1 = q[pa + i * kLW + 1];
+            const u32x2 qb0 = q[pb + i * kLW], qb1 = q[pb + i * kLW + 1];
+            const u32x4 ba = {qa0[0], qa0[1], qa1[0], qa1[1]}, bb = {qb0[0], qb0[1], qb1[0], qb1[1]};
+            f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wqa, __builtin_bit_cast(bf16x8, ba), cinit[i], 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wqb, __builtin_bit_cast(bf16x8, bb), acc, 0, 0, 0);
             u32x2 o;
             o[0] = relu_bf2(pack2bf(acc[0], acc[1]));
             o[1] = relu_bf2(pack2bf(acc[2], acc[3]));
@@ -803,20 +794,6 @@ constexpr int kXW = kFW + 2, kXH = kFH + 2;       // X region: 16 x 20
 constexpr int kFRW = kXW + 2, kFRH = kXH + 2;     // input ring plane: 18 x 22
 constexpr int kFRP = kFRW * kFRH;
 constexpr int kQE = kFRP + 4;                     // quads per ring entry (+ junk reads past the last row)
-// Lift operand reads from the quad ring (z-packed: quad = input planes s-1,
-// s, s+1 and a zero at one x-y position).  Two K = 32 MFMAs o = 0, 1: the
-// operand of lane block j is the quad pair at (dy, dx) = kQuad[o][j] and
-// (dy, dx + 1), one 2 x 8-byte read.  Together they cover the 9 (dy, dx)
-// positions; the other slots carry zero weights (q_real).  With the ring row
-// of 18 quads, the pairs of blocks 0/1 and of 2/3 lie 16 quads apart (mod 32)
-// or coincide, which keeps the 8-byte reads of each half-wave on distinct LDS
-// banks.
-__constant__ signed char kQuad[2][4][2] = {{{0, 2}, {1, 0}, {1, 2}, {2, 0}}, {{0, 0}, {0, 0}, {2, 2}, {2, 2}}};
-// whether quad h (0, 1) of MFMA o's pair for block j is a tap position
-__host__ __device__ constexpr bool q_real(int o, int j, int h)
-{
-    return o == 0 ? !(h == 1 && (j == 0 || j == 2)) : (j == 0 || (j == 2 && h == 0));
-}
 constexpr int kXQ = kXW * kXH;
 constexpr int kXQp = kXQ + 4;                     // P row stride: 4 mod 64 banks
 constexpr int kPRows = 9;                         // (dz, dy) row partials
@@ -1281,8 +1258,8 @@ void launch_upadd_back(const ActRef& dx, const ActRef& dskip, int accumulate_ski
 // empty member).
 // forward:        W'[tap][k][m] = W[tap][ci = k][co = m]
 // backward data:  W'[tap][k][m] = W[26 - tap][ci = m][co = k]   (flipped, transposed)
-// lift (mode 2, 1 input channel): one operand, k = 8*(lane/16) + e holds the tap
-// kLiftTap[k] (zero where it is -1)
+// lift (mode 4, 1 input channel): two operands o for the quad-ring MFMAs,
+// k = 8*(lane/16) + e holds plane e%4 of quad e/4 of the pair kQuad[o][lane/16]
 // row partials (mode 3, 1 output channel): three operands o = dx, column
 // m = 3 dz + dy, k = 8*(lane/16) + e -> input channel 4*(lane/16) + e%4
 __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int mode, uint16_t* __restrict__ wl)
@@ -1292,17 +1269,13 @@ __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int mode,
     const int j = lane >> 4, m = lane & 15;
     float v = 0.0f;
     if (mode == 4) {
-        // lift operands of the fused output kernel's quad ring: MFMA o, lane,
-        // slot e = quad h (e / 4) and plane d (e % 4); plane slot 3 and the
-        // quads without a tap position carry zeros
+        // lift operands of the quad ring: MFMA o, lane, slot e = quad h
+        // (e / 4) and plane d (e % 4); plane slot 3 and the quads without a
+        // tap position carry zeros
         if (t >= 1024) return;
         const int jj = j, mm = m, d = e & 3, h = e >> 2;
         const int dy = kQuad[o][jj][0], dx = kQuad[o][jj][1] + h;
         if (d < 3 && q_real(o, jj, h) && mm < cout) v = W[(9 * d + 3 * dy + dx) * 256 + mm];
-    } else if (mode == 2) {
-        if (t >= 512) return;
-        const int tap = kLiftTap[8 * j + e];
-        if (tap >= 0 && m < cout) v = W[tap * 256 + m];
     } else if (mode == 3) {
         // the fused output layer's row-partial weights: operand o = dx, column
         // m = 3 dz + dy (m < 9), k = 8*(lane/16) + e -> input channel
@@ -1451,7 +1424,7 @@ Net::~Net()
     release(inf_);
     release(trn_);
     for (void* p : {(void*)params_, (void*)grads_, (void*)adam_m_, (void*)adam_v_, (void*)wl_, (void*)wlt_, (void*)wlo_,
-                    (void*)wlq_, (void*)data_r_})
+                    (void*)data_r_})
         if (p) (void)hipFree(p);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -1478,7 +1451,6 @@ int Net::init(unsigned long long seed)
     if (!rc) rc = hip(hipMalloc(&wl_, (size_t)kLayers * kPackedHalves * 2), "hipMalloc weights");
     if (!rc) rc = hip(hipMalloc(&wlt_, (size_t)kLayers * kPackedHalves * 2), "hipMalloc weights");
     if (!rc) rc = hip(hipMalloc(&wlo_, 3 * 512 * 2), "hipMalloc weights");
-    if (!rc) rc = hip(hipMalloc(&wlq_, 1024 * 2), "hipMalloc weights");
     if (rc) return rc;
     // He initialisation (std = sqrt(2 / fan_in)) of the ReLU layers, sqrt(1 / fan_in) for the output
     std::vector<float> h(kParams, 0.0f);
@@ -1498,17 +1470,15 @@ void Net::pack_all()
 {
     for (int l = 0; l < kLayers; ++l) {
         const float* W = params_ + (size_t)l * kLayerParams;
-        // the first layer (one input channel) runs on the lift kernel's operand layout
+        // the first layer (one input channel) runs on the lift kernels' quad-ring operands
         hipLaunchKernelGGL(k_pack, dim3(kPackedHalves / 256), dim3(256), 0, stream_, W, kNet[l].cin, kNet[l].cout,
-                           l == 0 ? 2 : 0, wl_ + (size_t)l * kPackedHalves);
+                           l == 0 ? 4 : 0, wl_ + (size_t)l * kPackedHalves);
         hipLaunchKernelGGL(k_pack, dim3(kPackedHalves / 256), dim3(256), 0, stream_, W, kNet[l].cin, kNet[l].cout, 1,
                            wlt_ + (size_t)l * kPackedHalves);
     }
     // the output layer again as row-partial operands, for the fused inference kernel
     hipLaunchKernelGGL(k_pack, dim3(6), dim3(256), 0, stream_, params_ + (size_t)(kLayers - 1) * kLayerParams,
                        kNet[kLayers - 1].cin, kNet[kLayers - 1].cout, 3, wlo_);
-    // and the lift in the fused kernel's quad layout
-    hipLaunchKernelGGL(k_pack, dim3(4), dim3(256), 0, stream_, params_, kNet[0].cin, kNet[0].cout, 4, wlq_);
 }
 
 int Net::set_params(const float* host, long long n)
@@ -1668,7 +1638,7 @@ ConvArgs Net::layer_args(const BufSet& bs, int i, const FieldRef& fin, const Fie
         a.fused = 1;
         a.fin = fin;
         a.wl = wlo_;
-        a.wl2 = wlq_;  // layer 0's lift, quad layout
+        a.wl2 = wl_;  // layer 0's lift operand
         a.bias2 = params_ + kTaps * 256;
     }
     return a;

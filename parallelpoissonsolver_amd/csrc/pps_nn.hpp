@@ -188,7 +188,6 @@ private:
     float *params_ = nullptr, *grads_ = nullptr, *adam_m_ = nullptr, *adam_v_ = nullptr;
     uint16_t *wl_ = nullptr, *wlt_ = nullptr;  // packed forward / backward-data weights, kLayers each
     uint16_t* wlo_ = nullptr;                  // output layer as tap-partial operands (fused inference)
-    uint16_t* wlq_ = nullptr;                  // lift as quad-ring operands (fused inference)
     BufSet inf_, trn_;
     float* data_r_ = nullptr;
     long long data_n_ = 0, data_pairs_ = 0, data_cap_ = 0;
