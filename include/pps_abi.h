@@ -223,7 +223,8 @@ int pps_nn_apply_field(void* ctx, int out, int in);
 /* average device time (HIP events, context stream) of `reps` launches of
  * network layer `layer` (0..13) at the context's grid, input B in FP64 fields
  * (prec 64) or in FP32 copies as the mixed-precision inner solve uses them
- * (prec 32); the layer's algorithmic HBM bytes and FLOPs per launch */
+ * (prec 32); the layer's algorithmic HBM bytes and FLOPs per launch.  The
+ * output layer runs as FCG runs it, also summing z . (a field of z's layout) */
 int pps_nn_time_layer(void* ctx, int layer, int prec, int reps, double* ms, double* bytes, double* flops);
 /* PPS_NN stages of ctx use this network (nn = NULL detaches) */
 int pps_nn_attach(void* ctx, void* nn);

@@ -932,9 +932,14 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     const char* obase = (const char*)a.fout.p + a.fout.off * oesz;
     const char* dbase = a.dot ? (const char*)a.dot + a.fout.off * oesz : nullptr;
     const long long oszb = a.fout.sz * oesz;
-    // dot operand of the next output plane, raw (loaded one plane ahead)
-    u32x2 dnr = {0u, 0u};
-    double dacc = 0.0;
+    // dot operands: loaded with the input prefetch two steps ahead (one
+    // buffer per step parity, like the prefetch) and converted where the step
+    // waits for that prefetch anyway.  The load is one 8-byte load in every
+    // case (an FP32 operand is its low word; a plane outside the chunk, or no
+    // dot at all, gets num_records 0): a load under a branch would make the
+    // compiler wait for the prefetch right after it.
+    u32x2 dA = {0u, 0u}, dB = {0u, 0u};
+    double dcur = 0.0, dacc = 0.0;
 
     // planes s = z0-1 .. z1 of X; the ring holds input planes s-1 .. s+1
     fs.fetch(preA, z0 - 2, D);
@@ -945,9 +950,18 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     fs.fetch(preB, z0 + 1, D);
     if (z0 >= 1) fetch_e(ev, z0 - 1);  // uniform
     // one plane: two planes of input loads in flight, each buffer refilled once written
-    auto step = [&](int s, Raw* buf) {
+    auto step = [&](int s, Raw* buf, u32x2& dbuf) {
         store_plane(buf, s + 1);
-        fs.fetch(buf, s + 3, D);  // clamped beyond the chunk; unused there
+        {
+            const double d64 = __builtin_bit_cast(double, dbuf), d32 = (double)__uint_as_float(dbuf[0]);
+            dcur = o64 ? d64 : d32;  // the operand of output plane s-1
+        }
+        fs.fetch(buf, s + 3, D);  // zeros beyond the domain; unused beyond the chunk
+        {
+            const bool din = dbase != nullptr && s + 1 >= z0 && s + 1 < z1;  // uniform: output plane s+1
+            const __amdgpu_buffer_rsrc_t r = buf_rsrc(din ? dbase + (s + 1) * oszb : obase, din ? kInRange : 0);
+            dbuf = __builtin_amdgcn_raw_buffer_load_b64(r, ooff, 0, 0);
+        }
         const bool sin = s >= 0 && s < D;  // X plane inside the domain (uniform)
         __syncthreads();  // input plane s+1 staged; every thread is past the previous gather
         if (sin) partials(ev, s);
@@ -974,23 +988,16 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
             else __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, ooff, 0, 0);
             // threads without an output hold a dot operand of 0 but may hold a
             // non-finite v (gathered from the padding)
-            if (a.dot) {  // uniform
-                const double pr = (double)v * (o64 ? __builtin_bit_cast(double, dnr) : (double)__uint_as_float(dnr[0]));
-                dacc += mine ? pr : 0.0;
-            }
-        }
-        if (a.dot && s >= z0 && s < z1) {  // uniform
-            const __amdgpu_buffer_rsrc_t r = buf_rsrc(dbase + s * oszb, kInRange);
-            if (o64) dnr = __builtin_amdgcn_raw_buffer_load_b64(r, ooff, 0, 0);
-            else dnr[0] = __builtin_amdgcn_raw_buffer_load_b32(r, ooff, 0, 0);
+            const double pr = (double)v * dcur;
+            dacc += mine ? pr : 0.0;
         }
         am1 = a0;
         a0 = ap1;
         ap1 = 0.0f;
     };
     for (int s = z0 - 1; s <= z1; s += 2) {
-        step(s, preA);
-        if (s + 1 <= z1) step(s + 1, preB);
+        step(s, preA, dA);
+        if (s + 1 <= z1) step(s + 1, preB, dB);
     }
     if (a.dot) {
         __shared__ double red[4];
@@ -1800,9 +1807,18 @@ int Net::time_layer(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec
     field_refs(g, out, out_prec, in, in_prec, rr, fi, fo);
     const hipStream_t own = stream_;
     stream_ = s;
-    const ConvArgs a = layer_args(inf_, i, fi, fo, nullptr, nullptr);
-    hipEvent_t e0, e1;
-    int rc = hip(hipEventCreate(&e0), "event");
+    ConvArgs a = layer_args(inf_, i, fi, fo, nullptr, nullptr);
+    // the fused output layer runs as the Krylov loop runs it: also summing
+    // z . (a field of z's layout), here the input field
+    double* dpart = nullptr;
+    int rc = 0;
+    if (a.fused && in_prec == out_prec) {
+        rc = hip(hipMalloc(&dpart, sizeof(double) * outf_blocks(a.out.D, a.out.H, a.out.W)), "hipMalloc partials");
+        a.dot = in;
+        a.dpart = dpart;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc) rc = hip(hipEventCreate(&e0), "event");
     if (!rc) rc = hip(hipEventCreate(&e1), "event");
     if (!rc) {
         launch_layer(a, i);  // warm-up
@@ -1813,8 +1829,9 @@ int Net::time_layer(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec
     }
     float t = 0.0f;
     if (!rc) rc = hip(hipEventElapsedTime(&t, e0, e1), "elapsed");
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (dpart) (void)hipFree(dpart);
     stream_ = own;
     if (rc) return rc;
     *ms = t / reps;
@@ -1836,8 +1853,10 @@ int Net::time_layer(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec
     if (i < kActs && pooled_act(i)) wr += vox[d.level + 1] * act_b;
     double fl = 2.0 * kTaps * d.cin * d.cout * v;
     if (i == kLayers - 1) {
-        // fused: reads r and the coarse activation, recomputes the lift
+        // fused: reads r and the coarse activation, recomputes the lift, and
+        // reads the dot operand
         rd = v * (in_prec == 64 ? 8.0 : 4.0) + vox[1] * act_b;
+        if (a.dot) rd += v * (out_prec == 64 ? 8.0 : 4.0);
         fl += 2.0 * kTaps * kNet[0].cin * kNet[0].cout * v;
     }
     *bytes = rd + wr;
