@@ -1848,17 +1848,22 @@ int pps_nn_time_layer(void* ctx, int layer, int prec, int reps, double* ms, doub
     // an apply on T0 <- B sets up the activations the layer reads
     int rc = pps_nn_apply_field(ctx, PPS_FIELD_T0, PPS_FIELD_B);
     if (rc) return rc;
-    Field fo, fi;
+    Field fo, fi, fd;
     if ((rc = field_of(*C, PPS_FIELD_T0, fo)) || (rc = field_of(*C, PPS_FIELD_B, fi))) return rc;
+    // the output layer's dot operand (Ap_old in FCG): a field of its own, a copy of B
+    if ((rc = C->get_work("nntime.dot", prec, fd))) return rc;
+    const long long n = C->L64.total;
+    const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 65536);
     if (prec == 32) {
         // FP32 copies of B and an FP32 output, as the mixed-precision inner solve hands them over
         Field fi32, fo32;
         if ((rc = C->get_work("nntime.in", 32, fi32)) || (rc = C->get_work("nntime.out", 32, fo32))) return rc;
-        const long long n = C->L64.total;
-        hipLaunchKernelGGL(k_narrow, dim3((unsigned)std::min<long long>((n + 255) / 256, 65536)), dim3(256), 0,
-                           C->stream, fi.d(), fi32.f(), n);
+        hipLaunchKernelGGL(k_narrow, dim3(nb), dim3(256), 0, C->stream, fi.d(), fi32.f(), n);
+        hipLaunchKernelGGL(k_narrow, dim3(nb), dim3(256), 0, C->stream, fi.d(), fd.f(), n);
         fi = fi32;
         fo = fo32;
+    } else if ((rc = C->hip(hipMemcpyAsync(fd.p, fi.p, n * 8, hipMemcpyDeviceToDevice, C->stream), "copy"))) {
+        return rc;
     }
     Scal* S = C->scal(C->nscal - 1);
     const Layout& L = C->lay(prec);
@@ -1871,7 +1876,7 @@ int pps_nn_time_layer(void* ctx, int layer, int prec, int reps, double* ms, doub
     g.n2 = C->solver.hi[2] - C->solver.lo[2];
     auto* net = dynamic_cast<nn::Net*>(C->nn.get());
     if (!net) return C->fail(PPS_ERR_STATE, "no network attached");
-    if (net->time_layer(C->stream, g, fo.p, prec, fi.p, prec, &S->v[S_RR], layer, reps, ms, bytes, flops))
+    if (net->time_layer(C->stream, g, fo.p, prec, fi.p, prec, fd.p, &S->v[S_RR], layer, reps, ms, bytes, flops))
         return C->fail(PPS_ERR_ARG, "%s", net->error());
     return PPS_OK;
 }

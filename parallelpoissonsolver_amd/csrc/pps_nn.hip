@@ -1797,7 +1797,7 @@ int Net::apply_dot(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec,
 }
 
 int Net::time_layer(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
-                    const double* rr, int i, int reps, double* ms, double* bytes, double* flops)
+                    const void* dot, const double* rr, int i, int reps, double* ms, double* bytes, double* flops)
 {
     if (i < 0 || i >= kLayers || reps < 1) return fail("time_layer: layer %d / reps %d out of range", i, reps);
     const int n[3] = {g.n0, g.n1, g.n2};
@@ -1809,12 +1809,12 @@ int Net::time_layer(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec
     stream_ = s;
     ConvArgs a = layer_args(inf_, i, fi, fo, nullptr, nullptr);
     // the fused output layer runs as the Krylov loop runs it: also summing
-    // z . (a field of z's layout), here the input field
+    // z . dot (a field of z's precision and layout)
     double* dpart = nullptr;
     int rc = 0;
-    if (a.fused && in_prec == out_prec) {
+    if (a.fused && dot) {
         rc = hip(hipMalloc(&dpart, sizeof(double) * outf_blocks(a.out.D, a.out.H, a.out.W)), "hipMalloc partials");
-        a.dot = in;
+        a.dot = dot;
         a.dpart = dpart;
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;

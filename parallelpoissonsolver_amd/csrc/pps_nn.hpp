@@ -145,9 +145,10 @@ public:
     const char* error() const override { return err_.c_str(); }
     // average device time of `reps` launches of layer `layer` of the inference
     // pass over the region (run apply() on the same region first); algorithmic
-    // HBM bytes and FLOPs of one launch
+    // HBM bytes and FLOPs of one launch.  dot (a field of out's precision and
+    // layout, or null): the fused output layer also sums out . dot, as FCG runs it
     int time_layer(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
-                   const double* rr, int layer, int reps, double* ms, double* bytes, double* flops);
+                   const void* dot, const double* rr, int layer, int reps, double* ms, double* bytes, double* flops);
 
     int set_params(const float* host, long long n);
     int get_params(float* host, long long n);
