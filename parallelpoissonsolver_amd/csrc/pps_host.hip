@@ -66,6 +66,17 @@ struct Stage {
         *nblocks = 0;
         return apply(out, in, gate);
     }
+    // whether apply_update() is available: the stage forms its input as the
+    // updated residual in = r_prev - alpha ap_prev (r_prev while *skip) and
+    // writes it to `in`, so the Krylov loop needs no update pass of its own
+    virtual bool forms_update() const { return false; }
+    virtual int apply_update(Field out, Field in, Field r_prev, Field ap_prev, const double* alpha, const int* skip,
+                             const int* gate, Field dot, int* nblocks)
+    {
+        (void)out; (void)in; (void)r_prev; (void)ap_prev; (void)alpha; (void)skip; (void)gate; (void)dot;
+        *nblocks = 0;
+        return PPS_ERR_UNSUPPORTED;
+    }
     // operator()(x, b) as the main solver on the context's X and B
     virtual int solve_main() { return PPS_ERR_UNSUPPORTED; }
 };
@@ -710,6 +721,27 @@ __global__ void k_fallback(T* out, const T* in, Layout L, Region R, T c, const i
     }
 }
 
+// the same with the residual update folded in: in = r_prev - alpha ap_prev
+// (r_prev while *skip) is formed, written and preconditioned
+template <typename T>
+__global__ void k_fallback_upd(T* out, T* in, const T* r_prev, const T* ap_prev, const double* alpha, const int* skip,
+                               Layout L, Region R, T c, const int* done, const int* fallback)
+{
+    if (*done || !*fallback) return;
+    const T al = (T)*alpha;
+    const bool sk = *skip != 0;
+    const int n0 = R.hi[0] - R.lo[0], n1 = R.hi[1] - R.lo[1], n2 = R.hi[2] - R.lo[2];
+    const long long n = (long long)n0 * n1 * n2;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % n0);
+        const long long q = t / n0;
+        const long long o = L.at(R.lo[0] + i, R.lo[1] + (int)(q % n1), R.lo[2] + (int)(q / n1));
+        const T r = sk ? r_prev[o] : r_prev[o] - al * ap_prev[o];
+        in[o] = r;
+        out[o] = c * r;
+    }
+}
+
 // a preconditioner is called with gate = &caller->done of the Krylov stage it
 // preconditions; that stage's scalar block holds ||r||^2 and the fallback flag
 static Scal* caller_of(const int* gate)
@@ -732,9 +764,23 @@ struct NNStage : Stage {
         int nb;
         return apply_dot(out, in, gate, Field{}, &nb);
     }
-    // dot.p set: the output kernel also writes per-block partials of out . dot
-    // to the context's reduction workspace (gated like the network itself)
     int apply_dot(Field out, Field in, const int* gate, Field dot, int* nblocks) override
+    {
+        return run(out, in, gate, dot, nullptr, nblocks);
+    }
+    bool forms_update() const override { return true; }
+    int apply_update(Field out, Field in, Field r_prev, Field ap_prev, const double* alpha, const int* skip,
+                     const int* gate, Field dot, int* nblocks) override
+    {
+        if (r_prev.prec != in.prec || ap_prev.prec != in.prec) return C.fail(PPS_ERR_ARG, "PPS_NN: update precision differs");
+        const NNResUpdate u{r_prev.p, ap_prev.p, in.p, alpha, skip};
+        return run(out, in, gate, dot, &u, nblocks);
+    }
+    // dot.p set: the output kernel also writes per-block partials of out . dot
+    // to the context's reduction workspace (gated like the network itself).
+    // u set: the network's lift (or the fallback) forms `in` from the residual
+    // update first
+    int run(Field out, Field in, const int* gate, Field dot, const NNResUpdate* u, int* nblocks)
     {
         *nblocks = 0;
         if (!C.nn) return C.fail(PPS_ERR_STATE, "PPS_NN stage without a network (pps_nn_attach)");
@@ -753,16 +799,26 @@ struct NNStage : Stage {
         g.n2 = C.solver.hi[2] - C.solver.lo[2];
         if (dot.p && dot.prec != out.prec) return C.fail(PPS_ERR_ARG, "PPS_NN: dot field precision differs");
         if (C.nn->apply_dot(C.stream, g, out.p, out.prec, in.p, in.prec, &caller->done, &caller->fallback,
-                            &caller->v[S_RR], dot.p, C.rb.partials, C.rb.cap, nblocks))
+                            &caller->v[S_RR], dot.p, C.rb.partials, C.rb.cap, nblocks, u))
             return C.fail(PPS_ERR_HIP, "learned preconditioner: %s", C.nn->error());
         const double* ds = C.grid.ds;
         const double jac = 1.0 / (-2.0 * (1.0 / (ds[0] * ds[0]) + 1.0 / (ds[1] * ds[1]) + 1.0 / (ds[2] * ds[2])));
-        if (in.prec == 32)
+        if (u) {
+            if (in.prec == 32)
+                hipLaunchKernelGGL(k_fallback_upd<float>, dim3(2048), dim3(256), 0, C.stream, out.f(), in.f(),
+                                   (const float*)u->r_prev, (const float*)u->ap_prev, u->alpha, u->skip, L, C.solver,
+                                   (float)jac, &caller->done, &caller->fallback);
+            else
+                hipLaunchKernelGGL(k_fallback_upd<double>, dim3(2048), dim3(256), 0, C.stream, out.d(), in.d(),
+                                   (const double*)u->r_prev, (const double*)u->ap_prev, u->alpha, u->skip, L, C.solver,
+                                   jac, &caller->done, &caller->fallback);
+        } else if (in.prec == 32) {
             hipLaunchKernelGGL(k_fallback<float>, dim3(2048), dim3(256), 0, C.stream, out.f(), in.f(), L, C.solver,
                                (float)jac, &caller->done, &caller->fallback);
-        else
+        } else {
             hipLaunchKernelGGL(k_fallback<double>, dim3(2048), dim3(256), 0, C.stream, out.d(), in.d(), L, C.solver, jac,
                                &caller->done, &caller->fallback);
+        }
         return PPS_OK;
     }
 };
@@ -790,6 +846,12 @@ struct IRStage : KrylovStage {
         const bool pw = (M == nullptr) || M->pointwise(&c);
         const bool flexible = in->cfg.kind == PPS_FCG;
         if ((!pw || flexible) && (rc = C.get_work(k + ".z", 32, z))) return rc;
+        // FCG with a preconditioner that forms the residual update itself (the
+        // learned one): r alternates between r32 and a second field, written by
+        // the preconditioner, and the update pass drops out of the loop
+        const bool fused_r = flexible && M && M->forms_update();
+        Field rr[2] = {r32, r32};
+        if (fused_r && (rc = C.get_work(k + ".r32b", 32, rr[1]))) return rc;
         // SI was initialised by solve_main; the outer residual kernel has just
         // written SI[S_INNER_NORM], and ir_inner_init resets iter/done
         const Geo g = C.gsolver(32);
@@ -818,7 +880,18 @@ struct IRStage : KrylovStage {
             launch_cg_rz<float>(g, C.stream, C.rb, r32.f(), z.f(), SI);
         }
         rc = run_loop(C, SI, in->cfg.max_iter, host_each, [&]() -> int {
-            if (flexible) {
+            if (fused_r) {
+                // iteration k: r_k = r_{k-1} - alpha_{k-1} Ap_{k-1} into rr[cur ^ 1]
+                int nb = 0;
+                int e = M->apply_update(z, rr[cur ^ 1], rr[cur], Ap[cur ^ 1], &SI->v[S_ALPHA], &SI->restart, &SI->done,
+                                        Ap[cur ^ 1], &nb);
+                if (e) return e;
+                if (nb > 0) launch_fcg_gamma_finish<float>(C.stream, C.rb, nb, SI);
+                launch_fcg_gamma<float>(g, C.stream, C.rb, z.f(), Ap[cur ^ 1].f(), SI, nb > 0);
+                launch_fcg_p_fused<float>(g, C.stream, C.rb, z.f(), p[cur ^ 1].f(), p[cur].f(), Ap[cur].f(),
+                                          rr[cur ^ 1].f(), d.f(), SI, false);
+                cur ^= 1;
+            } else if (flexible) {
                 int nb = 0;
                 int e = applyM_dot(&SI->done, Ap[cur ^ 1], &nb);
                 if (e) return e;

@@ -1221,6 +1221,21 @@ void launch_cheb_sweep(const Geo& g, hipStream_t s, const T* Y, const T* B, cons
 
 // ------------------------------------------------------------------ FCG
 
+// end of an FCG iteration once ||r||^2 of the updated residual is known
+__device__ inline void update_finish(Scal* S, double rr, bool history)
+{
+    S->v[S_PAP_OLD] = S->v[S_PAP];
+    S->v[S_RR] = rr;
+    const double err = sqrt(rr);
+    finish_iteration(S, err, history);
+    // stall monitor: every stall_window iterations the residual must have
+    // dropped by the factor v[S_AUX1], else the fallback preconditioner takes over
+    if (S->stall_window > 0 && !S->fallback && S->iter % S->stall_window == 0) {
+        if (err > S->v[S_AUX1] * S->v[S_AUX2]) S->fallback = 1;
+        S->v[S_AUX2] = err;
+    }
+}
+
 // gamma = z.Apold and beta.  When the preconditioner sums z.Apold itself (the
 // learned one, from its output kernel), only the finish runs, gated off once
 // the fallback preconditioner has taken over; the pass over z and Apold then
@@ -1275,8 +1290,13 @@ void launch_fcg_gamma_finish(hipStream_t s, RedBuf rb, int nblocks, Scal* S)
 
 // p = z - beta pold ; Ap = A p ; p.Ap, p.r ; alpha = p.r / p.Ap ; and the
 // previous iteration's x += alpha_prev pold (folded in: this pass reads pold
-// anyway), unless that step was discarded (restart)
-template <typename T>
+// anyway), unless that step was discarded (restart).
+// FUSED: the residual update r -= alpha Ap has no pass of its own; the next
+// iteration's preconditioner forms it while reading r.  This pass then also
+// sums r.Ap, Ap.Ap and r.r, and its finish takes over the update's:
+// ||r - alpha Ap||^2 = r.r - 2 alpha r.Ap + alpha^2 Ap.Ap, the convergence
+// test and the stall monitor.
+template <typename T, bool FUSED = false>
 struct OpFcgP : Gate {
     const T* __restrict__ z;
     const T* __restrict__ pold;
@@ -1285,6 +1305,7 @@ struct OpFcgP : Gate {
     const T* __restrict__ r;
     T* __restrict__ x;
     Scal* S;
+    bool history;  // FUSED: record the residual history
     T beta, alpha_prev;
     bool upd;  // apply the previous step to x
     __device__ void load()
@@ -1307,21 +1328,30 @@ struct OpFcgP : Gate {
         if (upd) x[c] = o.x + alpha_prev * v.pold;
         acc[0] += (double)vc * (double)Av;
         acc[1] += (double)vc * (double)o.r;
+        if constexpr (FUSED) {
+            acc[2] += (double)o.r * (double)Av;
+            acc[3] += (double)Av * (double)Av;
+            acc[4] += (double)o.r * (double)o.r;
+        }
     }
     __device__ void finish(const double* t)
     {
         S->v[S_PAP] = t[0];
         const double a = t[1] / t[0];
+        double rr_next = 0.0;
         // a non-finite step (the preconditioner returned inf/NaN or a null
         // direction): discard it, switch to the fallback preconditioner, restart
         if (!isfinite(a)) {
             S->restart = 1;
             S->fallback = 1;
             S->v[S_ALPHA] = 0.0;
+            if constexpr (FUSED) rr_next = t[4];  // r stays
         } else {
             S->restart = 0;
             S->v[S_ALPHA] = a;
+            if constexpr (FUSED) rr_next = fmax(t[4] - 2.0 * a * t[2] + a * a * t[3], 0.0);
         }
+        if constexpr (FUSED) update_finish(S, rr_next, history);
     }
 };
 
@@ -1338,7 +1368,25 @@ void launch_fcg_p(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* p
     op.r = r;
     op.x = x;
     op.S = S;
+    op.history = false;
     run_stencil<T, 2>(g, s, op, rb);
+}
+
+template <typename T>
+void launch_fcg_p_fused(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* pold, T* p, T* Ap, const T* r,
+                        T* x, Scal* S, bool history)
+{
+    OpFcgP<T, true> op;
+    op.gate = &S->done;
+    op.z = z;
+    op.pold = pold;
+    op.p = p;
+    op.Ap = Ap;
+    op.r = r;
+    op.x = x;
+    op.S = S;
+    op.history = history;
+    run_stencil<T, 5>(g, s, op, rb);
 }
 
 // the last step of an FCG solve: x += alpha p unless it was discarded
@@ -1400,19 +1448,7 @@ struct OpFcgUpdate : Gate {
         }
         acc[0] += (double)rv * (double)rv;
     }
-    __device__ void finish(const double* t)
-    {
-        S->v[S_PAP_OLD] = S->v[S_PAP];
-        S->v[S_RR] = t[0];
-        const double err = sqrt(t[0]);
-        finish_iteration(S, err, history);
-        // stall monitor: every stall_window iterations the residual must have
-        // dropped by the factor v[S_AUX1], else the fallback preconditioner takes over
-        if (S->stall_window > 0 && !S->fallback && S->iter % S->stall_window == 0) {
-            if (err > S->v[S_AUX1] * S->v[S_AUX2]) S->fallback = 1;
-            S->v[S_AUX2] = err;
-        }
-    }
+    __device__ void finish(const double* t) { update_finish(S, t[0], history); }
 };
 
 template <typename T>
@@ -1574,6 +1610,8 @@ void launch_ir_inner_init(const Geo& g, hipStream_t s, RedBuf rb, float* r, floa
     template void launch_fcg_gamma<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, Scal*, bool);       \
     template void launch_fcg_gamma_finish<T>(hipStream_t, RedBuf, int, Scal*);                                  \
     template void launch_fcg_p<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, T*, T*, const T*, T*, Scal*); \
+    template void launch_fcg_p_fused<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, T*, T*, const T*, T*,    \
+                                        Scal*, bool);                                                              \
     template void launch_fcg_final_x<T>(const Geo&, hipStream_t, T*, const T*, const Scal*);                      \
     template void launch_fcg_update<T>(const Geo&, hipStream_t, RedBuf, T*, const T*, Scal*, bool);
 

@@ -21,7 +21,7 @@ constexpr int kTY = 4;    // thread rows per block: 4 wavefronts per block
 constexpr int kThreads = kTX * kTY;
 constexpr int kRY = 4;    // rows per thread in the stencil tile
 constexpr int kTH = kTY * kRY;  // stencil tile height (rows)
-constexpr int kMaxRedQ = 4;
+constexpr int kMaxRedQ = 5;
 constexpr int kMinKz = 4;   // smallest z-chunk; sizes the reduction workspace
 
 struct Layout {
@@ -197,6 +197,12 @@ void launch_fcg_gamma_finish(hipStream_t s, RedBuf rb, int nblocks, Scal* S);
 template <typename T>
 void launch_fcg_p(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* pold, T* p, T* Ap, const T* r, T* x,
                   Scal* S);
+// launch_fcg_p when the preconditioner applies the residual update itself
+// (r of the next iteration = r - alpha Ap, formed while it reads r): also ends
+// the iteration (||r||^2 of the updated residual, convergence, stall monitor)
+template <typename T>
+void launch_fcg_p_fused(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* pold, T* p, T* Ap, const T* r,
+                        T* x, Scal* S, bool history);
 template <typename T>
 void launch_fcg_final_x(const Geo& g, hipStream_t s, T* x, const T* p, const Scal* S);
 template <typename T>

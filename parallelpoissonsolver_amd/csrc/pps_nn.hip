@@ -598,7 +598,10 @@ struct FieldStage {
 // barrier per plane a wavefront ahead of the others writes entries z+1 ..
 // z+3 while the others still read entry z: four entries keep them apart.
 // Input stage as IN_FIELD (scaled to unit RMS).
-template <bool F64>
+// UPD: the input is the residual after the Krylov step's update, formed here
+// from the previous residual (a.fin) and A p (a.fap): r = fin - alpha fap
+// (fin while *a.skip), and written to a.fcur on this block's own tile.
+template <bool F64, bool UPD>
 __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
@@ -632,40 +635,91 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
     __syncthreads();
     uint16_t* const lq16 = (uint16_t*)lq;
     constexpr int kPer = (kP + 255) / 256;
-    FieldStage<F64, kPer> fs;
+    using T = std::conditional_t<F64, double, float>;
+    FieldStage<F64, kPer> fs, fa;
     fs.init(a.fin, b, x0 - 1, y0 - 1, kLW, kP, W, H, tid);
     using Raw = typename FieldStage<F64, kPer>::Raw;
-    Raw preA[kPer], preB[kPer];
-    auto store_plane = [&](const Raw* pre, int zi) {
+    // residual update: A p staged like the input, and the byte offsets of the
+    // slots on this block's own tile (the others are kOOB: their store is dropped)
+    T alpha = (T)0;
+    bool skip = true;
+    unsigned coff[kPer];
+    const char* cbase = nullptr;
+    if constexpr (UPD) {
+        FieldRef fr = a.fin;
+        fr.p = const_cast<void*>(a.fap);
+        fa.init(fr, b, x0 - 1, y0 - 1, kLW, kP, W, H, tid);
+        alpha = (T)*a.alpha;
+        skip = *a.skip != 0;
+        cbase = (const char*)a.fcur + (a.fin.off + b * a.fin.sb) * (F64 ? 8 : 4);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int t = tid + 256 * k;
+            const int yy = t / kLW, xx = t - (t / kLW) * kLW;
+            const bool own = t < kP && xx >= 1 && xx <= kTW && yy >= 1 && yy <= kTHt;
+            coff[k] = own ? fs.soff[k] : kOOB;
+        }
+    }
+    Raw preA[kPer], preB[kPer], apA[kPer], apB[kPer];
+    auto fetch = [&](Raw* pre, Raw* ap, int zi) {
+        fs.fetch(pre, zi, D);
+        if constexpr (UPD) fa.fetch(ap, zi, D);
+    };
+    auto store_plane = [&](const Raw* pre, const Raw* ap, int zi) {
         uint16_t* const d2 = lq16 + 4 * ((zi - 1) & 3) * kE + 2;
         uint16_t* const d1 = lq16 + 4 * (zi & 3) * kE + 1;
         uint16_t* const d0 = lq16 + 4 * ((zi + 1) & 3) * kE;
+        __amdgpu_buffer_rsrc_t rc;
+        if constexpr (UPD) {
+            const bool zown = zi >= z0 && zi < z1;  // uniform: a plane of this block's chunk
+            rc = buf_rsrc(cbase + (zown ? zi : z0) * fs.szb, zown ? kInRange : 0);
+        }
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int t = tid + 256 * k;
             if (t < kP) {
-                const uint16_t h = fs.bf(pre, k);
+                uint16_t h;
+                if constexpr (UPD) {
+                    T r, q;
+                    if constexpr (F64) {
+                        r = __builtin_bit_cast(double, pre[k]);
+                        q = __builtin_bit_cast(double, ap[k]);
+                    } else {
+                        r = __uint_as_float(pre[k]);
+                        q = __uint_as_float(ap[k]);
+                    }
+                    const T v = skip ? r : r - alpha * q;
+                    if constexpr (F64) {
+                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rc, coff[k], 0, 0);
+                        h = (uint16_t)f2bf((float)(v * (double)fs.scale));
+                    } else {
+                        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rc, coff[k], 0, 0);
+                        h = (uint16_t)f2bf(v * fs.scale);
+                    }
+                } else {
+                    h = fs.bf(pre, k);
+                }
                 d2[4 * t] = h;
                 d1[4 * t] = h;
                 d0[4 * t] = h;
             }
         }
     };
-    fs.fetch(preA, z0 - 1, D);
-    store_plane(preA, z0 - 1);
-    fs.fetch(preA, z0, D);
-    store_plane(preA, z0);
-    fs.fetch(preA, z0 + 1, D);
-    fs.fetch(preB, z0 + 2, D);
+    fetch(preA, apA, z0 - 1);
+    store_plane(preA, apA, z0 - 1);
+    fetch(preA, apA, z0);
+    store_plane(preA, apA, z0);
+    fetch(preA, apA, z0 + 1);
+    fetch(preB, apB, z0 + 2);
     // pooled copy (2x2x2 average of the stored BF16 outputs) for the next level:
     // y pairs are neighbouring rows of this wave and z pairs consecutive planes
     // (chunks start on even planes), summed per lane; x pairs are neighbouring
     // lanes, added when the pooled voxel is written
     float ps[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
     // one plane: two planes of loads in flight, each buffer refilled once written
-    auto step = [&](int z, Raw* buf) {
-        store_plane(buf, z + 1);
-        fs.fetch(buf, z + 3, D);  // clamped beyond the chunk; unused there
+    auto step = [&](int z, Raw* buf, Raw* abuf) {
+        store_plane(buf, abuf, z + 1);
+        fetch(buf, abuf, z + 3);  // zeros beyond the domain; unused beyond the chunk
         __syncthreads();
         const u32x2* const q = lq + (z & 3) * kE;  // entry of output plane z
 #pragma unroll
@@ -707,8 +761,8 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
         }
     };
     for (int z = z0; z < z1; z += 2) {
-        step(z, preA);
-        if (z + 1 < z1) step(z + 1, preB);
+        step(z, preA, apA);
+        if (z + 1 < z1) step(z + 1, preB, apB);
     }
 }
 
@@ -762,8 +816,13 @@ void launch_lift(const ConvArgs& a, hipStream_t s)
 {
     const int zc = pick_zc(a.out.D, a.out.H, a.out.W, a.out.B);
     const dim3 grid = conv_grid(a.out.D, a.out.H, a.out.W, a.out.B, zc);
-    if (a.fin.is64) hipLaunchKernelGGL(k_lift<true>, grid, dim3(256), 0, s, a, zc);
-    else hipLaunchKernelGGL(k_lift<false>, grid, dim3(256), 0, s, a, zc);
+    if (a.fcur) {
+        if (a.fin.is64) hipLaunchKernelGGL((k_lift<true, true>), grid, dim3(256), 0, s, a, zc);
+        else hipLaunchKernelGGL((k_lift<false, true>), grid, dim3(256), 0, s, a, zc);
+    } else {
+        if (a.fin.is64) hipLaunchKernelGGL((k_lift<true, false>), grid, dim3(256), 0, s, a, zc);
+        else hipLaunchKernelGGL((k_lift<false, false>), grid, dim3(256), 0, s, a, zc);
+    }
 }
 
 // Inference form of the output layer with the lift fused in:
@@ -1659,13 +1718,21 @@ void Net::launch_layer(const ConvArgs& a, int i)
 }
 
 void Net::forward(BufSet& bs, const FieldRef& fin, const FieldRef& fout, const int* gate, const int* gate2,
-                  const void* dot, double* dpart)
+                  const void* dot, double* dpart, const NNResUpdate* upd)
 {
     for (int i = 0; i < kLayers; ++i) {
         ConvArgs a = layer_args(bs, i, fin, fout, gate, gate2);
         if (a.fused && dot) {
             a.dot = dot;
             a.dpart = dpart;
+        }
+        if (upd && kNet[i].mode_in == IN_FIELD) {
+            // the lift forms the input from the previous residual and writes it
+            a.fin.p = const_cast<void*>(upd->r_prev);
+            a.fap = upd->ap_prev;
+            a.fcur = upd->r_cur;
+            a.alpha = upd->alpha;
+            a.skip = upd->skip;
         }
         launch_layer(a, i);
     }
@@ -1771,14 +1838,15 @@ int Net::apply(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, con
                const int* gate, const int* gate2, const double* rr)
 {
     int nb;
-    return apply_dot(s, g, out, out_prec, in, in_prec, gate, gate2, rr, nullptr, nullptr, 0, &nb);
+    return apply_dot(s, g, out, out_prec, in, in_prec, gate, gate2, rr, nullptr, nullptr, 0, &nb, nullptr);
 }
 
 int Net::apply_dot(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
                    const int* gate, const int* gate2, const double* rr, const void* dot, double* dpart, int dcap,
-                   int* nblocks)
+                   int* nblocks, const NNResUpdate* upd)
 {
     *nblocks = 0;
+    if (upd && upd->r_cur != in) return fail("apply: the updated residual must be the input field");
     const int n[3] = {g.n0, g.n1, g.n2};
     int rc = ensure(inf_, 1, n, false);
     if (rc) return rc;
@@ -1790,7 +1858,7 @@ int Net::apply_dot(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec,
     const ConvArgs last = layer_args(inf_, kLayers - 1, fi, fo, gate, gate2);
     const long long nb = outf_blocks(last.out.D, last.out.H, last.out.W);
     const bool with_dot = dot && dpart && last.fused && nb <= dcap;
-    forward(inf_, fi, fo, gate, gate2, with_dot ? dot : nullptr, with_dot ? dpart : nullptr);
+    forward(inf_, fi, fo, gate, gate2, with_dot ? dot : nullptr, with_dot ? dpart : nullptr, upd);
     if (with_dot) *nblocks = (int)nb;
     stream_ = own;
     return hip(hipGetLastError(), "network launch");

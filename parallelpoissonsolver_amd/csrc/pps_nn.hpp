@@ -33,6 +33,17 @@ struct NNFieldGeom {
     int n0, n1, n2;
 };
 
+// residual update folded into the preconditioner's input read:
+// r_cur = r_prev - alpha ap_prev (r_cur = r_prev while *skip != 0), written to
+// r_cur on the region; all three fields share the input's precision and layout
+struct NNResUpdate {
+    const void* r_prev;
+    const void* ap_prev;
+    void* r_cur;
+    const double* alpha;
+    const int* skip;
+};
+
 // learned preconditioner as seen by the Krylov stages
 struct NNPrecond {
     virtual ~NNPrecond() = default;
@@ -43,10 +54,12 @@ struct NNPrecond {
                       const int* gate, const int* gate2, const double* rr) = 0;
     // apply() that also sums out . dot over the region (dot: a field of out's
     // precision and layout): per-block FP64 partials go to dpart[0 .. *nblocks)
-    // when they fit in dcap; otherwise *nblocks = 0 and nothing is summed
+    // when they fit in dcap; otherwise *nblocks = 0 and nothing is summed.
+    // upd (or null): the input `in` is first formed as upd->r_cur from the
+    // previous residual (see NNResUpdate); *rr must then already hold ||in||^2
     virtual int apply_dot(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
                           const int* gate, const int* gate2, const double* rr, const void* dot, double* dpart, int dcap,
-                          int* nblocks) = 0;
+                          int* nblocks, const NNResUpdate* upd) = 0;
     virtual const char* error() const = 0;
 };
 
@@ -102,6 +115,12 @@ struct ConvArgs {
     const float* bias2;      // fused: lift bias
     const void* dot;         // fused: field of fout's precision and layout; z . dot per block into dpart
     double* dpart;
+    // lift with the residual update: fin is r_prev; the staged value is
+    // fin - alpha fap (fin while *skip), also written to fcur on the lift's own tile
+    const void* fap;
+    void* fcur;
+    const double* alpha;
+    const int* skip;
 };
 
 struct BufSet {
@@ -141,7 +160,7 @@ public:
               const int* gate, const int* gate2, const double* rr) override;
     int apply_dot(hipStream_t s, const NNFieldGeom& g, void* out, int out_prec, const void* in, int in_prec,
                   const int* gate, const int* gate2, const double* rr, const void* dot, double* dpart, int dcap,
-                  int* nblocks) override;
+                  int* nblocks, const NNResUpdate* upd) override;
     const char* error() const override { return err_.c_str(); }
     // average device time of `reps` launches of layer `layer` of the inference
     // pass over the region (run apply() on the same region first); algorithmic
@@ -171,7 +190,7 @@ private:
     int ensure(BufSet& bs, int B, const int n[3], bool train);
     void release(BufSet& bs);
     void forward(BufSet& bs, const FieldRef& fin, const FieldRef& fout, const int* gate, const int* gate2,
-                 const void* dot = nullptr, double* dpart = nullptr);
+                 const void* dot = nullptr, double* dpart = nullptr, const NNResUpdate* upd = nullptr);
     ConvArgs layer_args(const BufSet& bs, int i, const FieldRef& fin, const FieldRef& fout, const int* gate,
                         const int* gate2) const;
     void launch_layer(const ConvArgs& a, int i);
