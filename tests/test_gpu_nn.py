@@ -210,6 +210,47 @@ def test_nonfinite_output_restarts_on_fallback():
     net.close()
 
 
+def test_mixed_precision_stall_switches_to_fallback(trained):
+    """The FP32 inner FCG forms its residual update inside the preconditioner;
+    once the stall monitor switches to Jacobi, the fallback kernel forms it."""
+    net, _ = trained
+    C = _ctx(32)
+    net.attach(C)
+    r = _solve(C, [dict(kind=L.IR, tol=1e-10, maxit=50), dict(kind=L.FCG, precision=32, tol=1e-5, maxit=5000),
+                   dict(kind=L.NN, precision=32, stall_window=5, stall_ratio=1e-30)])
+    print(f"[nn] IR + FP32 FCG, forced fallback: {r['iterations']} inner, {r['outer_iterations']} outer, "
+          f"{r['fallback_switches']} switches", flush=True)
+    # every inner solve starts on the network and switches after 5 iterations
+    assert r["fallback_switches"] >= 1
+    assert r["error_compute_operator"] < 1e-10
+    C.close()
+
+
+def test_mixed_precision_nonfinite_output_restarts_on_fallback():
+    """A network that returns NaN: in every FP32 inner solve the first step is
+    discarded (the residual stays, the update formed by the fallback skips it)
+    and Jacobi-FCG takes over from the same residual."""
+    net = L.Network(seed=9)
+    p = net.get_params()
+    p[13 * O.LAYER_PARAMS + O.TAPS * 256] = np.nan  # output bias -> every output NaN
+    net.set_params(p)
+    C = _ctx(24)
+    net.attach(C)
+    inner = dict(kind=L.FCG, precision=32, tol=1e-5, maxit=5000)
+    r = _solve(C, [dict(kind=L.IR, tol=1e-10, maxit=50), inner, dict(kind=L.NN, precision=32)])
+    jac = _solve(C, [dict(kind=L.IR, tol=1e-10, maxit=50), inner, dict(kind=L.JACOBI, precision=32)])
+    print(f"[nn] IR + FP32 FCG, NaN network: {r['iterations']} inner ({r['outer_iterations']} outer); "
+          f"Jacobi {jac['iterations']} ({jac['outer_iterations']} outer)", flush=True)
+    assert r["error_compute_operator"] < 1e-10
+    assert r["fallback_switches"] == r["outer_iterations"] >= 1
+    # one discarded step per inner solve; the residual norm of the fused path
+    # comes from the expanded form, so a convergence test may land one
+    # iteration apart
+    assert abs(r["iterations"] - r["outer_iterations"] - jac["iterations"]) <= r["outer_iterations"]
+    C.close()
+    net.close()
+
+
 def test_mixed_precision_learned_fcg(trained):
     net, _ = trained
     C = _ctx(48)
