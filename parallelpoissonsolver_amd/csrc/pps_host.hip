@@ -668,8 +668,26 @@ struct FCGStage : KrylovStage {
         if (early) return PPS_OK;
         const Geo g = C.gsolver(64);
         Field p[2] = {p0, p1}, Ap[2] = {A0, A1};
+        // a preconditioner that forms the residual update itself (the learned
+        // one): r alternates between two fields and the update pass drops out
+        const bool fused_r = pre && pre->forms_update();
+        Field rr[2] = {r, r};
+        if (fused_r && (rc = C.get_work(k + ".r2", 64, rr[1]))) return rc;
         int cur = 0;  // p[cur] = p_new, p[cur^1] = p_old
         rc = run_loop(C, S, cfg.max_iter, pre_host(), [&]() -> int {
+            if (fused_r) {
+                // iteration k: r_k = r_{k-1} - alpha_{k-1} Ap_{k-1} into rr[cur ^ 1]
+                int nb = 0;
+                int e = pre->apply_update(z, rr[cur ^ 1], rr[cur], Ap[cur ^ 1], &S->v[S_ALPHA], &S->restart, &S->done,
+                                          Ap[cur ^ 1], &nb);
+                if (e) return e;
+                if (nb > 0) launch_fcg_gamma_finish<double>(C.stream, C.rb, nb, S);
+                launch_fcg_gamma<double>(g, C.stream, C.rb, z.d(), Ap[cur ^ 1].d(), S, nb > 0);
+                launch_fcg_p_fused<double>(g, C.stream, C.rb, z.d(), p[cur ^ 1].d(), p[cur].d(), Ap[cur].d(),
+                                           rr[cur ^ 1].d(), x.d(), S, main);
+                cur ^= 1;
+                return PPS_OK;
+            }
             // the learned preconditioner sums z . Ap_old in its output kernel
             int nb = 0;
             int e = pre_apply_dot(z, r, &S->done, Ap[cur ^ 1], &nb);
