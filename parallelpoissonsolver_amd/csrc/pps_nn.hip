@@ -852,7 +852,11 @@ constexpr int kFW = 14, kFH = 18;                 // output tile
 constexpr int kXW = kFW + 2, kXH = kFH + 2;       // X region: 16 x 20
 constexpr int kFRW = kXW + 2, kFRH = kXH + 2;     // input ring plane: 18 x 22
 constexpr int kFRP = kFRW * kFRH;
-constexpr int kQE = kFRP + 4;                     // quads per ring entry (+ junk reads past the last row)
+// quads per ring entry: the 396 positions, junk reads past the last row, and
+// the slots of the staging threads beyond the region (2 x 256), which store
+// their zeros there instead of branching
+constexpr int kQE = 512;
+static_assert(kQE >= kFRP + 4 && kQE >= 512, "ring entry holds the region, the junk reads and every staging slot");
 constexpr int kXQ = kXW * kXH;
 constexpr int kXQp = kXQ + 4;                     // P row stride: 4 mod 64 banks
 constexpr int kPRows = 9;                         // (dz, dy) row partials
@@ -864,9 +868,10 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
     __shared__ __align__(16) u32x2 lq[3 * kQE];  // input quad ring: 3 entries
-    // P[3 dz + dy][voxel of the X region]; the tail pads the gathers of the
-    // four threads without an output voxel
-    __shared__ __align__(16) float lp[kPRows * kXQp + 32];
+    // P[3 dz + dy][voxel of the X region]; rows 9 .. 15 take the stores of the
+    // MFMA columns without a partial (no branch), and the tail pads the
+    // gathers of the four threads without an output voxel
+    __shared__ __align__(16) float lp[16 * kXQp + 32];
     const int D = a.out.D, H = a.out.H, W = a.out.W;
     const TileIdx ti = tile_of_block();
     const int z0 = ti.z * zc;  // one sample
@@ -923,19 +928,20 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     fs.init(a.fin, 0, x0 - 2, y0 - 2, kFRW, kFRP, W, H, tid);
     using Raw = typename FieldStage<F64, kPer>::Raw;
     Raw preA[kPer], preB[kPer];
-    auto store_plane = [&](const Raw* pre, int zi) {
-        uint16_t* const d2 = lq16 + 4 * mod3(zi - 1) * kQE + 2;
-        uint16_t* const d1 = lq16 + 4 * mod3(zi) * kQE + 1;
-        uint16_t* const d0 = lq16 + 4 * mod3(zi + 1) * kQE;
+    // e = (zi - 1) mod 3: the entries of plane zi are e, e + 1, e + 2 (mod 3)
+    auto store_plane = [&](const Raw* pre, int e) {
+        const int e1 = e == 2 ? 0 : e + 1, e2 = e1 == 2 ? 0 : e1 + 1;
+        uint16_t* const d2 = lq16 + 4 * e * kQE + 2;
+        uint16_t* const d1 = lq16 + 4 * e1 * kQE + 1;
+        uint16_t* const d0 = lq16 + 4 * e2 * kQE;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int t = tid + 256 * k;
-            if (t < kFRP) {
-                const uint16_t h = fs.bf(pre, k);
-                d2[4 * t] = h;
-                d1[4 * t] = h;
-                d0[4 * t] = h;
-            }
+            // slots beyond the region hold zeros and land in the entry's tail
+            const uint16_t h = fs.bf(pre, k);
+            d2[4 * t] = h;
+            d1[4 * t] = h;
+            d0[4 * t] = h;
         }
     };
     // up(E1) of this wavefront's rows, loaded one plane ahead (zeros outside
@@ -949,8 +955,8 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
         for (int k = 0; k < kFG; ++k) e[k] = __builtin_amdgcn_raw_buffer_load_b64(r, eoff[k], 0, 0);
     };
     // one plane of X -> tap partials, this wavefront's rows
-    auto partials = [&](const u32x2* e, int s) {
-        const u32x2* const q = lq + mod3(s) * kQE;  // entry of X plane s
+    auto partials = [&](const u32x2* e, int es) {
+        const u32x2* const q = lq + es * kQE;  // entry of X plane s (es = s mod 3)
 #pragma unroll
         for (int k = 0; k < kFG; ++k) {
             const u32x2 qa0 = q[pa + 4 * k * kFRW], qa1 = q[pa + 4 * k * kFRW + 1];
@@ -975,7 +981,7 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
             pr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, bx), wp[1], pr, 0, 0, 0);
             pr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, bp), wp[2], pr, 0, 0, 0);
             // lane (column m, j): voxels 4j .. 4j+3 of row wave + 4k
-            if (n < kPRows) *(f32x4*)(lp + n * kXQp + (wave + 4 * k) * kXW + 4 * j) = pr;
+            *(f32x4*)(lp + n * kXQp + (wave + 4 * k) * kXW + 4 * j) = pr;
         }
     };
     // this thread's output voxel: byte offset inside an output plane, kOOB
@@ -1001,16 +1007,18 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     double dcur = 0.0, dacc = 0.0;
 
     // planes s = z0-1 .. z1 of X; the ring holds input planes s-1 .. s+1
+    const int e0 = mod3(z0 - 1);  // entry of X plane z0 - 1
     fs.fetch(preA, z0 - 2, D);
-    store_plane(preA, z0 - 2);
+    store_plane(preA, mod3(z0 - 3));  // plane z0 - 2
     fs.fetch(preA, z0 - 1, D);
-    store_plane(preA, z0 - 1);
+    store_plane(preA, mod3(z0 - 2));  // plane z0 - 1
     fs.fetch(preA, z0, D);
     fs.fetch(preB, z0 + 1, D);
     if (z0 >= 1) fetch_e(ev, z0 - 1);  // uniform
     // one plane: two planes of input loads in flight, each buffer refilled once written
-    auto step = [&](int s, Raw* buf, u32x2& dbuf) {
-        store_plane(buf, s + 1);
+    // es = s mod 3, kept incrementally by the loop
+    auto step = [&](int s, int es, Raw* buf, u32x2& dbuf) {
+        store_plane(buf, es);  // plane s + 1
         {
             const double d64 = __builtin_bit_cast(double, dbuf), d32 = (double)__uint_as_float(dbuf[0]);
             dcur = o64 ? d64 : d32;  // the operand of output plane s-1
@@ -1023,7 +1031,7 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
         }
         const bool sin = s >= 0 && s < D;  // X plane inside the domain (uniform)
         __syncthreads();  // input plane s+1 staged; every thread is past the previous gather
-        if (sin) partials(ev, s);
+        if (sin) partials(ev, es);
         if (s + 1 <= z1 && s + 1 < D) fetch_e(ev, s + 1);
         __syncthreads();  // partials of plane s complete
         if (sin) {  // every thread gathers; those without an output read the padding
@@ -1054,9 +1062,12 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
         a0 = ap1;
         ap1 = 0.0f;
     };
+    int es = e0;
     for (int s = z0 - 1; s <= z1; s += 2) {
-        step(s, preA, dA);
-        if (s + 1 <= z1) step(s + 1, preB, dB);
+        step(s, es, preA, dA);
+        es = es == 2 ? 0 : es + 1;
+        if (s + 1 <= z1) step(s + 1, es, preB, dB);
+        es = es == 2 ? 0 : es + 1;
     }
     if (a.dot) {
         __shared__ double red[4];
