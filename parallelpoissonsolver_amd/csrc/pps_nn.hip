@@ -606,7 +606,11 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
     constexpr int kP = kLH * kLW;
-    constexpr int kE = kP + 4;  // quads per entry (+ junk reads past the last row)
+    // quads per entry: the 324 positions, junk reads past the last row, and the
+    // slots of the staging threads beyond the region (2 x 256), which store
+    // their zeros there instead of branching
+    constexpr int kE = 512;
+    static_assert(kE >= kP + 4, "ring entry holds the region, the junk reads and every staging slot");
     __shared__ __align__(16) u32x2 lq[4 * kE];
     const int D = a.out.D, H = a.out.H, W = a.out.W;
     const int nzc = (D + zc - 1) / zc;
@@ -676,33 +680,33 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
         }
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
+            // slots beyond the region hold zeros (their offset is kOOB) and
+            // land in the entry's tail
             const int t = tid + 256 * k;
-            if (t < kP) {
-                uint16_t h;
-                if constexpr (UPD) {
-                    T r, q;
-                    if constexpr (F64) {
-                        r = __builtin_bit_cast(double, pre[k]);
-                        q = __builtin_bit_cast(double, ap[k]);
-                    } else {
-                        r = __uint_as_float(pre[k]);
-                        q = __uint_as_float(ap[k]);
-                    }
-                    const T v = skip ? r : r - alpha * q;
-                    if constexpr (F64) {
-                        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rc, coff[k], 0, 0);
-                        h = (uint16_t)f2bf((float)(v * (double)fs.scale));
-                    } else {
-                        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rc, coff[k], 0, 0);
-                        h = (uint16_t)f2bf(v * fs.scale);
-                    }
+            uint16_t h;
+            if constexpr (UPD) {
+                T r, q;
+                if constexpr (F64) {
+                    r = __builtin_bit_cast(double, pre[k]);
+                    q = __builtin_bit_cast(double, ap[k]);
                 } else {
-                    h = fs.bf(pre, k);
+                    r = __uint_as_float(pre[k]);
+                    q = __uint_as_float(ap[k]);
                 }
-                d2[4 * t] = h;
-                d1[4 * t] = h;
-                d0[4 * t] = h;
+                const T v = skip ? r : r - alpha * q;
+                if constexpr (F64) {
+                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rc, coff[k], 0, 0);
+                    h = (uint16_t)f2bf((float)(v * (double)fs.scale));
+                } else {
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rc, coff[k], 0, 0);
+                    h = (uint16_t)f2bf(v * fs.scale);
+                }
+            } else {
+                h = fs.bf(pre, k);
             }
+            d2[4 * t] = h;
+            d1[4 * t] = h;
+            d0[4 * t] = h;
         }
     };
     fetch(preA, apA, z0 - 1);
@@ -716,6 +720,15 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
     // (chunks start on even planes), summed per lane; x pairs are neighbouring
     // lanes, added when the pooled voxel is written
     float ps[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+    // byte offset of this lane's pooled voxel in a pooled plane (row pair ip),
+    // kOOB for odd lanes and voxels outside the level: their store is dropped
+    unsigned poff[2];
+#pragma unroll
+    for (int ip = 0; ip < 2; ++ip) {
+        const int yf = y0 + wave * 4 + 2 * ip;
+        poff[ip] = (!(n & 1) && x < W && yf < H) ? (unsigned)(2 * (((yf >> 1) * a.pout.W + (x >> 1)) * kC + 4 * j))
+                                                  : kOOB;
+    }
     // one plane: two planes of loads in flight, each buffer refilled once written
     auto step = [&](int z, Raw* buf, Raw* abuf) {
         store_plane(buf, abuf, z + 1);
@@ -742,19 +755,17 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
                 ps[i >> 1][3] += __uint_as_float(o[1] & 0xffff0000u);
             }
         }
-        if (a.pout.p && ((z & 1) || z == D - 1)) {
+        if (a.pout.p && ((z & 1) || z == D - 1)) {  // uniform
+            const __amdgpu_buffer_rsrc_t rp = buf_rsrc(a.pout.p + vox(a.pout, b, 0, 0, z >> 1), kInRange);
 #pragma unroll
             for (int ip = 0; ip < 2; ++ip) {
                 float v[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) v[r] = 0.125f * (ps[ip][r] + xor1(ps[ip][r]));
-                const int yf = y0 + wave * 4 + 2 * ip;
-                if (!(n & 1) && x < W && yf < H) {
-                    u32x2 o;
-                    o[0] = pack2bf(v[0], v[1]);
-                    o[1] = pack2bf(v[2], v[3]);
-                    *(u32x2*)(a.pout.p + vox(a.pout, b, x >> 1, yf >> 1, z >> 1) + 4 * j) = o;
-                }
+                u32x2 o;
+                o[0] = pack2bf(v[0], v[1]);
+                o[1] = pack2bf(v[2], v[3]);
+                __builtin_amdgcn_raw_buffer_store_b64(o, rp, poff[ip], 0, 0);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) ps[ip][r] = 0.0f;
             }
