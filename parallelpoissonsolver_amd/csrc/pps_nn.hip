@@ -874,8 +874,10 @@ constexpr int kPRows = 9;                         // (dz, dy) row partials
 constexpr int kFG = kXH / 4;                      // x-rows (groups) per wavefront
 static_assert(kXW == 16 && kXH % 4 == 0, "one 16-voxel x-row per group, equal rows per wavefront");
 
+// (FP64 input: three blocks per CU, the raw FP64 prefetch does not fit the
+// register budget of four)
 template <bool F64>
-__global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
+__global__ __launch_bounds__(256, F64 ? 3 : 4) void k_outf(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
     __shared__ __align__(16) u32x2 lq[3 * kQE];  // input quad ring: 3 entries
