@@ -129,6 +129,24 @@ __device__ __forceinline__ float field_scale(const FieldRef& f)
     const double ms = *f.scale_sum / f.scale_count;
     return ms > 0.0 ? (float)(1.0 / sqrt(ms)) : 1.0f;
 }
+// Raw buffer access.  A buffer load whose offset lies at or beyond the
+// resource's num_records returns 0 and such a store is dropped, so voxels
+// outside the domain (zero padding, lanes without an output) need no branch
+// and no mask: their offset is kOOB, and a plane outside the domain gets
+// num_records = 0.
+constexpr unsigned kOOB = 0x80000000u;     // beyond every num_records used below
+constexpr int kInRange = 0x7fffffff;        // num_records of a resource that is in range
+// p and nrec must be uniform; they are read from the first lane so that the
+// resource lives in scalar registers (a per-lane resource costs a loop per access)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int nrec)
+{
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    void* q = (void*)(((unsigned long long)hi << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(q, (short)0, __builtin_amdgcn_readfirstlane(nrec), 0x00020000);
+}
+
 __device__ __forceinline__ long long fidx(const FieldRef& f, int b, int x, int y, int z)
 {
     return f.off + x * f.sx + y * f.sy + z * f.sz + b * f.sb;
@@ -374,13 +392,44 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
         sl[k] = t < kStaged ? (rem & 1) * kHalf + yy * kLW + (rem >> 1) : 4 * kPlane;
         if (t < kStaged && xs >= 0 && ys >= 0 && xs < W && ys < H && (MI != IN_FIELD || sc[k] == 0)) sval |= 1u << k;
     }
+    // the inference input stages (plain, up + skip) load through buffer
+    // offsets: a chunk outside the level, or a plane outside it, reads zeros
+    constexpr bool kBuf = MI == IN_PLAIN || MI == IN_UPADD;
+    unsigned boff[kChunksPerThread], bcoff[kChunksPerThread];
+    if constexpr (kBuf) {
+#pragma unroll
+        for (int k = 0; k < kChunksPerThread; ++k) {
+            const bool v = (sval >> k) & 1u;
+            boff[k] = v ? (unsigned)(((sy[k] * W + sx[k]) * kC + sc[k] * 8) * 2) : kOOB;
+            // UPADD: the coarse voxel under (x, y) in the level below
+            bcoff[k] = v ? (unsigned)((((sy[k] >> 1) * a.in.W + (sx[k] >> 1)) * kC + sc[k] * 8) * 2) : kOOB;
+        }
+    }
     // planes in flight: preA / preB hold raw loads issued one or two steps earlier
     Raw<MI> preA[kChunksPerThread], preB[kChunksPerThread];
     auto fetch_plane = [&](Raw<MI>* pre, int zi) {
-        const int zc_ = min(max(zi, 0), D - 1);
+        if constexpr (kBuf) {
+            const bool zin = zi >= 0 && zi < D;  // uniform
+            const int zz = zin ? zi : 0, nrec = zin ? kInRange : 0;
+            if constexpr (MI == IN_PLAIN) {
+                const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.in.p + vox(a.in, b, 0, 0, zz), nrec);
 #pragma unroll
-        for (int k = 0; k < kChunksPerThread; ++k)
-            pre[k] = fetch_raw<MI>(a.in, a.in2, a.fin, fscale, D, H, W, b, sx[k], sy[k], zc_, sc[k]);
+                for (int k = 0; k < kChunksPerThread; ++k) pre[k].a = __builtin_amdgcn_raw_buffer_load_b128(r, boff[k], 0, 0);
+            } else {
+                const __amdgpu_buffer_rsrc_t rc = buf_rsrc(a.in.p + vox(a.in, b, 0, 0, zz >> 1), nrec);
+                const __amdgpu_buffer_rsrc_t rs = buf_rsrc(a.in2.p + vox(a.in2, b, 0, 0, zz), nrec);
+#pragma unroll
+                for (int k = 0; k < kChunksPerThread; ++k) {
+                    pre[k].a = __builtin_amdgcn_raw_buffer_load_b128(rc, bcoff[k], 0, 0);
+                    pre[k].s = __builtin_amdgcn_raw_buffer_load_b128(rs, boff[k], 0, 0);
+                }
+            }
+        } else {
+            const int zc_ = min(max(zi, 0), D - 1);
+#pragma unroll
+            for (int k = 0; k < kChunksPerThread; ++k)
+                pre[k] = fetch_raw<MI>(a.in, a.in2, a.fin, fscale, D, H, W, b, sx[k], sy[k], zc_, sc[k]);
+        }
     };
     auto store_plane = [&](const Raw<MI>* pre, int zi) {
         u32x4* dst = lds + (zi & 3) * kPlane;
@@ -388,8 +437,9 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
 #pragma unroll
         for (int k = 0; k < kChunksPerThread; ++k) {
             const u32x4 v = finish<MI>(pre[k], a.fin, fscale);
-            (sl[k] < 4 * kPlane ? dst[sl[k]] : lds[4 * kPlane]) =
-                (zin && ((sval >> k) & 1u)) ? v : u32x4{0u, 0u, 0u, 0u};
+            u32x4* const d = sl[k] < 4 * kPlane ? dst + sl[k] : lds + 4 * kPlane;
+            if constexpr (kBuf) *d = v;  // zeros already where the padding is
+            else *d = (zin && ((sval >> k) & 1u)) ? v : u32x4{0u, 0u, 0u, 0u};
         }
     };
     // lane geometry of the two pair members: member j>>1, channel half j&1
@@ -405,6 +455,21 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
     if constexpr (kDepth == 2) fetch_plane(preB, z0 + 2);
     const int x = x0 + n;
     float ps[PO ? 2 : 1][4] = {};  // pooled-copy sums over two planes
+    // ReLU output stores through buffer offsets: this lane's 4 channels of
+    // row i inside an output plane, and of its pooled voxel for row pair ip
+    // (kOOB outside the level and for odd lanes: those stores are dropped)
+    unsigned ooff[4], poff[2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int y = y0 + wave * 4 + i;
+        ooff[i] = (x < W && y < H) ? (unsigned)(((y * W + x) * kC + 4 * j) * 2) : kOOB;
+    }
+#pragma unroll
+    for (int ip = 0; ip < 2; ++ip) {
+        const int yf = y0 + wave * 4 + 2 * ip;
+        poff[ip] = (PO && !(n & 1) && x < W && yf < H) ? (unsigned)((((yf >> 1) * a.pout.W + (x >> 1)) * kC + 4 * j) * 2)
+                                                        : kOOB;
+    }
     // one plane; the buffer just written is refilled kDepth planes ahead
     // (clamped beyond the chunk, unused there)
     auto step = [&](int z, Raw<MI>* buf) {
@@ -445,38 +510,37 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
             if constexpr (kPipe) __builtin_amdgcn_sched_barrier(0);  // keep the reads kPf steps ahead
         }
         if constexpr (MO == OUT_RELU) {
+            const __amdgpu_buffer_rsrc_t ro = buf_rsrc(a.out.p + vox(a.out, b, 0, 0, z), kInRange);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int y = y0 + wave * 4 + i;
                 u32x2 o;
                 o[0] = pack2bf(fmaxf(acc[i][0] + bias[0], 0.0f), fmaxf(acc[i][1] + bias[1], 0.0f));
                 o[1] = pack2bf(fmaxf(acc[i][2] + bias[2], 0.0f), fmaxf(acc[i][3] + bias[3], 0.0f));
-                if (x < W && y < H) {
-                    *(u32x2*)(a.out.p + vox(a.out, b, x, y, z) + 4 * j) = o;
-                    if constexpr (PO) {
-                        ps[i >> 1][0] += bf2f(o[0] & 0xffffu);
-                        ps[i >> 1][1] += __uint_as_float(o[0] & 0xffff0000u);
-                        ps[i >> 1][2] += bf2f(o[1] & 0xffffu);
-                        ps[i >> 1][3] += __uint_as_float(o[1] & 0xffff0000u);
-                    }
+                __builtin_amdgcn_raw_buffer_store_b64(o, ro, ooff[i], 0, 0);
+                if constexpr (PO) {
+                    // a voxel outside the level counts as zero in the average
+                    // (with an odd extent its pair partner is inside)
+                    const u32x2 om = ooff[i] != kOOB ? o : u32x2{0u, 0u};
+                    ps[i >> 1][0] += bf2f(om[0] & 0xffffu);
+                    ps[i >> 1][1] += __uint_as_float(om[0] & 0xffff0000u);
+                    ps[i >> 1][2] += bf2f(om[1] & 0xffffu);
+                    ps[i >> 1][3] += __uint_as_float(om[1] & 0xffff0000u);
                 }
             }
             // pooled copy for a layer that reads this output through a 2x2x2
             // average: x pairs are neighbouring lanes, y pairs rows of this
             // wavefront, z pairs consecutive planes (chunks start on even planes)
-            if (PO && ((z & 1) || z == D - 1)) {
+            if (PO && ((z & 1) || z == D - 1)) {  // uniform
+                const __amdgpu_buffer_rsrc_t rp = buf_rsrc(a.pout.p + vox(a.pout, b, 0, 0, z >> 1), kInRange);
 #pragma unroll
                 for (int ip = 0; ip < 2; ++ip) {
                     float v[4];
 #pragma unroll
                     for (int r = 0; r < 4; ++r) v[r] = 0.125f * (ps[ip][r] + xor1(ps[ip][r]));
-                    const int yf = y0 + wave * 4 + 2 * ip;
-                    if (!(n & 1) && x < W && yf < H) {
-                        u32x2 o;
-                        o[0] = pack2bf(v[0], v[1]);
-                        o[1] = pack2bf(v[2], v[3]);
-                        *(u32x2*)(a.pout.p + vox(a.pout, b, x >> 1, yf >> 1, z >> 1) + 4 * j) = o;
-                    }
+                    u32x2 o;
+                    o[0] = pack2bf(v[0], v[1]);
+                    o[1] = pack2bf(v[2], v[3]);
+                    __builtin_amdgcn_raw_buffer_store_b64(o, rp, poff[ip], 0, 0);
 #pragma unroll
                     for (int r = 0; r < 4; ++r) ps[ip][r] = 0.0f;
                 }
@@ -523,24 +587,6 @@ __device__ __forceinline__ unsigned relu_bf2(unsigned h)
     return __builtin_bit_cast(unsigned, __builtin_elementwise_max(__builtin_bit_cast(s16x2, h), s16x2{0, 0}));
 }
 __device__ __forceinline__ int mod3(int v) { return ((v % 3) + 3) % 3; }
-
-// Raw buffer access.  A buffer load whose offset lies at or beyond the
-// resource's num_records returns 0 and such a store is dropped, so voxels
-// outside the domain (zero padding, lanes without an output) need no branch
-// and no mask: their offset is kOOB, and a plane outside the domain gets
-// num_records = 0.
-constexpr unsigned kOOB = 0x80000000u;     // beyond every num_records used below
-constexpr int kInRange = 0x7fffffff;        // num_records of a resource that is in range
-// p and nrec must be uniform; they are read from the first lane so that the
-// resource lives in scalar registers (a per-lane resource costs a loop per access)
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int nrec)
-{
-    const unsigned long long a = (unsigned long long)p;
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
-    void* q = (void*)(((unsigned long long)hi << 32) | lo);
-    return __builtin_amdgcn_make_buffer_rsrc(q, (short)0, __builtin_amdgcn_readfirstlane(nrec), 0x00020000);
-}
 
 // Staging of the one-channel network input (an FP64 or FP32 field) into a
 // BF16 LDS plane: a thread owns kPer slots of the staged region.  Slots
