@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -230,10 +231,22 @@ template <class Body>
 static int run_loop(Ctx& C, Scal* S, int maxit, bool host_each, Body body)
 {
     if (maxit <= 0) return PPS_OK;  // while (iter < maxIteration) never runs
+    // Batch size: 4 doubling, then sized from the measured time per iteration
+    // so that a batch holds about kBatchMs of device work (at most 64
+    // iterations).  Iterations queued past convergence retire as gated empty
+    // launches of a few microseconds each; with expensive iterations (the
+    // learned preconditioner: milliseconds) a 64-iteration batch would leave
+    // up to ~100 of them, with cheap ones (small grids) the host needs the
+    // large batches to stay ahead.
+    constexpr double kBatchMs = 5.0;
+    using clock = std::chrono::steady_clock;
     int batch = host_each ? 1 : 4;
     int queued = 0;
     int slot = 0;
     bool have_prev = false;
+    int n_slot[2] = {0, 0};  // iterations of the batch behind each event
+    clock::time_point t_done{};
+    bool have_done = false;
     for (;;) {
         const int nb = batch;
         for (int b = 0; b < nb; ++b) {
@@ -241,6 +254,7 @@ static int run_loop(Ctx& C, Scal* S, int maxit, bool host_each, Body body)
             if (rc) return rc;
         }
         queued += nb;
+        n_slot[slot] = nb;
         int rc = C.hip(hipMemcpyAsync(C.flag_h + 4 * slot, &S->iter, 2 * sizeof(int), hipMemcpyDeviceToHost, C.stream),
                        "flag copy");
         if (rc) return rc;
@@ -249,14 +263,27 @@ static int run_loop(Ctx& C, Scal* S, int maxit, bool host_each, Body body)
         rc = C.hip(hipGetLastError(), "kernel launch");
         if (rc) return rc;
         const int waitslot = host_each ? slot : slot ^ 1;
+        double ms_per_it = 0.0;
         if (host_each || have_prev) {
             rc = C.hip(hipEventSynchronize(C.ev_batch[waitslot]), "event sync");
             if (rc) return rc;
             if (C.flag_h[4 * waitslot + 1] != 0) break;
+            // with two batches in flight the device ran the waited batch
+            // between the two completions the host saw
+            const clock::time_point now = clock::now();
+            if (have_done && n_slot[waitslot] > 0)
+                ms_per_it = std::chrono::duration<double, std::milli>(now - t_done).count() / n_slot[waitslot];
+            t_done = now;
+            have_done = true;
         }
         have_prev = true;
         slot ^= 1;
-        if (!host_each) batch = std::min(batch * 2, 64);
+        if (!host_each) {
+            if (ms_per_it > 0.0)
+                batch = std::max(1, std::min(64, (int)std::ceil(kBatchMs / ms_per_it)));
+            else
+                batch = std::min(batch * 2, 64);
+        }
         if (queued > maxit + 2 * 64) break;  // every gate is closed by now
     }
     return C.hip(hipStreamSynchronize(C.stream), "sync");
