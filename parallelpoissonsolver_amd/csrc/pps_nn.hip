@@ -580,7 +580,6 @@ __host__ __device__ constexpr bool q_real(int o, int j, int h)
 
 // packed BF16 helpers of the lift kernels
 typedef short s16x2 __attribute__((ext_vector_type(2)));
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 // relu of two packed BF16 values: as int16 a BF16 with the sign bit set is negative
 __device__ __forceinline__ unsigned relu_bf2(unsigned h)
 {
@@ -916,7 +915,6 @@ constexpr int kQE = 512;
 static_assert(kQE >= kFRP + 4 && kQE >= 512, "ring entry holds the region, the junk reads and every staging slot");
 constexpr int kXQ = kXW * kXH;
 constexpr int kXQp = kXQ + 4;                     // P row stride: 4 mod 64 banks
-constexpr int kPRows = 9;                         // (dz, dy) row partials
 constexpr int kFG = kXH / 4;                      // x-rows (groups) per wavefront
 static_assert(kXW == 16 && kXH % 4 == 0, "one 16-voxel x-row per group, equal rows per wavefront");
 
