@@ -1,5 +1,11 @@
-"""GPU checks of the learned preconditioner at the sizes of its configurations,
-with the committed network (parallelpoissonsolver_amd/weights/nn_64.npy):
+"""GPU checks at the sizes of the configurations the solver is measured on.
+
+* 256^3, seeded N(0,1) right-hand side (mean-subtracted): FP64 CG and BiCGSTAB
+  with the Jacobi preconditioner;
+* 512^3, 64 Gaussian sources: FP32 inner CG to 1e-5 inside FP64 refinement.
+
+And the learned preconditioner with the committed network
+(parallelpoissonsolver_amd/weights/nn_64.npy):
 
 * 256^3, k^-2 random-spectrum right-hand side: FP64 FCG with the network and
   the Jacobi fallback against FP64 FCG with Jacobi alone (the plain
@@ -57,6 +63,36 @@ def net():
     train_nn.load(n, WEIGHTS)
     yield n
     n.close()
+
+
+@pytest.mark.parametrize("kind", ["cg", "bicgstab"])
+def test_jacobi_krylov_fp64_256(kind):
+    print(f"[cfg] 256^3 N(0,1): FP64 {kind} + Jacobi", flush=True)
+    C = _context(256)
+    C.rhs_normal(17)
+    C.copy_field(L.FIELD_T3, L.FIELD_B)
+    k = L.CG if kind == "cg" else L.BICGSTAB
+    r = _solve(C, [dict(kind=k, tol=TOL, maxit=20000), dict(kind=L.JACOBI)])
+    print(f"[cfg] {kind} + Jacobi: {r['iterations']} it, {r['wall']:.3f} s, true residual "
+          f"{r['error_compute_operator']:.2e}", flush=True)
+    # both stop on their recursive residual, as the reference does
+    assert r["error_compute_operator"] < 3 * TOL
+    C.close()
+
+
+def test_mixed_precision_cg_512():
+    print("[cfg] 512^3 Gaussian sources: FP32 CG to 1e-5 inside FP64 refinement", flush=True)
+    C = _context(512)
+    C.rhs_gauss(3, 64, 0.01, 0.1)
+    C.copy_field(L.FIELD_T3, L.FIELD_B)
+    r = _solve(C, [dict(kind=L.IR, tol=TOL, maxit=100), dict(kind=L.CG, precision=32, tol=1e-5, maxit=200000),
+                   dict(kind=L.NONE, precision=32)])
+    print(f"[cfg] IR + FP32 CG: {r['iterations']} inner it ({r['outer_iterations']} outer), {r['wall']:.3f} s, "
+          f"true residual {r['error_compute_operator']:.2e}", flush=True)
+    # refinement stops on the true FP64 residual
+    assert r["error_compute_operator"] < TOL
+    assert r["outer_iterations"] >= 2
+    C.close()
 
 
 def test_learned_fcg_fp64_256(net):
