@@ -24,7 +24,7 @@ def test_facade_header_compiles_standalone():
 
 
 class SineProblem:
-    """u = sin(pi x) sin(pi y) sin(pi z), f = Laplacian u (the driver's ExactSolutionAndBCs)."""
+    """u = sin(pi x) sin(pi y) sin(pi z), f = Laplacian u (ExactSolutionAndBCs of tests/cpp/facade_main.cpp)."""
 
     def setFieldB(self, x, y, z):
         return -3 * np.pi ** 2 * self.trueSolutionFxyz(x, y, z)
