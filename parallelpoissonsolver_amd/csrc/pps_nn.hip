@@ -585,7 +585,6 @@ __device__ __forceinline__ unsigned relu_bf2(unsigned h)
 {
     return __builtin_bit_cast(unsigned, __builtin_elementwise_max(__builtin_bit_cast(s16x2, h), s16x2{0, 0}));
 }
-__device__ __forceinline__ int mod3(int v) { return ((v % 3) + 3) % 3; }
 
 // Staging of the one-channel network input (an FP64 or FP32 field) into a
 // BF16 LDS plane: a thread owns kPer slots of the staged region.  Slots
@@ -637,11 +636,12 @@ struct FieldStage {
 // Lift (1 -> 16 channels): the 27 taps of one channel as two K = 32 MFMAs
 // per 16 voxels over the z-packed quad ring (kQuad); lane (voxel n, block j)
 // receives channels 4j .. 4j+3.  The accumulator starts at the bias, or at
-// -1e30 outside the domain, so relu leaves zeros there.  Ring entry q mod 4
-// holds the quads (plane q-1, q, q+1, 0) of output plane q; input plane p
-// goes to element 2 of entry p-1, 1 of entry p and 0 of entry p+1.  With one
-// barrier per plane a wavefront ahead of the others writes entries z+1 ..
-// z+3 while the others still read entry z: four entries keep them apart.
+// -1e30 outside the domain, so relu leaves zeros there.  Ring entry q mod 2
+// holds the quads (plane q-1, q, q+1, 0) of output plane q.  Each staging
+// thread keeps its slots' last two planes (BF16) in a register and writes the
+// whole quad of entry p-1 once plane p arrives: one 8-byte store per slot and
+// plane.  With one barrier per plane a wavefront ahead of the others writes
+// entry z+1 while the others still read entry z: two entries keep them apart.
 // Input stage as IN_FIELD (scaled to unit RMS).
 // UPD: the input is the residual after the Krylov step's update, formed here
 // from the previous residual (a.fin) and A p (a.fap): r = fin - alpha fap
@@ -656,7 +656,7 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
     // their zeros there instead of branching
     constexpr int kE = 512;
     static_assert(kE >= kP + 4, "ring entry holds the region, the junk reads and every staging slot");
-    __shared__ __align__(16) u32x2 lq[4 * kE];
+    __shared__ __align__(16) u32x2 lq[2 * kE];
     const int D = a.out.D, H = a.out.H, W = a.out.W;
     const int nzc = (D + zc - 1) / zc;
     const TileIdx ti = tile_of_block();
@@ -679,11 +679,8 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
     // this lane's quad pairs for X voxel (row 4 wave, column n)
     const int pa = (kQuad[0][j][0] + wave * 4) * kLW + kQuad[0][j][1] + n;
     const int pb = (kQuad[1][j][0] + wave * 4) * kLW + kQuad[1][j][1] + n;
-    // element 3 of every quad stays zero from this clearing
-    for (int i = tid; i < 4 * kE; i += 256) lq[i] = u32x2{0u, 0u};
-    __syncthreads();
-    uint16_t* const lq16 = (uint16_t*)lq;
     constexpr int kPer = (kP + 255) / 256;
+    unsigned hist[kPer] = {};  // BF16 of the slot's planes p-2 (low) and p-1 (high)
     using T = std::conditional_t<F64, double, float>;
     FieldStage<F64, kPer> fs, fa;
     fs.init(a.fin, b, x0 - 1, y0 - 1, kLW, kP, W, H, tid);
@@ -714,10 +711,9 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
         fs.fetch(pre, zi, D);
         if constexpr (UPD) fa.fetch(ap, zi, D);
     };
+    // plane zi completes the quads of entry zi-1
     auto store_plane = [&](const Raw* pre, const Raw* ap, int zi) {
-        uint16_t* const d2 = lq16 + 4 * ((zi - 1) & 3) * kE + 2;
-        uint16_t* const d1 = lq16 + 4 * (zi & 3) * kE + 1;
-        uint16_t* const d0 = lq16 + 4 * ((zi + 1) & 3) * kE;
+        u32x2* const dq = lq + ((zi - 1) & 1) * kE;
         __amdgpu_buffer_rsrc_t rc;
         if constexpr (UPD) {
             const bool zown = zi >= z0 && zi < z1;  // uniform: a plane of this block's chunk
@@ -749,11 +745,12 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
             } else {
                 h = fs.bf(pre, k);
             }
-            d2[4 * t] = h;
-            d1[4 * t] = h;
-            d0[4 * t] = h;
+            dq[t] = u32x2{hist[k], (unsigned)h};
+            hist[k] = (hist[k] >> 16) | ((unsigned)h << 16);
         }
     };
+    // the two planes before the chunk fill the slots' history (their quads
+    // land in entries that are rewritten before they are read)
     fetch(preA, apA, z0 - 1);
     store_plane(preA, apA, z0 - 1);
     fetch(preA, apA, z0);
@@ -779,7 +776,7 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
         store_plane(buf, abuf, z + 1);
         fetch(buf, abuf, z + 3);  // zeros beyond the domain; unused beyond the chunk
         __syncthreads();
-        const u32x2* const q = lq + (z & 3) * kE;  // entry of output plane z
+        const u32x2* const q = lq + (z & 1) * kE;  // entry of output plane z
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const u32x2 qa0 = q[pa + i * kLW], qa1 = q[pa + i * kLW + 1];
@@ -924,7 +921,7 @@ template <bool F64>
 __global__ __launch_bounds__(256, F64 ? 3 : 4) void k_outf(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
-    __shared__ __align__(16) u32x2 lq[3 * kQE];  // input quad ring: 3 entries
+    __shared__ __align__(16) u32x2 lq[2 * kQE];  // input quad ring: 2 entries
     // P[3 dz + dy][voxel of the X region]; rows 9 .. 15 take the stores of the
     // MFMA columns without a partial (no branch), and the tail pads the
     // gathers of the four threads without an output voxel
@@ -974,31 +971,25 @@ __global__ __launch_bounds__(256, F64 ? 3 : 4) void k_outf(ConvArgs a, int zc)
     const int pa = (kQuad[0][j][0] + wave) * kFRW + kQuad[0][j][1] + n;
     const int pb = (kQuad[1][j][0] + wave) * kFRW + kQuad[1][j][1] + n;
     // input ring over the region (x0 - 2 .., y0 - 2 ..), 18 x 22 positions:
-    // entry q mod 3 holds the quads (plane q-1, q, q+1, 0) of X plane q, so
-    // input plane p goes to element 2 of entry p-1, 1 of entry p and 0 of
-    // entry p+1.  Element 3 stays zero from this clearing.
-    for (int i = tid; i < 3 * kQE; i += 256) lq[i] = u32x2{0u, 0u};
-    __syncthreads();
-    uint16_t* const lq16 = (uint16_t*)lq;
+    // entry q mod 2 holds the quads (plane q-1, q, q+1, 0) of X plane q.  Each
+    // staging thread keeps its slots' last two planes (BF16) in a register and
+    // writes the whole quad of entry p-1 once plane p arrives.
     constexpr int kPer = (kFRP + 255) / 256;
+    unsigned hist[kPer] = {};  // BF16 of the slot's planes p-2 (low) and p-1 (high)
     FieldStage<F64, kPer> fs;
     fs.init(a.fin, 0, x0 - 2, y0 - 2, kFRW, kFRP, W, H, tid);
     using Raw = typename FieldStage<F64, kPer>::Raw;
     Raw preA[kPer], preB[kPer];
-    // e = (zi - 1) mod 3: the entries of plane zi are e, e + 1, e + 2 (mod 3)
-    auto store_plane = [&](const Raw* pre, int e) {
-        const int e1 = e == 2 ? 0 : e + 1, e2 = e1 == 2 ? 0 : e1 + 1;
-        uint16_t* const d2 = lq16 + 4 * e * kQE + 2;
-        uint16_t* const d1 = lq16 + 4 * e1 * kQE + 1;
-        uint16_t* const d0 = lq16 + 4 * e2 * kQE;
+    // plane zi completes the quads of entry zi-1
+    auto store_plane = [&](const Raw* pre, int zi) {
+        u32x2* const dq = lq + ((zi - 1) & 1) * kQE;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int t = tid + 256 * k;
             // slots beyond the region hold zeros and land in the entry's tail
-            const uint16_t h = fs.bf(pre, k);
-            d2[4 * t] = h;
-            d1[4 * t] = h;
-            d0[4 * t] = h;
+            const unsigned h = fs.bf(pre, k);
+            dq[t] = u32x2{hist[k], h};
+            hist[k] = (hist[k] >> 16) | (h << 16);
         }
     };
     // up(E1) of this wavefront's rows, loaded one plane ahead (zeros outside
@@ -1013,7 +1004,7 @@ __global__ __launch_bounds__(256, F64 ? 3 : 4) void k_outf(ConvArgs a, int zc)
     };
     // one plane of X -> tap partials, this wavefront's rows
     auto partials = [&](const u32x2* e, int es) {
-        const u32x2* const q = lq + es * kQE;  // entry of X plane s (es = s mod 3)
+        const u32x2* const q = lq + es * kQE;  // entry of X plane s (es = s mod 2)
 #pragma unroll
         for (int k = 0; k < kFG; ++k) {
             const u32x2 qa0 = q[pa + 4 * k * kFRW], qa1 = q[pa + 4 * k * kFRW + 1];
@@ -1064,18 +1055,19 @@ __global__ __launch_bounds__(256, F64 ? 3 : 4) void k_outf(ConvArgs a, int zc)
     double dcur = 0.0, dacc = 0.0;
 
     // planes s = z0-1 .. z1 of X; the ring holds input planes s-1 .. s+1
-    const int e0 = mod3(z0 - 1);  // entry of X plane z0 - 1
+    // planes z0 - 2 and z0 - 1 fill the slots' history (their quads land in
+    // entries that are rewritten before they are read)
     fs.fetch(preA, z0 - 2, D);
-    store_plane(preA, mod3(z0 - 3));  // plane z0 - 2
+    store_plane(preA, z0 - 2);
     fs.fetch(preA, z0 - 1, D);
-    store_plane(preA, mod3(z0 - 2));  // plane z0 - 1
+    store_plane(preA, z0 - 1);
     fs.fetch(preA, z0, D);
     fs.fetch(preB, z0 + 1, D);
     if (z0 >= 1) fetch_e(ev, z0 - 1);  // uniform
     // one plane: two planes of input loads in flight, each buffer refilled once written
-    // es = s mod 3, kept incrementally by the loop
+    // es = s mod 2: the ring entry of X plane s
     auto step = [&](int s, int es, Raw* buf, u32x2& dbuf) {
-        store_plane(buf, es);  // plane s + 1
+        store_plane(buf, s + 1);
         {
             const double d64 = __builtin_bit_cast(double, dbuf), d32 = (double)__uint_as_float(dbuf[0]);
             dcur = o64 ? d64 : d32;  // the operand of output plane s-1
@@ -1119,12 +1111,9 @@ __global__ __launch_bounds__(256, F64 ? 3 : 4) void k_outf(ConvArgs a, int zc)
         a0 = ap1;
         ap1 = 0.0f;
     };
-    int es = e0;
     for (int s = z0 - 1; s <= z1; s += 2) {
-        step(s, es, preA, dA);
-        es = es == 2 ? 0 : es + 1;
-        if (s + 1 <= z1) step(s + 1, es, preB, dB);
-        es = es == 2 ? 0 : es + 1;
+        step(s, s & 1, preA, dA);
+        if (s + 1 <= z1) step(s + 1, (s + 1) & 1, preB, dB);
     }
     if (a.dot) {
         __shared__ double red[4];
