@@ -563,8 +563,20 @@ __global__ __launch_bounds__(256, 3) void k_conv(ConvArgs a, int zc)
     }
 }
 
-// Lift operand reads from the quad ring (z-packed: quad = input planes s-1,
-// s, s+1 and a zero at one x-y position).  Two K = 32 MFMAs o = 0, 1: the
+// Lift operand from the quad ring (z-packed: quad = input planes s-1, s, s+1
+// and a zero, at one x-y position): one K = 16 MFMA per dy.  Lane block
+// j < 3 holds the quad at dx = j of its voxel, block 3 carries zero weights
+// and reads block 2's quad (a broadcast).  The operand of dy for output row y
+// is quad row y + dy, so each 8-byte row read serves up to three output rows.
+// The two blocks of a half-wave read 17 consecutive quads between them (the
+// common ones as broadcasts), so the reads are free of LDS bank conflicts.
+// The chain of one row's three MFMAs uses this one instruction only: a
+// 16x16x32 MFMA accumulated straight into a 16x16x16 one gave wrong results.
+typedef short s16x4 __attribute__((ext_vector_type(4)));  // BF16 bits, K = 16 MFMA operand
+__device__ __forceinline__ int lift_dx(int j) { return j < 3 ? j : 2; }
+
+// The fused output kernel keeps two K = 32 MFMAs o = 0, 1 per row (three
+// K = 16 ones cost it more MFMA time than the shared reads save): the
 // operand of lane block j is the quad pair at (dy, dx) = kQuad[o][j] and
 // (dy, dx + 1), one 2 x 8-byte read.  Together they cover the 9 (dy, dx)
 // positions; the other slots carry zero weights (q_real).  With the ring row
@@ -577,6 +589,9 @@ __host__ __device__ constexpr bool q_real(int o, int j, int h)
 {
     return o == 0 ? !(h == 1 && (j == 0 || j == 2)) : (j == 0 || (j == 2 && h == 0));
 }
+// where the fused output kernel's operands sit in layer 0's packed weights
+constexpr int kLiftPairOff = 1024;
+static_assert(kLiftPairOff >= 768 && kLiftPairOff + 1024 <= kPackedHalves, "both lift layouts fit one layer's slot");
 
 // packed BF16 helpers of the lift kernels
 typedef short s16x2 __attribute__((ext_vector_type(2)));
@@ -633,8 +648,8 @@ struct FieldStage {
     }
 };
 
-// Lift (1 -> 16 channels): the 27 taps of one channel as two K = 32 MFMAs
-// per 16 voxels over the z-packed quad ring (kQuad); lane (voxel n, block j)
+// Lift (1 -> 16 channels): the 27 taps of one channel as three K = 16 MFMAs
+// per 16 voxels over the z-packed quad ring; lane (voxel n, block j)
 // receives channels 4j .. 4j+3.  The accumulator starts at the bias, or at
 // -1e30 outside the domain, so relu leaves zeros there.  Ring entry q mod 2
 // holds the quads (plane q-1, q, q+1, 0) of output plane q.  Each staging
@@ -651,11 +666,11 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
     constexpr int kP = kLH * kLW;
-    // quads per entry: the 324 positions, junk reads past the last row, and the
-    // slots of the staging threads beyond the region (2 x 256), which store
-    // their zeros there instead of branching
+    // quads per entry: the 324 positions and the slots of the staging threads
+    // beyond the region (2 x 256), which store their zeros there instead of
+    // branching
     constexpr int kE = 512;
-    static_assert(kE >= kP + 4, "ring entry holds the region, the junk reads and every staging slot");
+    static_assert(kE >= kP, "ring entry holds the region and every staging slot");
     __shared__ __align__(16) u32x2 lq[2 * kE];
     const int D = a.out.D, H = a.out.H, W = a.out.W;
     const int nzc = (D + zc - 1) / zc;
@@ -666,8 +681,9 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
     const int x0 = ti.x * kTW, y0 = ti.y * kTHt;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int j = lane >> 4, n = lane & 15;
-    const bf16x8 wqa = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[lane]);
-    const bf16x8 wqb = __builtin_bit_cast(bf16x8, ((const u32x4*)a.wl)[64 + lane]);
+    s16x4 wq[3];  // per dy
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy) wq[dy] = __builtin_bit_cast(s16x4, ((const u32x2*)a.wl)[64 * dy + lane]);
     const int x = x0 + n;
     f32x4 cinit[4];
 #pragma unroll
@@ -676,9 +692,8 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
 #pragma unroll
         for (int r = 0; r < 4; ++r) cinit[i][r] = (x < W && y < H) ? (a.bias ? a.bias[4 * j + r] : 0.0f) : -1e30f;
     }
-    // this lane's quad pairs for X voxel (row 4 wave, column n)
-    const int pa = (kQuad[0][j][0] + wave * 4) * kLW + kQuad[0][j][1] + n;
-    const int pb = (kQuad[1][j][0] + wave * 4) * kLW + kQuad[1][j][1] + n;
+    // this lane's quad in ring row 4 wave (the first of the 6 rows it reads)
+    const int pq = wave * 4 * kLW + n + lift_dx(j);
     constexpr int kPer = (kP + 255) / 256;
     unsigned hist[kPer] = {};  // BF16 of the slot's planes p-2 (low) and p-1 (high)
     using T = std::conditional_t<F64, double, float>;
@@ -777,13 +792,15 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
         fetch(buf, abuf, z + 3);  // zeros beyond the domain; unused beyond the chunk
         __syncthreads();
         const u32x2* const q = lq + (z & 1) * kE;  // entry of output plane z
+        u32x2 qr[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) qr[r] = q[pq + r * kLW];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const u32x2 qa0 = q[pa + i * kLW], qa1 = q[pa + i * kLW + 1];
-            const u32x2 qb0 = q[pb + i * kLW], qb1 = q[pb + i * kLW + 1];
-            const u32x4 ba = {qa0[0], qa0[1], qa1[0], qa1[1]}, bb = {qb0[0], qb0[1], qb1[0], qb1[1]};
-            f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wqa, __builtin_bit_cast(bf16x8, ba), cinit[i], 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wqb, __builtin_bit_cast(bf16x8, bb), acc, 0, 0, 0);
+            f32x4 acc = cinit[i];
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wq[dy], __builtin_bit_cast(s16x4, qr[i + dy]), acc, 0, 0, 0);
             u32x2 o;
             o[0] = relu_bf2(pack2bf(acc[0], acc[1]));
             o[1] = relu_bf2(pack2bf(acc[2], acc[3]));
@@ -1381,7 +1398,9 @@ void launch_upadd_back(const ActRef& dx, const ActRef& dskip, int accumulate_ski
 // empty member).
 // forward:        W'[tap][k][m] = W[tap][ci = k][co = m]
 // backward data:  W'[tap][k][m] = W[26 - tap][ci = m][co = k]   (flipped, transposed)
-// lift (mode 4, 1 input channel): two operands o for the quad-ring MFMAs,
+// lift (mode 4, 1 input channel): three K = 16 operands, one per dy, for the
+// lift kernel: k = 4*(lane/16) + d holds plane d of the quad at dx = lane/16;
+// and from kLiftPairOff two K = 32 operands o for the fused output kernel:
 // k = 8*(lane/16) + e holds plane e%4 of quad e/4 of the pair kQuad[o][lane/16]
 // row partials (mode 3, 1 output channel): three operands o = dx, column
 // m = 3 dz + dy, k = 8*(lane/16) + e -> input channel 4*(lane/16) + e%4
@@ -1392,13 +1411,24 @@ __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int mode,
     const int j = lane >> 4, m = lane & 15;
     float v = 0.0f;
     if (mode == 4) {
-        // lift operands of the quad ring: MFMA o, lane, slot e = quad h
-        // (e / 4) and plane d (e % 4); plane slot 3 and the quads without a
-        // tap position carry zeros
-        if (t >= 1024) return;
-        const int jj = j, mm = m, d = e & 3, h = e >> 2;
-        const int dy = kQuad[o][jj][0], dx = kQuad[o][jj][1] + h;
-        if (d < 3 && q_real(o, jj, h) && mm < cout) v = W[(9 * d + 3 * dy + dx) * 256 + mm];
+        if (t < 768) {
+            // the lift kernel's operands: K = 16 MFMA dy (one per 256
+            // halves), lane (channel m, block jj), plane d; plane slot 3 and
+            // block 3 carry zeros
+            const int dy = t >> 8, ln = (t >> 2) & 63, d = t & 3;
+            const int jj = ln >> 4, mm = ln & 15;
+            if (d < 3 && jj < 3 && mm < cout) v = W[(9 * d + 3 * dy + jj) * 256 + mm];
+        } else if (t >= kLiftPairOff && t < kLiftPairOff + 1024) {
+            // the fused output kernel's operands: MFMA o, lane, slot e = quad
+            // h (e / 4) and plane d (e % 4); plane slot 3 and the quads
+            // without a tap position carry zeros
+            const int u = t - kLiftPairOff, oo = u >> 9, ln = (u >> 3) & 63, d = u & 3, hh = (u >> 2) & 1;
+            const int jj = ln >> 4, mm = ln & 15;
+            const int dy = kQuad[oo][jj][0], dx = kQuad[oo][jj][1] + hh;
+            if (d < 3 && q_real(oo, jj, hh) && mm < cout) v = W[(9 * d + 3 * dy + dx) * 256 + mm];
+        } else {
+            return;
+        }
     } else if (mode == 3) {
         // the fused output layer's row-partial weights: operand o = dx, column
         // m = 3 dz + dy (m < 9), k = 8*(lane/16) + e -> input channel
@@ -1761,7 +1791,7 @@ ConvArgs Net::layer_args(const BufSet& bs, int i, const FieldRef& fin, const Fie
         a.fused = 1;
         a.fin = fin;
         a.wl = wlo_;
-        a.wl2 = wl_;  // layer 0's lift operand
+        a.wl2 = wl_ + kLiftPairOff;  // layer 0's lift operands of this kernel
         a.bias2 = params_ + kTaps * 256;
     }
     return a;
