@@ -928,21 +928,25 @@ constexpr int kFRP = kFRW * kFRH;
 constexpr int kQE = 512;
 static_assert(kQE >= kFRP + 4 && kQE >= 512, "ring entry holds the region, the junk reads and every staging slot");
 constexpr int kXQ = kXW * kXH;
-constexpr int kXQp = kXQ + 4;                     // P row stride: 4 mod 64 banks
+// P row stride: 4 mod 64 banks (measured: a stride of 8 mod 64 doubles the
+// bank conflicts of the 16-byte partial stores)
+constexpr int kXQp = kXQ + 4;
 constexpr int kFG = kXH / 4;                      // x-rows (groups) per wavefront
 static_assert(kXW == 16 && kXH % 4 == 0, "one 16-voxel x-row per group, equal rows per wavefront");
 
 // (FP64 input: three blocks per CU, the raw FP64 prefetch does not fit the
 // register budget of four)
 template <bool F64>
-__global__ __launch_bounds__(256, F64 ? 3 : 4) void k_outf(ConvArgs a, int zc)
+__global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
     __shared__ __align__(16) u32x2 lq[2 * kQE];  // input quad ring: 2 entries
-    // P[3 dz + dy][voxel of the X region]; rows 9 .. 15 take the stores of the
-    // MFMA columns without a partial (no branch), and the tail pads the
-    // gathers of the four threads without an output voxel
-    __shared__ __align__(16) float lp[16 * kXQp + 32];
+    // P[3 dz + dy][voxel of the X region], double-buffered by X plane parity
+    // so that the partials of plane s and the gather of plane s - 1 share one
+    // barrier; the tail pads the gathers of the four threads without an
+    // output voxel
+    constexpr int kLPS = 9 * kXQp + 32;
+    __shared__ __align__(16) float lp[2 * kLPS];
     const int D = a.out.D, H = a.out.H, W = a.out.W;
     const TileIdx ti = tile_of_block();
     const int z0 = ti.z * zc;  // one sample
@@ -1019,21 +1023,32 @@ __global__ __launch_bounds__(256, F64 ? 3 : 4) void k_outf(ConvArgs a, int zc)
 #pragma unroll
         for (int k = 0; k < kFG; ++k) e[k] = __builtin_amdgcn_raw_buffer_load_b64(r, eoff[k], 0, 0);
     };
-    // one plane of X -> tap partials, this wavefront's rows
+    // row k of X from the quad ring: lift (two MFMAs, relu to BF16) beside
+    // up(E1) as the partials' operand
+    auto lift_row = [&](const u32x2* q, const u32x2* e, int k) {
+        const u32x2 qa0 = q[pa + 4 * k * kFRW], qa1 = q[pa + 4 * k * kFRW + 1];
+        const u32x2 qb0 = q[pb + 4 * k * kFRW], qb1 = q[pb + 4 * k * kFRW + 1];
+        const u32x4 ba = {qa0[0], qa0[1], qa1[0], qa1[1]}, bb = {qb0[0], qb0[1], qb1[0], qb1[1]};
+        f32x4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wqa, __builtin_bit_cast(bf16x8, ba), cinit[k], 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wqb, __builtin_bit_cast(bf16x8, bb), c, 0, 0, 0);
+        u32x4 bx;
+        bx[0] = relu_bf2(pack2bf(c[0], c[1]));
+        bx[1] = relu_bf2(pack2bf(c[2], c[3]));
+        bx[2] = e[k][0];
+        bx[3] = e[k][1];
+        return bx;
+    };
+    // one plane of X -> tap partials, this wavefront's rows.  On FP32 fields
+    // the lift of row k + 1 is issued before the partials of row k so that
+    // the two chains overlap; the FP64 variant has no registers to spare for
+    // that at 4 blocks per CU.
     auto partials = [&](const u32x2* e, int es) {
         const u32x2* const q = lq + es * kQE;  // entry of X plane s (es = s mod 2)
+        u32x4 bx = lift_row(q, e, 0);
 #pragma unroll
         for (int k = 0; k < kFG; ++k) {
-            const u32x2 qa0 = q[pa + 4 * k * kFRW], qa1 = q[pa + 4 * k * kFRW + 1];
-            const u32x2 qb0 = q[pb + 4 * k * kFRW], qb1 = q[pb + 4 * k * kFRW + 1];
-            const u32x4 ba = {qa0[0], qa0[1], qa1[0], qa1[1]}, bb = {qb0[0], qb0[1], qb1[0], qb1[1]};
-            f32x4 c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wqa, __builtin_bit_cast(bf16x8, ba), cinit[k], 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wqb, __builtin_bit_cast(bf16x8, bb), c, 0, 0, 0);
-            u32x4 bx;
-            bx[0] = relu_bf2(pack2bf(c[0], c[1]));
-            bx[1] = relu_bf2(pack2bf(c[2], c[3]));
-            bx[2] = e[k][0];
-            bx[3] = e[k][1];
+            u32x4 bn;
+            if (!F64 && k + 1 < kFG) bn = lift_row(q, e, k + 1);
             // the row's neighbours: voxel n-1 (row_shr:1) and n+1 (row_shl:1)
             u32x4 bm, bp;
 #pragma unroll
@@ -1045,8 +1060,10 @@ __global__ __launch_bounds__(256, F64 ? 3 : 4) void k_outf(ConvArgs a, int zc)
                                                                f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
             pr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, bx), wp[1], pr, 0, 0, 0);
             pr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, bp), wp[2], pr, 0, 0, 0);
-            // lane (column m, j): voxels 4j .. 4j+3 of row wave + 4k
-            *(f32x4*)(lp + n * kXQp + (wave + 4 * k) * kXW + 4 * j) = pr;
+            // lane (column m, j): voxels 4j .. 4j+3 of row wave + 4k; the MFMA
+            // columns m >= 9 carry no partial
+            if (n < 9) *(f32x4*)(lp + es * kLPS + n * kXQp + (wave + 4 * k) * kXW + 4 * j) = pr;
+            if (k + 1 < kFG) bx = F64 ? lift_row(q, e, k + 1) : bn;
         }
     };
     // this thread's output voxel: byte offset inside an output plane, kOOB
@@ -1062,8 +1079,9 @@ __global__ __launch_bounds__(256, F64 ? 3 : 4) void k_outf(ConvArgs a, int zc)
     const char* obase = (const char*)a.fout.p + a.fout.off * oesz;
     const char* dbase = a.dot ? (const char*)a.dot + a.fout.off * oesz : nullptr;
     const long long oszb = a.fout.sz * oesz;
-    // dot operands: loaded with the input prefetch two steps ahead (one
-    // buffer per step parity, like the prefetch) and converted where the step
+    // dot operands: loaded with the input prefetch two steps ahead of their
+    // output plane's step (one buffer per step parity, like the prefetch:
+    // step s loads the operand of output plane s) and converted where the step
     // waits for that prefetch anyway.  The load is one 8-byte load in every
     // case (an FP32 operand is its low word; a plane outside the chunk, or no
     // dot at all, gets num_records 0): a load under a branch would make the
@@ -1081,30 +1099,34 @@ __global__ __launch_bounds__(256, F64 ? 3 : 4) void k_outf(ConvArgs a, int zc)
     fs.fetch(preA, z0, D);
     fs.fetch(preB, z0 + 1, D);
     if (z0 >= 1) fetch_e(ev, z0 - 1);  // uniform
-    // one plane: two planes of input loads in flight, each buffer refilled once written
-    // es = s mod 2: the ring entry of X plane s
+    // step s: stage input plane s+1, partials of X plane s, gather of X plane
+    // s - 1 and output plane s - 2.  Two planes of input loads in flight,
+    // each buffer refilled once written; es = s mod 2: the ring entry and the
+    // partials buffer of X plane s.  One barrier per step: the ring entry of
+    // plane s is staged; the partials of plane s - 1 are complete; the gather
+    // of plane s - 2, which read the buffer that plane s fills, is done.
     auto step = [&](int s, int es, Raw* buf, u32x2& dbuf) {
         store_plane(buf, s + 1);
         {
             const double d64 = __builtin_bit_cast(double, dbuf), d32 = (double)__uint_as_float(dbuf[0]);
-            dcur = o64 ? d64 : d32;  // the operand of output plane s-1
+            dcur = o64 ? d64 : d32;  // the operand of output plane s-2
         }
         fs.fetch(buf, s + 3, D);  // zeros beyond the domain; unused beyond the chunk
         {
-            const bool din = dbase != nullptr && s + 1 >= z0 && s + 1 < z1;  // uniform: output plane s+1
-            const __amdgpu_buffer_rsrc_t r = buf_rsrc(din ? dbase + (s + 1) * oszb : obase, din ? kInRange : 0);
+            const bool din = dbase != nullptr && s >= z0 && s < z1;  // uniform: output plane s
+            const __amdgpu_buffer_rsrc_t r = buf_rsrc(din ? dbase + s * oszb : obase, din ? kInRange : 0);
             dbuf = __builtin_amdgcn_raw_buffer_load_b64(r, ooff, 0, 0);
         }
-        const bool sin = s >= 0 && s < D;  // X plane inside the domain (uniform)
-        __syncthreads();  // input plane s+1 staged; every thread is past the previous gather
-        if (sin) partials(ev, es);
+        __syncthreads();
+        if (s >= 0 && s < D && s <= z1) partials(ev, es);  // uniform
         if (s + 1 <= z1 && s + 1 < D) fetch_e(ev, s + 1);
-        __syncthreads();  // partials of plane s complete
-        if (sin) {  // every thread gathers; those without an output read the padding
+        const int sg = s - 1;                        // X plane gathered
+        if (sg >= 0 && sg < D && sg >= z0 - 1) {      // uniform; every thread gathers,
+            const float* const lg = lp + (es ^ 1) * kLPS;  // those without an output read the padding
             float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
             for (int dy = 0; dy < 3; ++dy) {
-                const float* pq = lp + dy * kXQp + q0 + dy * kXW;
+                const float* pq = lg + dy * kXQp + q0 + dy * kXW;
                 s0 += pq[0];
                 s1 += pq[3 * kXQp];
                 s2 += pq[6 * kXQp];
@@ -1113,7 +1135,7 @@ __global__ __launch_bounds__(256, F64 ? 3 : 4) void k_outf(ConvArgs a, int zc)
             a0 += s1;
             ap1 += s0;
         }
-        const int zo = s - 1;
+        const int zo = s - 2;
         if (zo >= z0 && zo < z1) {  // uniform
             const float v = am1 + bout;
             const __amdgpu_buffer_rsrc_t r = buf_rsrc(obase + zo * oszb, kInRange);
@@ -1128,9 +1150,9 @@ __global__ __launch_bounds__(256, F64 ? 3 : 4) void k_outf(ConvArgs a, int zc)
         a0 = ap1;
         ap1 = 0.0f;
     };
-    for (int s = z0 - 1; s <= z1; s += 2) {
+    for (int s = z0 - 1; s <= z1 + 1; s += 2) {
         step(s, s & 1, preA, dA);
-        if (s + 1 <= z1) step(s + 1, (s + 1) & 1, preB, dB);
+        if (s + 1 <= z1 + 1) step(s + 1, (s + 1) & 1, preB, dB);
     }
     if (a.dot) {
         __shared__ double red[4];
