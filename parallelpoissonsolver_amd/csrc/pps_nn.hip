@@ -910,8 +910,9 @@ void launch_lift(const ConvArgs& a, hipStream_t s)
 //      MFMAs (one per dx) into columns m = 3 dz + dy; the skip sum a + e
 //      happens inside the MFMA (the weight operand repeats W[.][4j .. 4j+3]
 //      for both halves).  The 9 rows go to LDS as 16-byte writes of 4 voxels;
-// then every thread adds the 9 partials of its output voxel (rows y-1 .. y+1)
-// into three running output planes; plane s completes output plane s-1.
+// in the next step (after its one barrier) every thread adds the 9 partials of
+// its output voxel (rows y-1 .. y+1) into three running output planes; the
+// gather of plane s completes output plane s-1.
 // The output columns are 1 .. 14 of each row, so a shift never reaches past
 // the row for a value that is used.  With a.dot set the kernel also sums
 // z . dot over its outputs (FP64) and writes one partial per block to a.dpart
