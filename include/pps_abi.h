@@ -220,6 +220,14 @@ int pps_nn_train(void* nn, const pps_nn_train_cfg* cfg, pps_nn_train_stats* stat
 /* out = M(in) with the attached network on FP64 fields, as a PPS_NN stage
  * applies it (input scaled to unit RMS over the solver region) */
 int pps_nn_apply_field(void* ctx, int out, int in);
+/* out = M(in) as pps_nn_apply_field, with the network run on FP64 fields
+ * (prec 64) or on FP32 copies as the mixed-precision inner solve runs it
+ * (prec 32; the output is widened into `out`).  With dot >= 0 the output
+ * layer also sums gamma = out . dot over the solver region in its epilogue,
+ * as FCG's gamma = z . Ap_old (FP32 copy of dot for prec 32); the block
+ * partials are added in block order on the host.  No reference counterpart:
+ * the reference has no learned preconditioner. */
+int pps_nn_apply_field_prec(void* ctx, int out, int in, int prec, int dot, double* gamma);
 /* average device time (HIP events, context stream) of `reps` launches of
  * network layer `layer` (0..13) at the context's grid, input B in FP64 fields
  * (prec 64) or in FP32 copies as the mixed-precision inner solve uses them

@@ -137,6 +137,7 @@ SIGNATURES = {
     "pps_nn_dataset": ([_VP, ctypes.c_longlong, _FP, _DP], ctypes.c_int),
     "pps_nn_train": ([_VP, ctypes.POINTER(CNNTrainCfg), ctypes.POINTER(CNNTrainStats)], ctypes.c_int),
     "pps_nn_apply_field": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "pps_nn_apply_field_prec": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _DP], ctypes.c_int),
     "pps_nn_time_layer": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP], ctypes.c_int),
     "pps_nn_attach": ([_VP, _VP], ctypes.c_int),
 }
@@ -286,9 +287,18 @@ class Context:
         self._chk(lib().pps_time_kernel(self.h, kernel, reps, ctypes.byref(ms), ctypes.byref(nbytes)))
         return ms.value, nbytes.value
 
-    def nn_apply(self, out, inp):
-        """out = M(inp) with the attached network (FP64 fields)."""
-        self._chk(lib().pps_nn_apply_field(self.h, out, inp))
+    def nn_apply(self, out, inp, prec=None, dot=None):
+        """out = M(inp) with the attached network.  prec None: FP64 fields
+        (pps_nn_apply_field); 64 or 32: the network on FP64 fields or on FP32
+        copies (pps_nn_apply_field_prec).  With a dot field, the output layer
+        also sums out . dot in its epilogue, as FCG's gamma; returned."""
+        if prec is None and dot is None:
+            self._chk(lib().pps_nn_apply_field(self.h, out, inp))
+            return None
+        g = ctypes.c_double(0.0)
+        self._chk(lib().pps_nn_apply_field_prec(self.h, out, inp, 64 if prec is None else prec,
+                                                -1 if dot is None else dot, ctypes.byref(g)))
+        return None if dot is None else g.value
 
     def nn_time_layer(self, layer, reps=10, prec=32):
         """(ms per launch, algorithmic bytes, flops) of one layer of the attached network,

@@ -801,6 +801,11 @@ __global__ void k_narrow(const double* __restrict__ a, float* __restrict__ b, lo
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         b[i] = (float)a[i];
 }
+__global__ void k_widen(const float* __restrict__ a, double* __restrict__ b, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        b[i] = (double)a[i];
+}
 
 struct NNStage : Stage {
     using Stage::Stage;
@@ -1956,6 +1961,54 @@ int pps_nn_apply_field(void* ctx, int out, int in)
     rc = stage.apply(fo, fi, &S->done);
     if (rc) return rc;
     return C->hip(hipStreamSynchronize(C->stream), "sync");
+}
+
+int pps_nn_apply_field_prec(void* ctx, int out, int in, int prec, int dot, double* gamma)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    if (prec != 32 && prec != 64) return C->fail(PPS_ERR_ARG, "pps_nn_apply_field_prec: precision must be 32 or 64");
+    if (!C->nn) return C->fail(PPS_ERR_STATE, "no network attached (pps_nn_attach)");
+    if (dot >= 0 && !gamma) return C->fail(PPS_ERR_ARG, "pps_nn_apply_field_prec: dot field without gamma");
+    Field fo, fi, fd;
+    int rc = field_of(*C, out, fo);
+    if (!rc) rc = field_of(*C, in, fi);
+    if (!rc && dot >= 0) rc = field_of(*C, dot, fd);
+    if (rc) return rc;
+    if (out == in || out == dot) return C->fail(PPS_ERR_ARG, "pps_nn_apply_field_prec: out must differ from in and dot");
+    // ||in||^2 of the FP64 field stands in for the calling Krylov stage's, as in pps_nn_apply_field
+    Scal* S = C->scal(C->nscal - 1);
+    rc = scal_init(*C, S, 0.0, 0, 1.0, nullptr, 0);
+    if (rc) return rc;
+    launch_dot<double>(C->gsolver(64), C->stream, C->rb, fi.d(), fi.d(), &S->v[S_RR], nullptr);
+    const long long n = C->L64.total;
+    const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 65536);
+    Field ai = fi, ao = fo, ad = fd;
+    if (prec == 32) {
+        // FP32 copies, as the mixed-precision inner solve hands its fields to the network
+        if ((rc = C->get_work("nnapply.in", 32, ai)) || (rc = C->get_work("nnapply.out", 32, ao))) return rc;
+        if (dot >= 0 && (rc = C->get_work("nnapply.dot", 32, ad))) return rc;
+        hipLaunchKernelGGL(k_narrow, dim3(nb), dim3(256), 0, C->stream, fi.d(), ai.f(), n);
+        if (dot >= 0) hipLaunchKernelGGL(k_narrow, dim3(nb), dim3(256), 0, C->stream, fd.d(), ad.f(), n);
+    }
+    pps_stage st{};
+    st.kind = PPS_NN;
+    NNStage stage(*C, st, 1);
+    int nblocks = 0;
+    rc = stage.apply_dot(ao, ai, &S->done, dot >= 0 ? ad : Field{}, &nblocks);
+    if (rc) return rc;
+    if (prec == 32) hipLaunchKernelGGL(k_widen, dim3(nb), dim3(256), 0, C->stream, ao.f(), fo.d(), n);
+    if (dot < 0) return C->hip(hipStreamSynchronize(C->stream), "sync");
+    if (nblocks <= 0) return C->fail(PPS_ERR_STATE, "pps_nn_apply_field_prec: the output layer formed no dot partials");
+    std::vector<double> part((size_t)nblocks);
+    if ((rc = C->hip(hipMemcpyAsync(part.data(), C->rb.partials, sizeof(double) * nblocks, hipMemcpyDeviceToHost,
+                                    C->stream), "copy partials")))
+        return rc;
+    if ((rc = C->hip(hipStreamSynchronize(C->stream), "sync"))) return rc;
+    double g = 0.0;
+    for (double v : part) g += v;
+    *gamma = g;
+    return PPS_OK;
 }
 
 int pps_nn_time_layer(void* ctx, int layer, int prec, int reps, double* ms, double* bytes, double* flops)

@@ -104,6 +104,48 @@ def test_inference_path_matches_training_forward(net, dims):
     C.close()
 
 
+@pytest.mark.parametrize("dims", [(24, 24, 24), (31, 45, 20)])
+def test_fp32_fields_and_fused_gamma(net, dims):
+    """The network on FP32 fields (as the mixed-precision inner FCG runs it)
+    against the FP64-field path, and the output layer's fused
+    gamma = z . d (FCG's z . Ap_old) against a dot of its own output.
+
+    Random-sign input is exact in FP32 and has RMS 1, so both paths stage the
+    same BF16 input and compute every output value the same way in FP32: the
+    widened FP32 output must equal the FP64 path's output bit for bit.  gamma
+    sums the FP64 products of the output with the dot field (its FP32 copy
+    for the FP32 path) per block and then over blocks; against math.fsum only
+    the order differs: 1e-12 of the sum of |products|."""
+    import math
+    nx, ny, nz = dims
+    print(f"[nn] FP32 fields and fused gamma {dims}", flush=True)
+    net.set_params(_params(33))
+    C = L.Context((nx + 2, ny + 2, nz + 2), (1.0 / (nx + 1), 1.0 / (ny + 1), 1.0 / (nz + 1)))
+    net.attach(C)
+    rng = np.random.default_rng(34)
+    f = np.zeros(C.shape)
+    f[2:-2, 2:-2, 2:-2] = rng.choice([-1.0, 1.0], size=(nz, ny, nx))
+    C.upload(L.FIELD_T0, f)
+    d = rng.normal(size=C.shape)
+    C.upload(L.FIELD_T2, d)
+    inside = (slice(2, -2),) * 3
+    outs, gams = {}, {}
+    for prec, field in ((64, L.FIELD_T1), (32, L.FIELD_T3)):
+        C.fill(field, 0.0)
+        gams[prec] = C.nn_apply(field, L.FIELD_T0, prec=prec, dot=L.FIELD_T2)
+        outs[prec] = C.download(field)
+    assert np.all(outs[64][~np.pad(np.ones((nz, ny, nx), bool), 2)] == 0.0)
+    assert np.array_equal(outs[32], outs[64])
+    for prec in (64, 32):
+        dd = d if prec == 64 else d.astype(np.float32).astype(np.float64)
+        prod = (outs[prec][inside] * dd[inside]).ravel()
+        ref = math.fsum(prod)
+        scale = math.fsum(np.abs(prod))
+        print(f"[nn] prec {prec}: gamma {gams[prec]:.15e} vs {ref:.15e} (|.| sum {scale:.3e})", flush=True)
+        assert abs(gams[prec] - ref) <= 1e-12 * scale
+    C.close()
+
+
 def test_loss_gradient_matches_oracle(net):
     print("[nn] loss/grad 2x16^3", flush=True)
     p = _params(21)
