@@ -900,9 +900,10 @@ void launch_lift(const ConvArgs& a, hipStream_t s)
 // so the 16-channel level-0 activation never goes to HBM.  A block owns a
 // 14 x 18 output tile; its X region (tile + 1-voxel halo) is 16 x 20 voxels,
 // one 16-voxel x-row per MFMA group, 5 rows per wavefront.  Per plane s and row:
-//   1. lift: two MFMAs (K = 32 + 16 slots for the 27 taps, accumulator
-//      started at the bias, or at -1e30 outside the domain so that relu gives
-//      the zero padding); lane (voxel n, block j) then holds channels
+//   1. lift: two K = 32 MFMAs over quad pairs of the input ring (64 slots
+//      for the 27 taps, the rest with zero weights; accumulator started at
+//      the bias, or at -1e30 outside the domain so that relu gives the zero
+//      padding); lane (voxel n, block j) then holds channels
 //      4j .. 4j+3; relu + BF16 in packed 16-bit form.  With e = up(E1) of the
 //      same channels, (a, e) is lane (n, j)'s A operand X;
 //   2. row partials P[dz,dy][n] = sum_dx sum_ci W[dz,dy,dx][ci] (a + e)[n+dx-1][ci]:
