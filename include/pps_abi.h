@@ -228,6 +228,15 @@ int pps_nn_apply_field(void* ctx, int out, int in);
  * partials are added in block order on the host.  No reference counterpart:
  * the reference has no learned preconditioner. */
 int pps_nn_apply_field_prec(void* ctx, int out, int in, int prec, int dot, double* gamma);
+/* out = M(r_cur) with r_cur = r_prev - alpha * ap_prev formed by the
+ * network's first layer, as FCG with the learned preconditioner runs it (the
+ * residual update folded into the lift); r_cur is written on the solver
+ * region.  rr = ||r_cur||^2 over the solver region (in the solve it comes
+ * from the stage's own reduction).  prec 64: FP64 fields; prec 32: FP32
+ * copies of the fields, with out and r_cur widened back over the whole
+ * field.  No reference counterpart: the reference has no learned
+ * preconditioner. */
+int pps_nn_apply_update(void* ctx, int out, int r_prev, int ap_prev, int r_cur, double alpha, double rr, int prec);
 /* average device time (HIP events, context stream) of `reps` launches of
  * network layer `layer` (0..13) at the context's grid, input B in FP64 fields
  * (prec 64) or in FP32 copies as the mixed-precision inner solve uses them

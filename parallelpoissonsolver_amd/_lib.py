@@ -138,6 +138,8 @@ SIGNATURES = {
     "pps_nn_train": ([_VP, ctypes.POINTER(CNNTrainCfg), ctypes.POINTER(CNNTrainStats)], ctypes.c_int),
     "pps_nn_apply_field": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "pps_nn_apply_field_prec": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _DP], ctypes.c_int),
+    "pps_nn_apply_update": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                             ctypes.c_double, ctypes.c_int], ctypes.c_int),
     "pps_nn_time_layer": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP], ctypes.c_int),
     "pps_nn_attach": ([_VP, _VP], ctypes.c_int),
 }
@@ -299,6 +301,11 @@ class Context:
         self._chk(lib().pps_nn_apply_field_prec(self.h, out, inp, 64 if prec is None else prec,
                                                 -1 if dot is None else dot, ctypes.byref(g)))
         return None if dot is None else g.value
+
+    def nn_apply_update(self, out, r_prev, ap_prev, r_cur, alpha, rr, prec=64):
+        """out = M(r_cur), r_cur = r_prev - alpha * ap_prev formed inside the
+        network's first layer (FCG's fused residual update); rr = ||r_cur||^2."""
+        self._chk(lib().pps_nn_apply_update(self.h, out, r_prev, ap_prev, r_cur, float(alpha), float(rr), prec))
 
     def nn_time_layer(self, layer, reps=10, prec=32):
         """(ms per launch, algorithmic bytes, flops) of one layer of the attached network,

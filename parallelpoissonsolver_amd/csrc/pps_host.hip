@@ -2011,6 +2011,54 @@ int pps_nn_apply_field_prec(void* ctx, int out, int in, int prec, int dot, doubl
     return PPS_OK;
 }
 
+int pps_nn_apply_update(void* ctx, int out, int r_prev, int ap_prev, int r_cur, double alpha, double rr, int prec)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    if (prec != 32 && prec != 64) return C->fail(PPS_ERR_ARG, "pps_nn_apply_update: precision must be 32 or 64");
+    if (!C->nn) return C->fail(PPS_ERR_STATE, "no network attached (pps_nn_attach)");
+    if (!(rr > 0.0)) return C->fail(PPS_ERR_ARG, "pps_nn_apply_update: rr must be positive");
+    Field fo, fp, fa, fc;
+    int rc = field_of(*C, out, fo);
+    if (!rc) rc = field_of(*C, r_prev, fp);
+    if (!rc) rc = field_of(*C, ap_prev, fa);
+    if (!rc) rc = field_of(*C, r_cur, fc);
+    if (rc) return rc;
+    if (out == r_prev || out == ap_prev || out == r_cur || r_cur == r_prev || r_cur == ap_prev)
+        return C->fail(PPS_ERR_ARG, "pps_nn_apply_update: out and r_cur must differ from the other fields");
+    // the FCG stage's scalars: step length alpha and ||r_cur||^2 (which the
+    // stage's own reduction supplies in the solve), no restart
+    Scal* S = C->scal(C->nscal - 1);
+    rc = scal_init(*C, S, 0.0, 0, 1.0, nullptr, 0);
+    if (rc) return rc;
+    C->scal_h->v[S_ALPHA] = alpha;
+    C->scal_h->v[S_RR] = rr;
+    if ((rc = C->hip(hipMemcpyAsync(S, C->scal_h, sizeof(Scal), hipMemcpyHostToDevice, C->stream), "scal")))
+        return rc;
+    const long long n = C->L64.total;
+    const unsigned nb = (unsigned)std::min<long long>((n + 255) / 256, 65536);
+    Field ao = fo, ap = fp, aa = fa, ac = fc;
+    if (prec == 32) {
+        if ((rc = C->get_work("nnupd.out", 32, ao)) || (rc = C->get_work("nnupd.rp", 32, ap)) ||
+            (rc = C->get_work("nnupd.ap", 32, aa)) || (rc = C->get_work("nnupd.rc", 32, ac)))
+            return rc;
+        hipLaunchKernelGGL(k_narrow, dim3(nb), dim3(256), 0, C->stream, fp.d(), ap.f(), n);
+        hipLaunchKernelGGL(k_narrow, dim3(nb), dim3(256), 0, C->stream, fa.d(), aa.f(), n);
+        hipLaunchKernelGGL(k_narrow, dim3(nb), dim3(256), 0, C->stream, fc.d(), ac.f(), n);
+    }
+    pps_stage st{};
+    st.kind = PPS_NN;
+    NNStage stage(*C, st, 1);
+    int nblocks = 0;
+    rc = stage.apply_update(ao, ac, ap, aa, &S->v[S_ALPHA], &S->restart, &S->done, Field{}, &nblocks);
+    if (rc) return rc;
+    if (prec == 32) {
+        hipLaunchKernelGGL(k_widen, dim3(nb), dim3(256), 0, C->stream, ao.f(), fo.d(), n);
+        hipLaunchKernelGGL(k_widen, dim3(nb), dim3(256), 0, C->stream, ac.f(), fc.d(), n);
+    }
+    return C->hip(hipStreamSynchronize(C->stream), "sync");
+}
+
 int pps_nn_time_layer(void* ctx, int layer, int prec, int reps, double* ms, double* bytes, double* flops)
 {
     Ctx* C = as_ctx(ctx);

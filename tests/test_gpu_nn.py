@@ -146,6 +146,52 @@ def test_fp32_fields_and_fused_gamma(net, dims):
     C.close()
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+def test_fused_residual_update(net, prec):
+    """FCG's residual update folded into the network's first layer:
+    r_cur = r_prev - alpha * Ap_prev is written by the lift (on the solver
+    region only) and the output equals the plain apply on that r_cur, bit for
+    bit (both stage the same values; the scale comes from the same ||r_cur||^2
+    up to summation order, which moves it by ~1e-16).  r_cur against numpy in
+    the field's precision: 1 ulp (the kernel may fuse the multiply-add)."""
+    nx, ny, nz = 31, 45, 20
+    print(f"[nn] fused residual update, prec {prec}", flush=True)
+    net.set_params(_params(35))
+    C = L.Context((nx + 2, ny + 2, nz + 2), (1.0 / (nx + 1), 1.0 / (ny + 1), 1.0 / (nz + 1)))
+    net.attach(C)
+    rng = np.random.default_rng(36)
+    inside = (slice(2, -2),) * 3
+    rp = np.zeros(C.shape)
+    rp[inside] = rng.normal(size=(nz, ny, nx))
+    ap = rng.normal(size=C.shape)
+    alpha = 0.37
+    if prec == 32:
+        rp, ap = rp.astype(np.float32).astype(np.float64), ap.astype(np.float32).astype(np.float64)
+        ref = (np.float32(rp) - np.float32(alpha) * np.float32(ap)).astype(np.float64)
+        ulp = 2.0 ** -23
+    else:
+        ref = rp - alpha * ap
+        ulp = 2.0 ** -52
+    C.upload(L.FIELD_T0, rp)
+    C.upload(L.FIELD_T2, ap)
+    C.fill(L.FIELD_X, 0.0)
+    C.fill(L.FIELD_T1, 0.0)
+    rr = float(np.sum(ref[inside] ** 2))
+    C.nn_apply_update(L.FIELD_T1, L.FIELD_T0, L.FIELD_T2, L.FIELD_X, alpha, rr, prec=prec)
+    rc, out = C.download(L.FIELD_X), C.download(L.FIELD_T1)
+    outside = ~np.pad(np.ones((nz, ny, nx), bool), 2)
+    assert np.all(rc[outside] == 0.0) and np.all(out[outside] == 0.0)
+    err = np.abs(rc[inside] - ref[inside])
+    assert np.all(err <= ulp * (np.abs(ref[inside]) + alpha * np.abs(ap[inside]))), float(err.max())
+    C.fill(L.FIELD_T3, 0.0)
+    C.nn_apply(L.FIELD_T3, L.FIELD_X, prec=prec)
+    plain = C.download(L.FIELD_T3)
+    print(f"[nn] r_cur max err {err.max():.3e}; fused vs plain output max diff "
+          f"{np.abs(out - plain).max():.3e}", flush=True)
+    assert np.array_equal(out, plain)
+    C.close()
+
+
 def test_loss_gradient_matches_oracle(net):
     print("[nn] loss/grad 2x16^3", flush=True)
     p = _params(21)
