@@ -14,7 +14,9 @@ form is timed over K steps after W warm-up steps, bracketed by device
 synchronisation.  Forms:
   - ir_nnfcg (headline): FP64 outer residual/update refinement around FCG in
     FP32 preconditioned by the learned network (BF16 MFMA convolutions), inner
-    tolerance 1e-4;
+    tolerance 1e-4, 2e-3 for the first outer step: the first FP32 correction
+    leaves a true residual of about 2e-3 (kappa * eps32 at 512^3 is ~6e-3)
+    however far the inner solve goes;
   - nn_fcg_fp64: the same network under FP64 FCG;
   - plain_fp64_cg: the reference method.
 The network is the committed one (parallelpoissonsolver_amd/weights/nn_64.npy,
@@ -60,7 +62,8 @@ def chains():
         "ir_fp32cg_fp64": [dict(kind=L.IR, tol=TOL, maxit=100), dict(kind=L.CG, precision=32, tol=1e-5, maxit=200000),
                            dict(kind=L.NONE, precision=32)],
         "nn_fcg_fp64": [dict(kind=L.FCG, tol=TOL, maxit=20000), dict(kind=L.NN, stall_window=50, stall_ratio=0.9)],
-        "ir_nnfcg": [dict(kind=L.IR, tol=TOL, maxit=100), dict(kind=L.FCG, precision=32, tol=1e-4, maxit=20000),
+        "ir_nnfcg": [dict(kind=L.IR, tol=TOL, maxit=100, first_tol=2e-3),
+                     dict(kind=L.FCG, precision=32, tol=1e-4, maxit=20000),
                      dict(kind=L.NN, precision=32, stall_window=50, stall_ratio=0.9)],
     }
 

@@ -91,6 +91,10 @@ typedef struct pps_stage {
     int cheb_local;      /* alpaka `local` eigenvalue variant                       */
     int stall_window;    /* FCG over PPS_NN: iterations per stall test (0 = never fall back); set on either stage */
     double stall_ratio;  /* fall back to Jacobi when the residual drop over the window is above this factor */
+    double ir_first_tol; /* PPS_IR: the inner stage's tolerance for the first outer step (0: its own tol).  The
+                            first FP32 correction cannot bring the true residual much below kappa * eps32, so a
+                            looser first inner solve loses little; no reference counterpart (the reference has
+                            no refinement) */
 } pps_stage;
 
 typedef struct pps_result {

@@ -47,6 +47,7 @@ class Stage(ctypes.Structure):
         ("cheb_local", ctypes.c_int),
         ("user_fn", USER_FN),
         ("user", ctypes.c_void_p),
+        ("ir_first_tol", ctypes.c_double),
     ]
 
 
@@ -201,6 +202,7 @@ class OracleGrid:
             st[i].cheb_rmin = c.get("cheb_rmin", 500.0)
             st[i].cheb_rmax = c.get("cheb_rmax", 1 - 1e-4)
             st[i].cheb_local = c.get("cheb_local", 0)
+            st[i].ir_first_tol = c.get("first_tol", 0.0)
             if "user_fn" in c:
                 fn = USER_FN(c["user_fn"])
                 keep.append(fn)

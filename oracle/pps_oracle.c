@@ -279,6 +279,7 @@ typedef struct og_state {
     int iters;
     long prec_iters;
     double err_it, err_op;
+    double tol_override; /* > 0: tolerance of the next FP32 inner solve (set by og_ir) */
 } og_state;
 
 typedef struct og_run {
@@ -628,7 +629,7 @@ static void og_cg_f(og_run* R, int s, float* x, float* b, int flexible)
 {
     const og_grid* g = R->g;
     og_state* S = &R->st[s];
-    const double tol = R->chain[s].tol;
+    const double tol = S->tol_override > 0.0 ? S->tol_override : R->chain[s].tol;
     const int maxit = R->chain[s].maxit;
     const size_t nb = sizeof(float) * (size_t)g->ntot;
     OG_ENSUREF(S->fp, g); OG_ENSUREF(S->fr, g); OG_ENSUREF(S->fAp, g); OG_ENSUREF(S->fz, g);
@@ -781,6 +782,7 @@ static void og_ir(og_run* R, int s, double* x, double* b)
     while (outer < maxouter && err >= tol) {
         memset(SI->fin, 0, sizeof(float) * g->ntot);
         OG_FOR_SOLVER(g, c) SI->fin[c] = (float)S->r[c];
+        SI->tol_override = outer == 0 ? R->chain[s].ir_first_tol : 0.0;
         og_cg_f(R, s + 1, SI->fout, SI->fin, R->chain[s + 1].kind == OG_FCG);
         inner_total += SI->iters;
         OG_FOR_SOLVER(g, c) x[c] += (double)SI->fout[c];

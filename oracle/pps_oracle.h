@@ -67,6 +67,7 @@ typedef struct og_stage {
     int cheb_local;                        /* 1: local eigenvalues, no rescale (alpaka)   */
     og_user_fn user_fn;                    /* OG_USER */
     void* user;
+    double ir_first_tol;                   /* OG_IR: inner tolerance of the first outer step (0: the inner's) */
 } og_stage;
 
 typedef struct og_result {

@@ -52,6 +52,7 @@ class CStage(ctypes.Structure):
         ("cheb_local", ctypes.c_int),
         ("stall_window", ctypes.c_int),
         ("stall_ratio", ctypes.c_double),
+        ("ir_first_tol", ctypes.c_double),
     ]
 
 
@@ -244,6 +245,7 @@ class Context:
             st[i].cheb_local = c.get("cheb_local", 0)
             st[i].stall_window = c.get("stall_window", 0)
             st[i].stall_ratio = c.get("stall_ratio", 0.0)
+            st[i].ir_first_tol = c.get("first_tol", 0.0)
         res = CResult()
         self._chk(lib().pps_solve(self.h, st, len(chain), ctypes.byref(res)))
         out = {k: getattr(res, k) for k, _ in CResult._fields_}

@@ -1003,7 +1003,8 @@ struct IRStage : KrylovStage {
         scale_box(C, b, S, 0);
         C.ghost(x, true, S, nullptr);
         const Geo g64 = C.gsolver(64);
-        launch_ir_residual(g64, C.stream, C.rb, x.d(), b.d(), r32.f(), S, SI);
+        const double itol = pre->cfg.tol, itol0 = cfg.ir_first_tol > 0.0 ? cfg.ir_first_tol : itol;
+        launch_ir_residual(g64, C.stream, C.rb, x.d(), b.d(), r32.f(), S, SI, itol, itol0);
         Scal h;
         rc = scal_read(C, S, h);
         if (rc) return rc;
@@ -1014,7 +1015,7 @@ struct IRStage : KrylovStage {
             launch_ir_update(g64, C.stream, x.d(), d.f(), SI);
             C.ghost(x, true, S, nullptr);
             scal_op(C, S, SC_OUTER_INC);
-            launch_ir_residual(g64, C.stream, C.rb, x.d(), b.d(), r32.f(), S, SI);
+            launch_ir_residual(g64, C.stream, C.rb, x.d(), b.d(), r32.f(), S, SI, itol, itol0);
             outer_steps++;
             rc = scal_read(C, S, h);
             if (rc) return rc;

@@ -1471,6 +1471,7 @@ struct OpIrRes : Gate {
     float* __restrict__ r32;
     Scal* So;
     Scal* SI;
+    double itol, itol0;  // inner tolerance: first outer step, later ones
     __device__ void load() {}
     struct Ox { double b; };
     __device__ double in(long long c) const { return x[c]; }
@@ -1493,11 +1494,12 @@ struct OpIrRes : Gate {
         if (o == 0) So->v[S_ERR0] = e;
         So->done = e < So->v[S_TOL] ? 1 : 0;
         SI->v[S_INNER_NORM] = sqrt(t[1]);
+        SI->v[S_TOL] = o == 0 ? itol0 : itol;
     }
 };
 
 void launch_ir_residual(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, float* r32, Scal* So,
-                        Scal* SI)
+                        Scal* SI, double itol, double itol0)
 {
     OpIrRes op;
     op.gate = nullptr;
@@ -1506,6 +1508,8 @@ void launch_ir_residual(const Geo& g, hipStream_t s, RedBuf rb, const double* x,
     op.r32 = r32;
     op.So = So;
     op.SI = SI;
+    op.itol = itol;
+    op.itol0 = itol0;
     run_stencil<double, 2>(g, s, op, rb);
 }
 

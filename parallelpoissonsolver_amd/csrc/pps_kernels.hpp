@@ -211,8 +211,11 @@ void launch_fcg_update(const Geo& g, hipStream_t s, RedBuf rb, T* r, const T* Ap
 // ---- mixed-precision iterative refinement ----
 // r = b - A x (FP64), r32 = (float) r; So: S_ERR = ||r||, history[outer];
 // SI: S_INNER_NORM = ||r32||
+// r32 = b - A x (FP64, narrowed), the outer residual norm and convergence
+// test; sets the inner stage's tolerance to itol0 before the first inner
+// solve and to itol before the later ones
 void launch_ir_residual(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, float* r32, Scal* So,
-                        Scal* SI);
+                        Scal* SI, double itol, double itol0);
 // x += ||r32|| * d on the solver region (the inner solve's un-normalisation)
 void launch_ir_update(const Geo& g, hipStream_t s, double* x, const float* d, const Scal* SI);
 // inner start: r = r32 / ||r32||, x = 0, p = 0, err0, rz (pointwise jac) -> SI

@@ -295,6 +295,31 @@ def test_ir_inner_fp32_cg_matches_oracle_structure():
     assert res["error_compute_operator"] < 1e-10 and ro["error_compute_operator"] < 1e-10
 
 
+def test_ir_first_step_tolerance_matches_oracle():
+    """ir_first_tol (a looser inner tolerance for the first outer step) on the
+    GPU follows the oracle: same outer step count, inner counts within 2 %,
+    both first outer residuals near the looser tolerance, and the FP64 true
+    residual at the outer tolerance."""
+    n = 34
+    G = O.OracleGrid((n,) * 3, (1 / (n - 1),) * 3, (0, 0, 0), (0,) * 6)
+    b = G.rhs("gauss", 3)
+    x = G.zeros()
+    C = ctx_for(G, (0,) * 6)
+    C.upload(L.FIELD_X, x)
+    C.upload(L.FIELD_B, b)
+    res = C.solve([dict(kind=L.IR, tol=1e-10, maxit=50, first_tol=1e-2),
+                   dict(kind=L.CG, precision=32, tol=1e-5, maxit=5000), dict(kind=L.NONE, precision=32)])
+    xo, bo = x.copy(), b.copy()
+    ro = G.solve([dict(kind=O.IR, tol=1e-10, maxit=50, first_tol=1e-2), dict(kind=O.CG, tol=1e-5, maxit=5000),
+                  dict(kind=O.NONE)], xo, bo)
+    print(f"[ir] GPU {res['iterations']} inner ({res['outer_iterations']} outer), history {list(res['history'])}; "
+          f"oracle {ro['iterations']} ({ro['outer_iterations']}), {list(ro['history'])}", flush=True)
+    assert res["outer_iterations"] == ro["outer_iterations"]
+    assert abs(res["iterations"] - ro["iterations"]) <= max(2, round(0.02 * ro["iterations"]))
+    assert 1e-3 < res["history"][1] < 2e-2 and 1e-3 < ro["history"][1] < 2e-2
+    assert res["error_compute_operator"] < 1e-10 and ro["error_compute_operator"] < 1e-10
+
+
 def test_chebyshev_fp32_hook_matches_fp64_preconditioner():
     """T_data_chebyshev = float (chebyshevIterationAlpaka.hpp:149-152): the FP32
     preconditioner changes iterates only slightly and still converges."""
