@@ -317,14 +317,19 @@ public:
 };
 
 // FP64 outer refinement r = b - A u, u += d around an FP32 inner solver type
-// (BaseCG or FCG with T_data = float); no reference counterpart
-template <int DIM, int tolerance, int maxOuter, typename T_Inner>
-class MixedPrecisionIR : public SolverFacade<MixedPrecisionIR<DIM, tolerance, maxOuter, T_Inner>, DIM, double> {
+// (BaseCG or FCG with T_data = float); no reference counterpart.
+// firstInnerTolMicro > 0: the inner tolerance of the first outer step, in
+// units of 1e-6 (pps_stage::ir_first_tol)
+template <int DIM, int tolerance, int maxOuter, typename T_Inner, int firstInnerTolMicro = 0>
+class MixedPrecisionIR
+    : public SolverFacade<MixedPrecisionIR<DIM, tolerance, maxOuter, T_Inner, firstInnerTolMicro>, DIM, double> {
 public:
     using SolverFacade<MixedPrecisionIR, DIM, double>::SolverFacade;
     static void stages(std::vector<pps_stage>& v)
     {
-        v.push_back(detail::stage(PPS_IR, 64, tolerance * tollScalingFactor, maxOuter));
+        pps_stage s = detail::stage(PPS_IR, 64, tolerance * tollScalingFactor, maxOuter);
+        s.ir_first_tol = firstInnerTolMicro * 1e-6;
+        v.push_back(s);
         T_Inner::stages(v);
     }
 };

@@ -35,6 +35,9 @@ using T_NoneSolver = NoneSolver<DIM, T_data, 1, 1>;
 using T_Preconditioner2 = ChebyshevIteration<DIM, T_data, 1, 11, false, false, T_NoneSolver>;
 using T_SolverBiCG = BiCGSTAB<DIM, T_data, 1, 1000, true, true, T_Preconditioner2>;
 using T_SolverCG = BaseCG<DIM, T_data, 1, 5000, true, true, T_NoneSolver>;
+// FP64 refinement around FP32 CG to 1e-5, the first inner solve to 1e-2
+using T_InnerIR = BaseCG<DIM, float, 100000, 5000, false, false, NoneSolver<DIM, float, 1, 1>>;
+using T_SolverIR = MixedPrecisionIR<DIM, 1, 50, T_InnerIR, 10000>;
 
 template <class T_Solver>
 static int run(const BlockGrid<DIM, T_data>& blockGrid, const char* out)
@@ -60,7 +63,7 @@ static int run(const BlockGrid<DIM, T_data>& blockGrid, const char* out)
 int main(int argc, char** argv)
 {
     if (argc != 10) {
-        std::fprintf(stderr, "usage: %s n cg|bicgstab bc*6 xout\n", argv[0]);
+        std::fprintf(stderr, "usage: %s n cg|bicgstab|ir bc*6 xout\n", argv[0]);
         return 2;
     }
     const int n = std::atoi(argv[1]);
@@ -70,5 +73,6 @@ int main(int argc, char** argv)
     BlockGrid<DIM, T_data> blockGrid({1, 1, 1}, 0, {n, n, n}, {ds, ds, ds}, {0.5, 0.25, 1.0}, {1, 1, 1}, bcs,
                                      {0, 0, 0, 0, 0, 0});
     if (std::strcmp(argv[2], "cg") == 0) return run<T_SolverCG>(blockGrid, argv[9]);
+    if (std::strcmp(argv[2], "ir") == 0) return run<T_SolverIR>(blockGrid, argv[9]);
     return run<T_SolverBiCG>(blockGrid, argv[9]);
 }

@@ -72,3 +72,18 @@ def test_cpp_facade_matches_python_mirror(kind):
     assert res["error_compute_operator"] < 1e-10
     assert abs(res["error_compute_operator"] - solver.getErrorComputeOperator()) <= 1e-3 * solver.getErrorComputeOperator()
     assert np.max(np.abs(xc - x)) <= 1e-12 * np.max(np.abs(x))
+
+
+@pytest.mark.gpu
+def test_cpp_facade_mixed_precision_ir():
+    """MixedPrecisionIR with a first-step inner tolerance, driven through the
+    C++ facade: FP64 refinement around FP32 CG reaches the FP64 tolerance."""
+    n = 18
+    print(f"[facade] ir {n}^3", flush=True)
+    with tempfile.TemporaryDirectory() as tmp:
+        out = os.path.join(tmp, "x.bin")
+        p = subprocess.run([EXE, str(n), "ir", *(["0"] * 6), out], capture_output=True, text=True, timeout=120)
+        assert p.returncode == 0, p.stderr
+        res = json.loads(p.stdout.strip().splitlines()[-1])
+    assert res["iterations"] > 0
+    assert res["error_compute_operator"] < 1e-10
