@@ -77,13 +77,27 @@ def test_cpp_facade_matches_python_mirror(kind):
 @pytest.mark.gpu
 def test_cpp_facade_mixed_precision_ir():
     """MixedPrecisionIR with a first-step inner tolerance, driven through the
-    C++ facade: FP64 refinement around FP32 CG reaches the FP64 tolerance."""
+    C++ facade and through the Python mirror: the same chain, so the same
+    inner iteration count and solution; FP64 refinement around FP32 CG
+    reaches the FP64 tolerance."""
+    import parallelpoissonsolver_amd as pps
     n = 18
+    bcs = (0,) * 6
     print(f"[facade] ir {n}^3", flush=True)
     with tempfile.TemporaryDirectory() as tmp:
         out = os.path.join(tmp, "x.bin")
-        p = subprocess.run([EXE, str(n), "ir", *(["0"] * 6), out], capture_output=True, text=True, timeout=120)
+        p = subprocess.run([EXE, str(n), "ir", *map(str, bcs), out], capture_output=True, text=True, timeout=120)
         assert p.returncode == 0, p.stderr
         res = json.loads(p.stdout.strip().splitlines()[-1])
-    assert res["iterations"] > 0
+        xc = np.fromfile(out, dtype=np.float64)
+    ds = 1.0 / (n - 1)
+    grid = pps.BlockGrid((1, 1, 1), 0, (n, n, n), (ds, ds, ds), (0.5, 0.25, 1.0), (1, 1, 1), bcs)
+    inner = pps.BaseCG(3, np.float32, 100000, 5000, False, False, pps.NoneSolver(3, np.float32, 1, 1))
+    solver = pps.MixedPrecisionIR(1, 50, inner, first_inner_tol=1e-2)(grid, SineProblem(), pps.CommunicatorMPI(grid))
+    x = np.zeros(grid.getNtotLocalGuards())
+    b = np.zeros(grid.getNtotLocalGuards())
+    solver.setProblem(x, b)
+    solver(x, b)
+    assert res["iterations"] > 0 and res["iterations"] == solver.getNumIterationFinal()
     assert res["error_compute_operator"] < 1e-10
+    assert np.max(np.abs(xc - x)) <= 1e-12 * np.max(np.abs(x))
