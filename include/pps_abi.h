@@ -95,6 +95,12 @@ typedef struct pps_stage {
                             first FP32 correction cannot bring the true residual much below kappa * eps32, so a
                             looser first inner solve loses little; no reference counterpart (the reference has
                             no refinement) */
+    int true_finish;     /* CG, BiCGSTAB, FCG as the main solver: 0 stops on the recursive residual, the
+                            reference's rule (baseCG.hpp:224, BiCGSTAB.hpp:290).  n > 0: at that stop r = b - A x
+                            is recomputed in FP64; while its norm is at or above tol (and iterations remain), r
+                            is replaced by it and the loop resumes (CG, FCG keep their search direction,
+                            BiCGSTAB restarts with r0 = r), at most n times.  The recursive residual drifts from
+                            the true one as iterations accumulate; this makes the true residual reach tol. */
 } pps_stage;
 
 typedef struct pps_result {
@@ -129,6 +135,14 @@ int pps_set_neumann_flux(void* ctx, int face, const double* host_plane);
 
 /* ---- the solve (BaseCG / BiCGSTAB / ChebyshevIteration operator()) ------ */
 int pps_solve(void* ctx, const pps_stage* chain, int nstages, pps_result* res);
+/* one preconditioner application out = M(in): the preconditioner protocol
+ * void operator()(Buf& out, Buf& in) (BiCGstabAlpaka.hpp:213-214,
+ * chebyshevIterationAlpaka.hpp:119; CPU form BiCGSTAB.hpp:170,190).  chain[0]
+ * is applied once (None, Jacobi, CG, BiCGSTAB, Chebyshev or FCG; Chebyshev
+ * with precision 32 is the T_data_chebyshev = float form), chain[1..] nest as
+ * in pps_solve; `in` has its Neumann ghosts reset in place, as the reference
+ * preconditioners do.  *iterations (may be NULL) = iterations spent. */
+int pps_precondition(void* ctx, const pps_stage* chain, int nstages, int out, int in, long long* iterations);
 /* residual history of the last solve (errorFromIterationHistory_); returns the
  * number of entries copied (<= cap) or a negative error */
 int pps_get_history(void* ctx, double* out, int cap);

@@ -48,6 +48,7 @@ class Stage(ctypes.Structure):
         ("user_fn", USER_FN),
         ("user", ctypes.c_void_p),
         ("ir_first_tol", ctypes.c_double),
+        ("precision", ctypes.c_int),
     ]
 
 
@@ -90,6 +91,7 @@ def lib():
         L.og_residual.argtypes = [gp, dp, dp, dp]
         L.og_residual.restype = ctypes.c_double
         L.og_solve.argtypes = [gp, ctypes.POINTER(Stage), ctypes.c_int, dp, dp, ctypes.c_void_p, ctypes.POINTER(Result)]
+        L.og_precondition.argtypes = [gp, ctypes.POINTER(Stage), ctypes.c_int, dp, dp, ctypes.POINTER(Result)]
         L.og_rhs_normal.argtypes = [gp, ctypes.c_uint64, dp]
         L.og_rhs_gauss.argtypes = [gp, ctypes.c_uint64, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp]
         L.og_rhs_spectrum.argtypes = [gp, ctypes.c_uint64, ctypes.c_int, dp]
@@ -190,10 +192,9 @@ class OracleGrid:
             raise ValueError(kind)
         return b
 
-    def solve(self, chain, x, b, flux=None, history_cap=200000):
-        """chain: list of dicts {kind, tol, maxit, cheb_eps, cheb_rmin, cheb_rmax, cheb_local, user_fn}."""
+    @staticmethod
+    def _stages(chain, keep):
         st = (Stage * len(chain))()
-        keep = []
         for i, c in enumerate(chain):
             st[i].kind = c["kind"]
             st[i].tol = c.get("tol", 1e-10)
@@ -203,10 +204,29 @@ class OracleGrid:
             st[i].cheb_rmax = c.get("cheb_rmax", 1 - 1e-4)
             st[i].cheb_local = c.get("cheb_local", 0)
             st[i].ir_first_tol = c.get("first_tol", 0.0)
+            st[i].precision = c.get("precision", 64)
             if "user_fn" in c:
                 fn = USER_FN(c["user_fn"])
                 keep.append(fn)
                 st[i].user_fn = fn
+        return st
+
+    def precondition(self, chain, inp):
+        """out = M(inp) for the preconditioner chain (chain[0] applied, chain[1..]
+        nested); `inp` gets its Neumann ghosts reset in place."""
+        keep = []
+        st = self._stages(chain, keep)
+        out = self.zeros()
+        res = Result()
+        rc = lib().og_precondition(ctypes.byref(self.g), st, len(chain), _dp(out), _dp(inp), ctypes.byref(res))
+        if rc != 0:
+            raise ValueError(f"og_precondition failed {rc}")
+        return out, {"iterations": res.iterations, "precond_iterations": res.precond_iterations}
+
+    def solve(self, chain, x, b, flux=None, history_cap=200000):
+        """chain: list of dicts {kind, tol, maxit, cheb_eps, cheb_rmin, cheb_rmax, cheb_local, precision, user_fn}."""
+        keep = []
+        st = self._stages(chain, keep)
         hist = np.zeros(history_cap)
         res = Result()
         res.history = _dp(hist)

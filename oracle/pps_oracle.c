@@ -735,6 +735,80 @@ static void og_cheb_f(og_run* R, int s, float* X, float* B)
     S->prec_iters += m;
 }
 
+/* The alpaka Chebyshev preconditioner with T_data_chebyshev = float under an
+ * FP64 caller (chebyshevIterationAlpaka.hpp:119-196, 291-296): B's Neumann
+ * ghosts reset in FP64 (:127), CastPrecisionFieldKernel over the whole box
+ * (kernelsAlpakaChebyshev.hpp:7-18), Chebyshev1Kernel (:115-184) and m-1
+ * sweeps of Chebyshev2Kernel (:191-273, jumpCheb = 0) in FP32 with the
+ * coefficients and the rho recurrence evaluated in FP32 as the members and
+ * kernel locals are (chebyshevIterationAlpaka.hpp:23-30,123-124,157-158,
+ * 597-606), Y's ghosts reset in FP32 before each sweep, X = -1 * W on the
+ * solver region (AssignFieldWith1FieldKernel, :23-44).  delta keeps the alpaka
+ * sign (positive); ds[0] is x here where the alpaka Vec is z-y-x. */
+static void og_cheb_alp32(og_run* R, int s, double* X, double* B64)
+{
+    const og_grid* g = R->g;
+    og_state* S = &R->st[s];
+    const og_stage* st = &R->chain[s];
+    const int m = st->maxit;
+    float theta, delta;
+    if (st->cheb_local) {
+        theta = (float)((g->eig_local[0] + g->eig_local[1]) * 0.5);
+        delta = (float)((g->eig_local[1] - g->eig_local[0]) * 0.5);
+    } else {
+        theta = (float)((g->eig_global[0] * st->cheb_rmin + g->eig_global[1] * st->cheb_rmax) * 0.5 * (1.0 + st->cheb_eps));
+        delta = (float)((g->eig_global[1] * st->cheb_rmax - g->eig_global[0] * st->cheb_rmin) * 0.5);
+    }
+    const float sigma = theta / delta;
+    OG_ENSUREF(S->fin, g); OG_ENSUREF(S->fY, g); OG_ENSUREF(S->fZ, g); OG_ENSUREF(S->fW, g);
+    og_reset_neumann(g, B64, NULL, 1.0);
+    float* B = S->fin;
+    for (long i = 0; i < g->ntot; ++i) B[i] = (float)B64[i];
+    const long sj = g->sj, sk = g->sk;
+    const float r0 = (float)(g->ds[0] * g->ds[0]), r1 = (float)(g->ds[1] * g->ds[1]), r2 = (float)(g->ds[2] * g->ds[2]);
+    float rhoOld = 1 / sigma;
+    float rhoCurr = 1 / (2 * sigma - rhoOld);
+    {
+        const float f0 = 2 * rhoCurr / delta * (1 / r0 / theta);
+        const float f1 = 2 * rhoCurr / delta * (1 / r1 / theta);
+        const float f2 = 2 * rhoCurr / delta * (1 / r2 / theta);
+        const float fc0 = 2 * rhoCurr / delta * (2 - 2 * (1 / r0 + 1 / r1 + 1 / r2) / theta);
+        OG_FOR_SOLVER(g, c)
+        {
+            S->fZ[c] = B[c] / theta;
+            float tmp = B[c] * fc0 + (B[c - 1] + B[c + 1]) * f0;
+            tmp += (B[c - sj] + B[c + sj]) * f1 + (B[c - sk] + B[c + sk]) * f2;
+            S->fY[c] = tmp;
+        }
+    }
+    for (int it = 2; it <= m; ++it) {
+        rhoOld = rhoCurr;
+        rhoCurr = 1 / (2 * sigma - rhoOld);
+        og_reset_neumann_f(g, S->fY);
+        const float f0 = rhoCurr * 2 / delta * (1 / r0);
+        const float f1 = rhoCurr * 2 / delta * (1 / r1);
+        const float f2 = rhoCurr * 2 / delta * (1 / r2);
+        const float fB = rhoCurr * 2 / delta;
+        const float fZ = -rhoCurr * rhoOld;
+        const float fc0 = rhoCurr * (2 * sigma - 4 / delta * (1 / r0 + 1 / r1 + 1 / r2));
+        const float *Y = S->fY, *Z = S->fZ;
+        float* W = S->fW;
+        OG_FOR_SOLVER(g, c)
+        {
+            float tmp = Y[c] * fc0 + (Y[c - 1] + Y[c + 1]) * f0;
+            tmp += (Y[c - sj] + Y[c + sj]) * f1 + (Y[c - sk] + Y[c + sk]) * f2;
+            tmp += B[c] * fB;
+            tmp += Z[c] * fZ;
+            W[c] = tmp;
+        }
+        float* t = S->fZ; S->fZ = S->fY; S->fY = t; /* swap(bufZ, bufY) */
+        t = S->fW; S->fW = S->fY; S->fY = t;        /* swap(bufW, bufY) */
+    }
+    const float minus1 = -1.0f;
+    OG_FOR_SOLVER(g, c) X[c] = (double)(minus1 * S->fW[c]);
+    S->prec_iters += m;
+}
+
 static void apply_stage_f(og_run* R, int s, float* out, float* in)
 {
     const og_grid* g = R->g;
@@ -812,7 +886,10 @@ static void apply_stage(og_run* R, int s, double* out, double* in)
     }
     case OG_CG: og_cg(R, s, 0, out, in); S->prec_iters += S->iters; break;
     case OG_BICGSTAB: og_bicgstab(R, s, 0, out, in); S->prec_iters += S->iters; break;
-    case OG_CHEB: og_cheb(R, s, 0, out, in); break;
+    case OG_CHEB:
+        if (R->chain[s].precision == 32) og_cheb_alp32(R, s, out, in);
+        else og_cheb(R, s, 0, out, in);
+        break;
     case OG_FCG: og_fcg(R, s, 0, out, in); S->prec_iters += S->iters; break;
     case OG_USER: {
         OG_ENSUREF(S->fin, g); OG_ENSUREF(S->fout, g);
@@ -865,6 +942,30 @@ int og_solve(const og_grid* g, const og_stage* chain, int nstages, double* x, do
         res->error_compute_operator = st[0].err_op;
         long pi = 0;
         for (int s = 1; s < nstages; ++s) pi += st[s].prec_iters;
+        res->precond_iterations = pi;
+    }
+    for (int s = 0; s < 9; ++s) free_state(&st[s]);
+    return 0;
+}
+
+/* One preconditioner application out = M(in) of chain[0] with chain[1..]
+ * nested: the solvers' preconditioner protocol operator()(out, in)
+ * (BiCGstabAlpaka.hpp:213-214; BiCGSTAB.hpp:170,190).  `in` has its Neumann
+ * ghosts reset in place, as the reference's preconditioners do. */
+int og_precondition(const og_grid* g, const og_stage* chain, int nstages, double* out, double* in, og_result* res)
+{
+    if (nstages < 1 || nstages > 8) return -1;
+    if (chain[0].kind == OG_IR) return -2;
+    og_state st[9];
+    memset(st, 0, sizeof st);
+    for (int s = 0; s < 9; ++s) st[s].norm = 1.0;
+    og_run R = {g, chain, nstages, NULL, st, res};
+    if (res) res->history_len = 0;
+    apply_stage(&R, 0, out, in);
+    if (res) {
+        res->iterations = st[0].iters;
+        long pi = 0;
+        for (int s = 0; s < nstages; ++s) pi += st[s].prec_iters;
         res->precond_iterations = pi;
     }
     for (int s = 0; s < 9; ++s) free_state(&st[s]);

@@ -23,6 +23,10 @@
  *   og_solve (Jacobi)     Jacobi == scaled identity: same iterates as None
  *   og_solve (FCG)        reduces to CG for a fixed SPD preconditioner; residual drop
  *   og_solve (IR)         true FP64 residual below tolerance; FP32 CG vs FP64 CG
+ *   og_precondition (Chebyshev, precision 32)
+ *                         the FP64 Chebyshev (itself pinned by the fixtures) to FP32
+ *                         rounding, the DST closed form of the polynomial, and a
+ *                         solve preconditioned by it converging like the FP64 one
  *   og_rhs_*              statistics of the recipes (mean, variance, closed forms)
  */
 #ifndef PPS_ORACLE_H
@@ -68,6 +72,8 @@ typedef struct og_stage {
     og_user_fn user_fn;                    /* OG_USER */
     void* user;
     double ir_first_tol;                   /* OG_IR: inner tolerance of the first outer step (0: the inner's) */
+    int precision;                         /* OG_CHEB under an FP64 caller: 32 runs the alpaka
+                                              T_data_chebyshev = float form, else FP64          */
 } og_stage;
 
 typedef struct og_result {
@@ -114,6 +120,10 @@ double og_residual(const og_grid* g, const double* x, const double* b, double* r
  * flux: Neumann data per face or NULL.  Returns 0 on success. */
 int og_solve(const og_grid* g, const og_stage* chain, int nstages, double* x, double* b,
              const double* const flux[6], og_result* res);
+
+/* one preconditioner application out = M(in) of chain[0] (chain[1..] nested);
+ * `in` gets its Neumann ghosts reset in place.  Returns 0 on success. */
+int og_precondition(const og_grid* g, const og_stage* chain, int nstages, double* out, double* in, og_result* res);
 
 /* seeded right-hand sides written as b = -f on the solver region (pps_rhs.h) */
 void og_rhs_normal(const og_grid* g, uint64_t seed, double* b);

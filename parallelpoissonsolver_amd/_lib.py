@@ -53,6 +53,7 @@ class CStage(ctypes.Structure):
         ("stall_window", ctypes.c_int),
         ("stall_ratio", ctypes.c_double),
         ("ir_first_tol", ctypes.c_double),
+        ("true_finish", ctypes.c_int),
     ]
 
 
@@ -113,6 +114,7 @@ SIGNATURES = {
     "pps_copy_field": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "pps_set_neumann_flux": ([_VP, ctypes.c_int, _DP], ctypes.c_int),
     "pps_solve": ([_VP, ctypes.POINTER(CStage), ctypes.c_int, ctypes.POINTER(CResult)], ctypes.c_int),
+    "pps_precondition": ([_VP, ctypes.POINTER(CStage), ctypes.c_int, ctypes.c_int, ctypes.c_int, _LLP], ctypes.c_int),
     "pps_get_history": ([_VP, _DP, ctypes.c_int], ctypes.c_int),
     "pps_synchronize": ([_VP], ctypes.c_int),
     "pps_apply_A": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
@@ -232,7 +234,8 @@ class Context:
         plane = np.ascontiguousarray(plane, dtype=np.float64)
         self._chk(lib().pps_set_neumann_flux(self.h, face, _dp(plane)))
 
-    def solve(self, chain):
+    @staticmethod
+    def _stages(chain):
         st = (CStage * len(chain))()
         for i, c in enumerate(chain):
             st[i].kind = c["kind"]
@@ -246,6 +249,18 @@ class Context:
             st[i].stall_window = c.get("stall_window", 0)
             st[i].stall_ratio = c.get("stall_ratio", 0.0)
             st[i].ir_first_tol = c.get("first_tol", 0.0)
+            st[i].true_finish = c.get("true_finish", 0)
+        return st
+
+    def precondition(self, chain, out, inp):
+        """out = M(inp) for the preconditioner chain (pps_precondition); returns
+        the iterations spent."""
+        it = ctypes.c_longlong()
+        self._chk(lib().pps_precondition(self.h, self._stages(chain), len(chain), out, inp, ctypes.byref(it)))
+        return it.value
+
+    def solve(self, chain):
+        st = self._stages(chain)
         res = CResult()
         self._chk(lib().pps_solve(self.h, st, len(chain), ctypes.byref(res)))
         out = {k: getattr(res, k) for k, _ in CResult._fields_}

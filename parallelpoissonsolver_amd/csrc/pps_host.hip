@@ -514,6 +514,15 @@ struct KrylovStage : Stage {
         scale_box(C, x, S, 1);
         scale_box(C, b, S, 1);
     }
+    // true-residual finish of a main solve (pps_stage::true_finish): r = b - A x
+    // of the complete x in FP64 (launch_residual modes 2-4); h->done tells
+    // whether the solve ends here
+    int replace_residual(Field x, Field b, Field r, int mode, Scal* h)
+    {
+        C.ghost(x, true, S, nullptr);
+        launch_residual(C.gsolver(64), C.stream, C.rb, x.d(), b.d(), r.d(), S, mode);
+        return scal_read(C, S, *h);
+    }
 };
 
 // ------------------------------------------------------------------------ CG
@@ -553,7 +562,7 @@ struct CGStage : KrylovStage {
         }
         Field p[2] = {p0, p1};
         int cur = 0;
-        rc = run_loop(C, S, cfg.max_iter, pre_host(), [&]() -> int {
+        auto body = [&]() -> int {
             // iteration k reads p[k&1], writes p[(k+1)&1], and applies
             // iteration k-1's x update on the way
             launch_cg_A2<double>(g, C.stream, C.rb, pw ? r.d() : z.d(), p[cur].d(), p[cur ^ 1].d(), Ap.d(), x.d(), S, pw);
@@ -565,13 +574,28 @@ struct CGStage : KrylovStage {
                 launch_cg_rz<double>(g, C.stream, C.rb, r.d(), z.d(), S);
             }
             return PPS_OK;
-        });
-        if (rc) return rc;
-        // the last executed iteration K-1 left x += alpha_{K-1} p_{K-1} pending; p_{K-1} is p[K&1]
+        };
         Scal h;
-        rc = scal_read(C, S, h);
-        if (rc) return rc;
-        if (h.iter > 0) launch_cg_final_x<double>(g, C.stream, x.d(), p[h.iter & 1].d(), S);
+        for (int repl = 0;; ++repl) {
+            rc = run_loop(C, S, cfg.max_iter, pre_host(), body);
+            if (rc) return rc;
+            // the last executed iteration K-1 left x += alpha_{K-1} p_{K-1} pending; p_{K-1} is p[K&1]
+            rc = scal_read(C, S, h);
+            if (rc) return rc;
+            if (h.iter > 0) launch_cg_final_x<double>(g, C.stream, x.d(), p[h.iter & 1].d(), S);
+            if (!main || repl >= cfg.true_finish) break;
+            // true-residual finish: r = b - A x of the complete x replaces the
+            // recursive residual; the search direction p[K&1] and beta stay
+            rc = replace_residual(x, b, r, pw ? 3 : 2, &h);
+            if (rc) return rc;
+            if (h.done) break;
+            if (!pw) {
+                rc = pre_apply(z, r, &S->done);
+                if (rc) return rc;
+                launch_cg_rz<double>(g, C.stream, C.rb, r.d(), z.d(), S);
+            }
+            cur = h.iter & 1;
+        }
         epilogue(main, x, b, r);
         return PPS_OK;
     }
@@ -623,7 +647,7 @@ struct BiCGStage : KrylovStage {
         launch_copy<double>(g, C.stream, r.d(), p.d());
         launch_copy<double>(g, C.stream, r.d(), r0.d());
         const int* gate = &S->done;
-        rc = run_loop(C, S, cfg.max_iter, pre_host(), [&]() -> int {
+        auto body = [&]() -> int {
             if (pw) {
                 C.ghost(p, false, nullptr, gate);
                 launch_bicg_k1<double>(g, C.stream, C.rb, p.d(), c, AMp.d(), r0.d(), S);
@@ -647,8 +671,19 @@ struct BiCGStage : KrylovStage {
             }
             launch_bicg_k6<double>(g, C.stream, p.d(), r.d(), AMp.d(), S);
             return PPS_OK;
-        });
-        if (rc) return rc;
+        };
+        for (int repl = 0;; ++repl) {
+            rc = run_loop(C, S, cfg.max_iter, pre_host(), body);
+            if (rc) return rc;
+            if (!main || repl >= cfg.true_finish) break;
+            // true-residual finish: restart from r = b - A x with r0 = p = r
+            Scal h;
+            rc = replace_residual(x, b, r, 4, &h);
+            if (rc) return rc;
+            if (h.done) break;
+            launch_copy<double>(g, C.stream, r.d(), p.d());
+            launch_copy<double>(g, C.stream, r.d(), r0.d());
+        }
         // resetNeumanBCs<isMainLoop,true>(fieldX) (BiCGSTAB.hpp:301) before the true residual
         C.ghost(x, main, S, nullptr);
         epilogue(main, x, b, r);
@@ -701,7 +736,7 @@ struct FCGStage : KrylovStage {
         Field rr[2] = {r, r};
         if (fused_r && (rc = C.get_work(k + ".r2", 64, rr[1]))) return rc;
         int cur = 0;  // p[cur] = p_new, p[cur^1] = p_old
-        rc = run_loop(C, S, cfg.max_iter, pre_host(), [&]() -> int {
+        auto body = [&]() -> int {
             if (fused_r) {
                 // iteration k: r_k = r_{k-1} - alpha_{k-1} Ap_{k-1} into rr[cur ^ 1]
                 int nb = 0;
@@ -725,13 +760,24 @@ struct FCGStage : KrylovStage {
             launch_fcg_update<double>(g, C.stream, C.rb, r.d(), Ap[cur].d(), S, main);
             cur ^= 1;
             return PPS_OK;
-        });
-        if (rc) return rc;
+        };
         Scal h;
-        rc = scal_read(C, S, h);
-        if (rc) return rc;
-        // iteration k wrote p[k & 1]; its x update waits for the next stencil pass
-        if (h.iter > 0) launch_fcg_final_x<double>(g, C.stream, x.d(), p[(h.iter - 1) & 1].d(), S);
+        for (int repl = 0;; ++repl) {
+            rc = run_loop(C, S, cfg.max_iter, pre_host(), body);
+            if (rc) return rc;
+            rc = scal_read(C, S, h);
+            if (rc) return rc;
+            // iteration k wrote p[k & 1]; its x update waits for the next stencil pass
+            if (h.iter > 0) launch_fcg_final_x<double>(g, C.stream, x.d(), p[(h.iter - 1) & 1].d(), S);
+            if (!main || repl >= cfg.true_finish) break;
+            // true-residual finish: r = b - A x of the complete x, written where
+            // iteration K reads its residual (rr[K & 1] when the preconditioner
+            // forms the update: with alpha = 0 it forms r_K = r); the direction stays
+            rc = replace_residual(x, b, fused_r ? rr[h.iter & 1] : r, 2, &h);
+            if (rc) return rc;
+            if (h.done) break;
+            cur = h.iter & 1;
+        }
         C.fallback_switches += h.fallback;
         epilogue(main, x, b, r);
         return PPS_OK;
@@ -1064,7 +1110,9 @@ int ChebStage::solve_main()
 
 using namespace pps;
 
-static int build_chain(Ctx& C, const pps_stage* chain, int n, std::vector<std::unique_ptr<Stage>>& st)
+// main: stage 0 solves (pps_solve); otherwise stage 0 is applied once as a
+// preconditioner (pps_precondition)
+static int build_chain(Ctx& C, const pps_stage* chain, int n, std::vector<std::unique_ptr<Stage>>& st, bool main = true)
 {
     if (!chain || n < 1 || n > 8) return C.fail(PPS_ERR_ARG, "solver chain needs 1..8 stages");
     for (int i = 0; i < n; ++i) {
@@ -1089,8 +1137,11 @@ static int build_chain(Ctx& C, const pps_stage* chain, int n, std::vector<std::u
     }
     for (int i = 0; i + 1 < n; ++i) st[i]->pre = st[i + 1].get();
     const int k0 = chain[0].kind;
-    if (k0 == PPS_NONE || k0 == PPS_JACOBI || k0 == PPS_NN)
+    if (main && (k0 == PPS_NONE || k0 == PPS_JACOBI || k0 == PPS_NN))
         return C.fail(PPS_ERR_ARG, "stage 0 must be a solver (CG, BiCGSTAB, Chebyshev, FCG, IR)");
+    if (!main && (k0 == PPS_IR || k0 == PPS_NN))
+        return C.fail(PPS_ERR_ARG, "stage 0 of a preconditioner chain must be None, Jacobi, CG, BiCGSTAB, Chebyshev "
+                                   "or FCG (the learned preconditioner: pps_nn_apply_field)");
     return PPS_OK;
 }
 
@@ -1421,6 +1472,32 @@ int pps_solve(void* ctx, const pps_stage* chain, int nstages, pps_result* res)
         res->precond_iterations = pi;
         res->fallback_switches = C->fallback_switches;
         C->hist_len = res->history_len;
+    }
+    return PPS_OK;
+}
+
+int pps_precondition(void* ctx, const pps_stage* chain, int nstages, int out, int in, long long* iterations)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    std::vector<std::unique_ptr<Stage>> st;
+    int rc = build_chain(*C, chain, nstages, st, false);
+    if (rc) return rc;
+    C->err.clear();
+    Field fo, fi;
+    rc = field_of(*C, out, fo);
+    if (!rc) rc = field_of(*C, in, fi);
+    if (rc) return rc;
+    if (out == in) return C->fail(PPS_ERR_ARG, "pps_precondition: out and in must differ");
+    rc = st[0]->apply(fo, fi, nullptr);
+    if (rc) return rc;
+    rc = C->hip(hipStreamSynchronize(C->stream), "sync");
+    if (!rc) rc = C->hip(hipGetLastError(), "kernel");
+    if (rc) return rc;
+    if (iterations) {
+        long long pi = 0;
+        for (auto& s : st) pi += s->prec_iters;
+        *iterations = pi;
     }
     return PPS_OK;
 }

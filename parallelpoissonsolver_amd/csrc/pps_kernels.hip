@@ -596,8 +596,18 @@ struct OpResidual : Gate {
             S->iter = 0;
             if (S->hist != nullptr && S->hist_cap > 0) S->hist[0] = e;
             S->done = (e < S->v[S_TOL]) ? 2 : 0;
-        } else {
+        } else if (mode == 1) {
             S->v[S_ERR_OP] = e;
+        } else {
+            // residual replacement after a recursive stop: r now holds the true
+            // residual of the complete x; the loop resumes from it unless it is
+            // below the tolerance or the iteration cap is reached
+            S->v[S_ERR_OP] = e;
+            S->v[S_RR] = t[0];
+            S->v[S_ALPHA] = 0.0;  // x is complete: no pending update
+            if (mode == 3) S->v[S_RZ] = t[1];   // CG, pointwise preconditioner: r.(jac r) of the new r
+            if (mode == 4) S->v[S_RHO0] = t[0]; // BiCGSTAB restarted with r0 = r: rho = r0.r
+            S->done = (e < S->v[S_TOL] || S->iter >= S->maxit) ? 1 : 0;
         }
     }
 };
@@ -1349,7 +1359,12 @@ struct OpFcgP : Gate {
         } else {
             S->restart = 0;
             S->v[S_ALPHA] = a;
-            if constexpr (FUSED) rr_next = fmax(t[4] - 2.0 * a * t[2] + a * a * t[3], 0.0);
+            // the expansion cancels when one step removes almost all of r: its
+            // error is a few eps * r.r, so below 1e-8 r.r it is noise.  Clamping
+            // there over-estimates the next residual (the loop runs on, and the
+            // next pass sums r.r exactly) and never gives the next network input
+            // a zero scale
+            if constexpr (FUSED) rr_next = fmax(t[4] - 2.0 * a * t[2] + a * a * t[3], 1e-8 * t[4]);
         }
         if constexpr (FUSED) update_finish(S, rr_next, history);
     }

@@ -112,7 +112,10 @@ template <typename T>
 void launch_dot(const Geo& g, hipStream_t s, RedBuf rb, const T* a, const T* b, double* out, const int* gate);
 
 // r = b - A x; S[S_ERR0 or S_ERR_OP] = ||r||, S_RZ = r.(jac r); mode 0: start of solve
-// (history[0], iteration reset, early-exit flag), mode 1: final true residual
+// (history[0], iteration reset, early-exit flag), mode 1: final true residual;
+// modes 2-4: residual replacement after a recursive stop (S_RR, S_ALPHA = 0, done
+// = 0 unless below tolerance or at the iteration cap), 3 also sets S_RZ (CG with a
+// pointwise preconditioner), 4 S_RHO0 = ||r||^2 (BiCGSTAB restarted with r0 = r)
 void launch_residual(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, double* r, Scal* S, int mode);
 
 // ||b_adj||^2 of the boundary-adjusted rhs over the solver region -> S_NORM = sqrt

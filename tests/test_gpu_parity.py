@@ -157,6 +157,8 @@ def test_solve_matches_oracle(golden, name):
     sl = G.solver_slice()
     diff = relmax(xg[sl], xo[sl])
     info = (it_g, it_o, diff, res["error_compute_operator"], ro["error_compute_operator"])
+    print(f"[solve] {name}: GPU {it_g} it, oracle {it_o} it, max rel dx {diff:.3e}, true residual "
+          f"{res['error_compute_operator']:.3e} / {ro['error_compute_operator']:.3e}", flush=True)
     if ro["error_compute_operator"] < 1e-10:
         assert res["error_compute_operator"] < 1e-10, info
     else:
@@ -165,14 +167,18 @@ def test_solve_matches_oracle(golden, name):
         # fixed-degree smoother): same value as the reference
         assert abs(res["error_compute_operator"] - ro["error_compute_operator"]) <= 1e-8 * ro["error_compute_operator"]
     if c["args"][c["args"].index("--solver") + 1] == "bicgstab":
-        # BiCGSTAB amplifies reduction-order rounding: the reference's own 1-rank
-        # and 4-rank runs of one problem take 103 and 108 iterations
-        # (bicgstab_none_normal_34 / _r4).  Bar: iterations within 5%, solutions
-        # equal to the accuracy both solves reached, ||dx||/||x|| <= kappa (r1 + r2).
-        assert abs(it_g - it_o) <= max(1, round(0.05 * it_o)), info
+        # BiCGSTAB amplifies reduction-order rounding (the reference's own 1-rank
+        # and 4-rank runs of one problem take 103 and 108 iterations,
+        # bicgstab_none_normal_34 / _r4), so the two solves stop at different
+        # iterates.  Bar: iterations within 2 % (at least 1; measured 0..3, at
+        # most 1.6 %), and the solutions equal to the accuracy both solves
+        # reached: ||dx||/||x|| <= kappa (r1 + r2), the perturbation bound of
+        # two relative residuals r1, r2 (measured: up to 3.5e-9 at 66^3, where
+        # the bound is 3.2e-7; cases that reach 1e-12 agree to 1e-12 and below).
+        assert abs(it_g - it_o) <= max(1, round(0.02 * it_o)), info
         e = G.g.eig_global
         kappa = e[1] / e[0]
-        assert diff <= 10 * kappa * (res["error_compute_operator"] + ro["error_compute_operator"]), info
+        assert diff <= kappa * (res["error_compute_operator"] + ro["error_compute_operator"]), info
     else:
         assert abs(it_g - it_o) <= max(0, round(0.02 * it_o)), info
         assert diff < 1e-12, info
@@ -224,6 +230,46 @@ def test_jacobi_equals_unpreconditioned():
         e = G.g.eig_global
         bound = 1e-9 if kind == L.CG else 10 * e[1] / e[0] * (r0["error_compute_operator"] + r1["error_compute_operator"])
         assert relmax(x1, x0) < bound
+
+
+@pytest.mark.parametrize("kind", [L.CG, L.BICGSTAB, L.FCG])
+@pytest.mark.parametrize("tol", [1e-10, 1e-14])
+def test_true_residual_finish(kind, tol):
+    """pps_stage.true_finish: without it the solve stops on the recursive
+    residual (the reference's rule); with it the true FP64 residual of the
+    returned x, recomputed by the oracle, is below tol.  When the recursive
+    stop already meets tol on the true residual nothing changes, bit for bit.
+    At tol = 1e-14 the recursive residual has drifted below the true one (the
+    oracle's CG and FCG stop at a true 1.05e-14 at 66^3), so the finish has to
+    replace it and resume."""
+    n = 66
+    G = O.OracleGrid((n,) * 3, (1 / (n - 1),) * 3, (0, 0, 0), (0,) * 6)
+    b = G.rhs("normal", 13)
+    x = G.zeros()
+    out = {}
+    for fin in (0, 10):
+        C = ctx_for(G, (0,) * 6)
+        C.upload(L.FIELD_X, x)
+        C.upload(L.FIELD_B, b)
+        res = C.solve([dict(kind=kind, tol=tol, maxit=20000, true_finish=fin), dict(kind=L.NONE)])
+        out[fin] = (res, C.download(L.FIELD_X))
+        C.close()
+    (r0, x0), (r1, x1) = out[0], out[10]
+    sl = G.solver_slice()
+    bt = b.copy()
+    G.adjust_b(x, bt, None)
+    nb = np.sqrt((bt[sl] ** 2).sum())
+    true0 = G.residual(x0, b)[0] / nb
+    true1 = G.residual(x1, b)[0] / nb
+    print(f"[finish] kind {kind} tol {tol:.0e}: recursive stop {r0['iterations']} it true {true0:.3e} "
+          f"(lib {r0['error_compute_operator']:.3e}); finish {r1['iterations']} it true {true1:.3e} "
+          f"(lib {r1['error_compute_operator']:.3e})", flush=True)
+    assert abs(true1 - r1["error_compute_operator"]) <= 1e-3 * true1 + 1e-16
+    assert true1 < tol
+    if true0 < tol:
+        assert r1["iterations"] == r0["iterations"] and np.array_equal(x1, x0)
+    else:
+        assert r1["iterations"] > r0["iterations"]
 
 
 def test_fcg_with_fixed_preconditioner_equals_cg():
@@ -320,24 +366,71 @@ def test_ir_first_step_tolerance_matches_oracle():
     assert res["error_compute_operator"] < 1e-10 and ro["error_compute_operator"] < 1e-10
 
 
-def test_chebyshev_fp32_hook_matches_fp64_preconditioner():
-    """T_data_chebyshev = float (chebyshevIterationAlpaka.hpp:149-152): the FP32
-    preconditioner changes iterates only slightly and still converges."""
-    n = 34
-    G = O.OracleGrid((n,) * 3, (1 / (n - 1),) * 3, (0, 0, 0), (0,) * 6)
-    b = G.rhs("gauss", 2)
-    x = G.zeros()
-    its = []
-    for prec in (64, 32):
-        C = ctx_for(G, (0,) * 6)
-        C.upload(L.FIELD_X, x)
-        C.upload(L.FIELD_B, b)
-        res = C.solve([dict(kind=L.BICGSTAB, tol=1e-10, maxit=1000),
-                       dict(kind=L.CHEBYSHEV, precision=prec, maxit=24, cheb_eps=0, cheb_rmin=100, cheb_rmax=1 - 1e-4)])
-        assert res["error_compute_operator"] < 1e-10
-        its.append(res["iterations"])
-        C.close()
-    assert abs(its[0] - its[1]) <= 3
+ALP_CHEB = dict(cheb_eps=0.0, cheb_rmin=100.0, cheb_rmax=1 - 1e-4)  # solverSetup.hpp (alpaka):59-62
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+@pytest.mark.parametrize("bcs", [(0, 0, 0, 0, 0, 0), (0, 1, 1, 0, 1, 0)])
+@pytest.mark.parametrize("m", [4, 11, 24])
+def test_chebyshev_preconditioner_matches_oracle(prec, bcs, m):
+    """One Chebyshev preconditioner application (pps_precondition) against the
+    oracle's (og_precondition).  precision 32 is the alpaka T_data_chebyshev =
+    float form (chebyshevIterationAlpaka.hpp:149-196): FP64 input with its
+    Neumann ghosts reset, cast to FP32, FP32 sweeps, widened output.  Bars:
+    FP64 1e-12 relative max-norm; FP32 5e-6 (the oracle's FP32 form differs
+    from its FP64 form by 1e-7 .. 6e-7 on these inputs)."""
+    n = (34, 30, 26)
+    ds = tuple(1.0 / (v - 1) for v in n)
+    G = O.OracleGrid(n, ds, (3, 2.5, 10), bcs)
+    inp = np.random.default_rng(5).standard_normal(G.shape)
+    chain = [dict(kind=L.CHEBYSHEV, precision=prec, maxit=m, **ALP_CHEB)]
+    C = ctx_for(G, bcs)
+    C.upload(L.FIELD_T0, inp)
+    C.fill(L.FIELD_T1, 0.0)
+    its = C.precondition(chain, L.FIELD_T1, L.FIELD_T0)
+    yg = C.download(L.FIELD_T1)
+    ing = C.download(L.FIELD_T0)
+    C.close()
+    ino = inp.copy()
+    yo, info = G.precondition([dict(kind=O.CHEB, precision=prec, maxit=m, **ALP_CHEB)], ino)
+    sl = G.solver_slice()
+    diff = relmax(yg[sl], yo[sl])
+    print(f"[cheb-pre] prec {prec} bcs {bcs} m {m}: max rel diff {diff:.3e}", flush=True)
+    assert its == info["precond_iterations"] == m
+    assert diff < (1e-12 if prec == 64 else 5e-6), diff
+    # the input's Neumann ghosts were reset in place, as the oracle's
+    assert np.array_equal(ing, ino)
+
+
+@pytest.mark.parametrize("n,bcs", [(34, (0, 1, 1, 0, 1, 0)), (66, (0, 0, 0, 0, 0, 0))])
+def test_bicgstab_fp32_chebyshev_solve_matches_oracle(n, bcs):
+    """The reference's default GPU composition with T_data_chebyshev = float:
+    BiCGSTAB (FP64) preconditioned by Chebyshev(24, global, epsilon 0,
+    rescaleEigMin 100) in FP32, GPU against the oracle on the same problem.
+    Bars: iterations within 2 % (at least 1), solutions within
+    10 kappa (r_gpu + r_oracle) as for every BiCGSTAB case, both at 1e-10."""
+    G = O.OracleGrid((n,) * 3, (1 / (n - 1),) * 3, (3, 2.5, 10), bcs)
+    x, b, flux = G.set_problem(0)
+    chain = [dict(kind=L.BICGSTAB, tol=1e-10, maxit=1000),
+             dict(kind=L.CHEBYSHEV, precision=32, maxit=24, **ALP_CHEB)]
+    C = ctx_for(G, bcs)
+    upload_problem(C, G, x, b, flux, bcs)
+    res = C.solve(chain)
+    xg = C.download(L.FIELD_X)
+    C.close()
+    xo, bo = x.copy(), b.copy()
+    ro = G.solve([dict(kind=O.BICGSTAB, tol=1e-10, maxit=1000),
+                  dict(kind=O.CHEB, precision=32, maxit=24, **ALP_CHEB)], xo, bo, flux)
+    sl = G.solver_slice()
+    diff = relmax(xg[sl], xo[sl])
+    it_g, it_o = res["iterations"], ro["iterations"]
+    e = G.g.eig_global
+    kappa = e[1] / e[0]
+    print(f"[cheb32-solve] {n}^3 bcs {bcs}: GPU {it_g} it, oracle {it_o} it, max rel dx {diff:.3e}, "
+          f"true residual {res['error_compute_operator']:.3e} / {ro['error_compute_operator']:.3e}", flush=True)
+    assert res["error_compute_operator"] < 1e-10 and ro["error_compute_operator"] < 1e-10
+    assert abs(it_g - it_o) <= max(1, round(0.02 * it_o)), (it_g, it_o)
+    assert diff <= 10 * kappa * (res["error_compute_operator"] + ro["error_compute_operator"]), diff
 
 
 @pytest.mark.parametrize("name", ["cg_none_normal_258_r8", "bicgstab_none_normal_258_r8", "cg_none_gauss_130_r8"])
