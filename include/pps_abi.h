@@ -157,6 +157,21 @@ int pps_apply_A(void* ctx, int y, int x);
 int pps_ghost_fill(void* ctx, int field, int with_flux, double norm);
 /* r = b - A x on the solver region, returns ||r|| (iterativeSolverBase.hpp:236-280) */
 int pps_residual(void* ctx, int r, int x, int b, double* norm);
+/* pps_residual without a context and without allocating: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * r = b - A x on the solver region of `grid` (BlockGrid limits, one rank) and
+ * ||r|| over it, on caller-owned device arrays in the reference's dense
+ * guarded layout ((n0+2)(n1+2)(n2+2) doubles, x fastest); r is written on the
+ * solver region only.  ||r|| goes to the device double *norm_dev and/or the
+ * host double *norm_host (either may be NULL; a host result synchronises
+ * `stream`).  stream: a hipStream_t, NULL for the null stream.  The reduction
+ * partials live in a static device array of the library (1 MiB per GPU, kept
+ * until the process ends): an x-y plane may need at most 131072 tiles of
+ * 64 x 16 points, larger grids are refused with PPS_ERR_UNSUPPORTED.  Calls
+ * that overlap on one GPU share that array and race.  Errors: pps_last_error(NULL). */
+int pps_residual_noalloc(const pps_grid* grid, const double* x, const double* b, double* r, double* norm_dev,
+                         double* norm_host, void* stream);
 /* sum a.b over the solver region, deterministic single-pass grid reduction */
 int pps_dot(void* ctx, int a, int b, double* out);
 /* ||b_adj|| of the boundary-adjusted right-hand side (iterativeSolverBase.hpp:171-234) */

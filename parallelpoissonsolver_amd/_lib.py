@@ -120,6 +120,7 @@ SIGNATURES = {
     "pps_apply_A": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "pps_ghost_fill": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double], ctypes.c_int),
     "pps_residual": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _DP], ctypes.c_int),
+    "pps_residual_noalloc": ([ctypes.POINTER(CGrid), _VP, _VP, _VP, _VP, _DP, _VP], ctypes.c_int),
     "pps_dot": ([_VP, ctypes.c_int, ctypes.c_int, _DP], ctypes.c_int),
     "pps_norm_b_adjusted": ([_VP, _DP], ctypes.c_int),
     "pps_rhs_normal": ([_VP, ctypes.c_uint64], ctypes.c_int),

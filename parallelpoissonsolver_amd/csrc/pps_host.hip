@@ -24,6 +24,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1563,6 +1564,127 @@ int pps_residual(void* ctx, int r, int x, int b, double* norm)
     rc = scal_read(*C, S, h);
     if (rc) return rc;
     if (norm) *norm = h.v[S_ERR_OP];
+    return PPS_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------- allocation-free residual variant
+//
+// pps_residual_noalloc keeps its reduction partials and its result slot in
+// this file-scope device array instead of context memory, so it calls no
+// allocator.  Each GPU holds its own copy until the process ends; every call
+// on one GPU shares it (the last call's contents stay), so calls that overlap
+// on one GPU race.  Host code never passes the array's name to a kernel or a
+// copy: that name is a host-side stand-in, not a device address.  It uses the
+// address hipGetSymbolAddress returns for the current device, cached per
+// device at that device's first call.
+// 1 MiB of partials (one per block of one launch) and a 256-B result slot
+constexpr int kScratchPartials = 1 << 17;
+static __device__ __align__(256) double scratch[kScratchPartials + 32];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's address of `scratch` (0 on success)
+int scratch_for_current_device(double** out, std::string& err)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+        return PPS_ERR_NOGPU;
+    }
+    if (dev < 0 || dev >= kMaxDevices) {
+        err = "device index beyond the scratch table";
+        return PPS_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return PPS_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return PPS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pps_residual_noalloc(const pps_grid* grid, const double* x, const double* b, double* r, double* norm_dev,
+                         double* norm_host, void* stream)
+{
+    g_create_error.clear();
+    if (!grid || !x || !b || !r) {
+        g_create_error = "pps_residual_noalloc: null argument";
+        return PPS_ERR_ARG;
+    }
+    if (grid->dim != 3 || grid->nranks[0] * grid->nranks[1] * grid->nranks[2] != 1 || grid->my_rank != 0) {
+        g_create_error = "pps_residual_noalloc: DIM 3 on a 1x1x1 rank grid only";
+        return PPS_ERR_UNSUPPORTED;
+    }
+    Layout L{};
+    Region R{};
+    double coef[3];
+    for (int d = 0; d < 3; ++d) {
+        if (grid->guards[d] != 1 || grid->npglobal[d] < 3 || !(grid->ds[d] > 0)) {
+            g_create_error = "pps_residual_noalloc: guards must be 1, npglobal >= 3 and ds > 0";
+            return PPS_ERR_ARG;
+        }
+        for (int s = 0; s < 2; ++s)
+            if (grid->bcs_type[2 * d + s] != 0 && grid->bcs_type[2 * d + s] != 1) {
+                g_create_error = "pps_residual_noalloc: bcsType must be 0 or 1";
+                return PPS_ERR_ARG;
+            }
+        L.ng[d] = grid->npglobal[d] + 2;
+        coef[d] = 1.0 / (grid->ds[d] * grid->ds[d]);
+        // BlockGrid's solver limits for one rank (blockGrid.hpp:208-222)
+        R.lo[d] = 1 + (grid->bcs_type[2 * d] == 0);
+        R.hi[d] = grid->npglobal[d] + 1 - (grid->bcs_type[2 * d + 1] == 0);
+    }
+    // the reference's dense guarded host layout, x fastest (iterativeSolverBase.hpp:34-35)
+    L.px = L.ng[0];
+    L.ox = 0;
+    L.plane = (long long)L.ng[0] * L.ng[1];
+    L.total = L.plane * L.ng[2];
+    // z-chunk: the library default (16 planes), longer when a launch would
+    // need more blocks than the scratch holds partials
+    const long long tiles = (long long)((R.hi[0] - R.lo[0] + kTX - 1) / kTX) * ((R.hi[1] - R.lo[1] + kTH - 1) / kTH);
+    if (tiles > kScratchPartials) {
+        g_create_error = "pps_residual_noalloc: an x-y plane needs more than 131072 tiles of 64x16 points "
+                         "(the scratch holds one partial per block)";
+        return PPS_ERR_UNSUPPORTED;
+    }
+    const int nz = R.hi[2] - R.lo[2];
+    int kz = 16;
+    while (tiles * ((nz + kz - 1) / kz) > kScratchPartials) kz *= 2;
+    double* sp = nullptr;
+    int rc = scratch_for_current_device(&sp, g_create_error);
+    if (rc) return rc;
+    RedBuf rb{sp, nullptr, kScratchPartials};
+    double* out = norm_dev ? norm_dev : sp + kScratchPartials;  // else the result slot past the partials
+    const Geo g = make_geo(L, R, coef, kz, kz, 0);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (launch_residual_norm(g, s, rb, x, b, r, out) != 0) {
+        g_create_error = "pps_residual_noalloc: launch exceeds the scratch partials";
+        return PPS_ERR_UNSUPPORTED;
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && norm_host) {
+        e = hipMemcpyAsync(norm_host, out, sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    if (e != hipSuccess) {
+        g_create_error = std::string("pps_residual_noalloc: ") + hipGetErrorString(e);
+        return PPS_ERR_HIP;
+    }
     return PPS_OK;
 }
 

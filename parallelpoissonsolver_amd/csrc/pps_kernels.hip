@@ -624,6 +624,40 @@ void launch_residual(const Geo& g, hipStream_t s, RedBuf rb, const double* x, co
     run_stencil<double, 2>(g, s, op, rb);
 }
 
+// r = b - A x and ||r|| into one device double: the residual above without a
+// solver's scalar block, for callers that bring their own fields and partials
+struct OpResidualNorm : Gate {
+    const double* __restrict__ x;
+    const double* __restrict__ b;
+    double* __restrict__ r;
+    double* norm;
+    __device__ void load() {}
+    struct Ox { double b; };
+    __device__ double in(long long c) const { return x[c]; }
+    __device__ Ox out_raw(long long c) const { return {b[c]}; }
+    __device__ void out(long long c, double, double Av, const double&, const Ox& o, double* acc) const
+    {
+        const double rv = o.b - Av;
+        r[c] = rv;
+        acc[0] += rv * rv;
+    }
+    __device__ void finish(const double* t) { *norm = sqrt(t[0]); }
+};
+
+int launch_residual_norm(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, double* r, double* out)
+{
+    const Geo gt = retile(g, kTH, g.kz_st);
+    if (gt.nblocks > rb.cap) return -1;
+    OpResidualNorm op;
+    op.gate = nullptr;
+    op.x = x;
+    op.b = b;
+    op.r = r;
+    op.norm = out;
+    run_stencil<double, 1>(g, s, op, rb);
+    return 0;
+}
+
 // boundary adjustment of b at one point (adjustFieldBForDirichletNeumanBCs,
 // iterativeSolverBase.hpp:429-534): Dirichlet faces subtract the boundary value
 // of x over ds^2 on the first interior layer, Neumann faces add -+ 2 du/dn / ds

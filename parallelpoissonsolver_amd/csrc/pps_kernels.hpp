@@ -118,6 +118,11 @@ void launch_dot(const Geo& g, hipStream_t s, RedBuf rb, const T* a, const T* b, 
 // pointwise preconditioner), 4 S_RHO0 = ||r||^2 (BiCGSTAB restarted with r0 = r)
 void launch_residual(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, double* r, Scal* S, int mode);
 
+// r = b - A x on g's region and ||r|| into *out (device); the partials go to
+// rb.partials.  Returns -1 (nothing launched) when the launch would need more
+// than rb.cap blocks
+int launch_residual_norm(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, double* r, double* out);
+
 // ||b_adj||^2 of the boundary-adjusted rhs over the solver region -> S_NORM = sqrt
 void launch_bnorm(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, const int bc[6],
                   const int lim_data[6], const int lim_solver[6], const double* const flux[6], const int fng[6][2],
