@@ -20,9 +20,12 @@
 
 #include "../pps_abi.h"
 
+#include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <fstream>
 #include <functional>
 #include <iostream>
 #include <string>
@@ -31,11 +34,29 @@
 
 namespace pps {
 
-// solverSetup.hpp:22,37-39 (CPU form)
+// Tolerance scale and Chebyshev spectrum bounds.  With PPS_USE_SOLVER_SETUP
+// defined, the caller's own solverSetup.hpp (included before this header, as
+// the drop-in headers of include/pps/dropin/ do) sets them, as it does for the
+// reference; otherwise the defaults of the reference's CPU solverSetup.hpp:22,
+// 37-39 hold, and the alpaka types take those of the alpaka
+// solverSetup.hpp:59-62 (epsilon 0, rescaleEigMin 100).
+#ifdef PPS_USE_SOLVER_SETUP
+constexpr double tollScalingFactor = ::tollScalingFactor;
+constexpr double epsilon = ::epsilon;
+constexpr double rescaleEigMin = ::rescaleEigMin;
+constexpr double rescaleEigMax = ::rescaleEigMax;
+constexpr double alpakaEpsilon = ::epsilon;
+constexpr double alpakaRescaleEigMin = ::rescaleEigMin;
+constexpr double alpakaRescaleEigMax = ::rescaleEigMax;
+#else
 constexpr double tollScalingFactor = 1e-10;
 constexpr double epsilon = 1e-4;
 constexpr double rescaleEigMin = 500;
 constexpr double rescaleEigMax = 1 - 1e-4;
+constexpr double alpakaEpsilon = 0;
+constexpr double alpakaRescaleEigMin = 100;
+constexpr double alpakaRescaleEigMax = 1 - 1e-4;
+#endif
 
 namespace detail {
 [[noreturn]] inline void config_error(const std::string& msg)
@@ -56,7 +77,63 @@ inline pps_stage stage(int kind, int prec, double tol, int maxit)
     s.cheb_rescale_max = rescaleEigMax;
     return s;
 }
+// a solver type whose operator()(x, b) sets the problem itself, as the alpaka
+// main solvers do (BiCGstabAlpaka.hpp:136-138)
+template <class T, class = void> struct SetsProblemInside : std::false_type {};
+template <class T> struct SetsProblemInside<T, std::void_t<decltype(T::kSetsProblemInside)>>
+    : std::integral_constant<bool, T::kSetsProblemInside> {};
 }  // namespace detail
+
+// TimeCounter<DIM,T_data> (solverPoissonMPI_alpaka/include/solverSetup.hpp:206-286):
+// the per-phase times of the alpaka main loop.  The device path runs a whole
+// iteration as one queued sequence of fused kernels with no host-timed phases,
+// so only the total (the device time of the solve) is filled; the phase
+// fields stay zero.
+template <int DIM, typename T_data>
+class TimeCounter {
+public:
+    using dur = std::chrono::duration<double>;
+    dur timeTotPreconditioner1{0}, timeTotPreconditioner2{0}, timeTotCommunication1{0}, timeTotCommunication2{0};
+    dur timeTotKernel1{0}, timeTotKernel2{0}, timeTotKernel3{0}, timeTotKernel4{0}, timeTotKernel5{0}, timeTotKernel6{0};
+    dur timeTotResetNeumanBCs{0}, timeTotAllReduction1{0}, timeTotAllReduction2{0};
+    dur timeDevice{0};  // device time of the whole solve
+    dur timeTotal{0};
+    dur getTimeTotal()
+    {
+        timeTotal = timeTotPreconditioner1 + timeTotPreconditioner2 + timeTotCommunication1 + timeTotCommunication2 +
+                    timeTotKernel1 + timeTotKernel2 + timeTotKernel3 + timeTotKernel4 + timeTotKernel5 + timeTotKernel6 +
+                    timeTotResetNeumanBCs + timeTotAllReduction1 + timeTotAllReduction2 + timeDevice;
+        return timeTotal;
+    }
+    // the reference's printout, milliseconds per cycle
+    void printAverageTime(const int numCycles)
+    {
+        const double n = numCycles > 0 ? numCycles : 1;
+        auto ms = [n](dur d) { return d.count() * 1000 / n; };
+        std::cout << "Time in milliseconds " << std::endl;
+        std::cout << "timeTotPreconditioner1 " << ms(timeTotPreconditioner1) << std::endl;
+        std::cout << "timeTotPreconditioner2 " << ms(timeTotPreconditioner2) << std::endl;
+        std::cout << "timeTotCommunication1 " << ms(timeTotCommunication1) << std::endl;
+        std::cout << "timeTotCommunication2 " << ms(timeTotCommunication2) << std::endl;
+        std::cout << "timeTotAllReduction1 " << ms(timeTotAllReduction1) << std::endl;
+        std::cout << "timeTotAllReduction2 " << ms(timeTotAllReduction2) << std::endl;
+        std::cout << "timeTotKernel1 " << ms(timeTotKernel1) << std::endl;
+        std::cout << "timeTotKernel2 " << ms(timeTotKernel2) << std::endl;
+        std::cout << "timeTotKernel3 " << ms(timeTotKernel3) << std::endl;
+        std::cout << "timeTotKernel4 " << ms(timeTotKernel4) << std::endl;
+        std::cout << "timeTotKernel5 " << ms(timeTotKernel5) << std::endl;
+        std::cout << "timeTotKernel6 " << ms(timeTotKernel6) << std::endl;
+        std::cout << "timeTotPreconditionerTot " << ms(timeTotPreconditioner1 + timeTotPreconditioner2) << std::endl;
+        std::cout << "timeTotCommunicationTot " << ms(timeTotCommunication1 + timeTotCommunication2) << std::endl;
+        std::cout << "timeTotAllReductionTot " << ms(timeTotAllReduction1 + timeTotAllReduction2) << std::endl;
+        std::cout << "timeTotKernelsBBiCGstabTot "
+                  << ms(timeTotKernel1 + timeTotKernel2 + timeTotKernel3 + timeTotKernel4 + timeTotKernel5 + timeTotKernel6)
+                  << std::endl;
+        std::cout << "timeTotResetNeumanBCs " << ms(timeTotResetNeumanBCs) << std::endl;
+        std::cout << "timeTotDevice " << ms(timeDevice) << std::endl;
+        std::cout << "timeTotal " << ms(getTimeTotal()) << std::endl;
+    }
+};
 
 // BlockGrid<DIM,T_data> (blockGrid.hpp:11-367) on one GPU
 template <int DIM, typename T_data>
@@ -120,12 +197,47 @@ class MatrixFreeOperatorA {
 public:
     explicit MatrixFreeOperatorA(const BlockGrid<DIM, T_data>&) {}
 };
+// AlpakaHelper (solverPoissonMPI_alpaka/include/alpakaHelper.hpp): device,
+// queues and work divisions of the alpaka build; the library holds its own
+template <int DIM, typename T_data>
+class AlpakaHelper {
+public:
+    explicit AlpakaHelper(const BlockGrid<DIM, T_data>&) {}
+};
+
+// solution.dat as the alpaka main program writes it (main.cpp:135-146): the
+// rank's whole guarded array fieldX, raw T_data in host byte order, at byte
+// offset sizeof(T_data) * ntotLocalGuards * rank of a file opened for writing
+// without truncation (MPI_MODE_CREATE | MPI_MODE_WRONLY).  Returns false if
+// the file cannot be written.
+template <int DIM, typename T_data>
+bool writeSolutionDat(const BlockGrid<DIM, T_data>& blockGrid, const T_data fieldX[],
+                      const std::string& path = "solution.dat")
+{
+    std::fstream f(path, std::ios::in | std::ios::out | std::ios::binary);
+    if (!f.is_open()) {
+        f.clear();
+        f.open(path, std::ios::out | std::ios::binary);  // create
+    }
+    if (!f.is_open()) return false;
+    const std::streamoff off =
+        (std::streamoff)sizeof(T_data) * (std::streamoff)blockGrid.getNtotLocalGuards() * blockGrid.getMyrank();
+    f.seekp(off);
+    f.write(reinterpret_cast<const char*>(fieldX), (std::streamsize)sizeof(T_data) * blockGrid.getNtotLocalGuards());
+    return (bool)f;
+}
 
 // ------------------------------------------------------------- solver base
 
 template <class Derived, int DIM, typename T_data>
 class SolverFacade {
 public:
+    // the alpaka constructor (BiCGstabAlpaka.hpp:22-23) takes the AlpakaHelper too
+    template <class T_Exact, class T_Comm>
+    SolverFacade(const BlockGrid<DIM, T_data>& g, const T_Exact& exact, T_Comm& comm, const AlpakaHelper<DIM, T_data>&)
+        : SolverFacade(g, exact, comm)
+    {
+    }
     template <class T_Exact, class T_Comm>
     SolverFacade(const BlockGrid<DIM, T_data>& g, const T_Exact& exact, T_Comm&) : g_(g)
     {
@@ -134,6 +246,7 @@ public:
         trueSolution_ = [&exact](T_data x, T_data y, T_data z) { return exact.trueSolutionFxyz(x, y, z); };
         trueDdir_ = [&exact](T_data x, T_data y, T_data z, int d) { return exact.trueSolutionDdir(x, y, z, d); };
         Derived::stages(chain_);
+        maxIteration_ = chain_[0].max_iter;
         pps_grid pg{};
         pg.dim = DIM;
         for (int d = 0; d < 3; ++d) {
@@ -180,9 +293,12 @@ public:
     }
 
     // BiCGstabAlpaka::operator()(T_data*, T_data*) (BiCGstabAlpaka.hpp:68-475):
-    // copy in, solve on the device, copy x back
+    // copy in, solve on the device, copy x back.  The alpaka main solvers set
+    // the problem themselves first (BiCGstabAlpaka.hpp:136-138); the CPU ones
+    // leave it to the caller's setProblem (main.cpp:94)
     void operator()(T_data fieldX[], T_data fieldB[])
     {
+        if constexpr (detail::SetsProblemInside<Derived>::value) setProblem(fieldX, fieldB);
         check(pps_upload(ctx_, PPS_FIELD_X, fieldX));
         check(pps_upload(ctx_, PPS_FIELD_B, fieldB));
         set_flux();
@@ -193,7 +309,8 @@ public:
         numIterationPreconditionerFinal_ = (int)r.precond_iterations;
         errorFromIteration_ = r.error_from_iteration;
         errorComputeOperator_ = r.error_compute_operator;
-        durationSolver_ = r.seconds;
+        durationSolver_ = std::chrono::duration<double>(r.seconds);
+        timeCounter.timeDevice += durationSolver_;
         outerIterations_ = r.outer_iterations;
         fallbackSwitches_ = r.fallback_switches;
         history_.assign(r.history_len, 0.0);
@@ -211,8 +328,58 @@ public:
     int getNumIterationPreconditionerFinal() const { return numIterationPreconditionerFinal_; }
     T_data getErrorFromIteration() const { return errorFromIteration_; }
     T_data getErrorComputeOperator() const { return errorComputeOperator_; }
-    double getDurationSolver() const { return durationSolver_; }
+    // iterativeSolverBase.hpp:410 (the device time of the solve, HIP events)
+    std::chrono::duration<double> getDurationSolver() const { return durationSolver_; }
     const std::vector<double>& getErrorFromIterationHistory() const { return history_; }
+
+    // checkSolutionLocalGlobal (iterativeSolverBase.hpp:283-408): resets x's
+    // Neumann ghosts with the exact flux, compares x with trueSolutionFxyz on
+    // the data region, prints the largest block-average and point errors as
+    // the reference's rank 0 does, and returns the sum of |x - u| (errorLocal).
+    // One GPU: the rank grid is one block, so both maxima are this block's.
+    T_data checkSolutionLocalGlobal(T_data fieldX[])
+    {
+        reset_neumann_host(fieldX);
+        const auto L = g_.getIndexLimitsData();
+        T_data errorLocal = 0, errorLocalMax = -1;
+        for (int k = L[4]; k < L[5]; ++k)
+            for (int j = L[2]; j < L[3]; ++j)
+                for (int i = L[0]; i < L[1]; ++i) {
+                    const T_data u = trueSolution_(g_.coord(0, i), g_.coord(1, j), g_.coord(2, k));
+                    const T_data err = std::abs(fieldX[idx(i, j, k)] - u);
+                    errorLocal += err;
+                    if (err > errorLocalMax) errorLocalMax = err;
+                }
+        std::cout << "Max error local block avg " << errorLocal / g_.getNtotLocalNoGuards() << " in rank " << 0
+                  << std::endl;
+        std::cout << "Max error local point " << errorLocalMax << " in rank " << 0 << std::endl;
+        return errorLocal;
+    }
+
+    // writeResidualHistory (iterativeSolverBaseAlpaka.hpp:620-638): duration,
+    // main iterations, preconditioner iterations, then the recursive residual
+    // of iterations 0, 1, ... up to the first non-positive entry or
+    // maxIteration entries, one value per line in default stream format
+    void writeResidualHistory(const std::string& path = "residualHistory.txt") const
+    {
+        std::ofstream outFile(path);
+        if (!outFile) std::cerr << "Error: Could not open the file!" << std::endl;
+        outFile << getDurationSolver().count() << "\n";
+        outFile << getNumIterationFinal() << "\n";
+        outFile << getNumIterationPreconditionerFinal() << "\n";
+        const int n = std::min<int>(maxIteration_, (int)history_.size());
+        for (int i = 0; i < n && history_[i] > 0; i++) outFile << history_[i] << "\n";
+        outFile.close();
+        std::cout << "residualHistory has been written to " << path << std::endl;
+    }
+
+    // pps_stage::true_finish of the main solver (CG, BiCGSTAB, FCG): after the
+    // reference's recursive stop, resume from r = b - A x until the true FP64
+    // residual is below the tolerance, at most n times (0, the default: the
+    // reference's stop rule)
+    void setTrueResidualFinish(int n) { chain_[0].true_finish = n; }
+
+    TimeCounter<DIM, T_data> timeCounter;
     int getOuterIterations() const { return outerIterations_; }
     int getFallbackSwitches() const { return fallbackSwitches_; }
     // the device context (kernel-level entry points, pps_nn_attach)
@@ -223,6 +390,32 @@ private:
     {
         const auto ng = g_.getNlocalGuards();
         return i + ng[0] * (j + ng[1] * k);
+    }
+    // resetNeumanBCs<true,true> on a host array (iterativeSolverBase.hpp:62-169,
+    // orderNeumanBcs 2, normFieldB_ 1 after the solve): guard = mirrored data
+    // point -+ 2 ds du/dn
+    void reset_neumann_host(T_data* f) const
+    {
+        const auto L = g_.getIndexLimitsData();
+        const auto ng = g_.getNlocalGuards();
+        const int stride[3] = {1, ng[0], ng[0] * ng[1]};
+        for (int face = 0; face < 6; ++face) {
+            if (g_.getBcsType()[face] != 1) continue;
+            const int d = face / 2, side = face % 2;
+            int lo[3] = {L[0], L[2], L[4]}, hi[3] = {L[1], L[3], L[5]};
+            const int pos = side ? L[2 * d + 1] - 1 : L[2 * d];
+            lo[d] = pos;
+            hi[d] = pos + 1;
+            const T_data ds = g_.getDs()[d];
+            for (int k = lo[2]; k < hi[2]; ++k)
+                for (int j = lo[1]; j < hi[1]; ++j)
+                    for (int i = lo[0]; i < hi[0]; ++i) {
+                        const int p = idx(i, j, k);
+                        const T_data q = trueDdir_(g_.coord(0, i), g_.coord(1, j), g_.coord(2, k), d);
+                        if (side == 0) f[p - stride[d]] = f[p + stride[d]] - 2 * ds * q;
+                        else f[p + stride[d]] = f[p - stride[d]] + 2 * ds * q;
+                    }
+        }
     }
     void check(int rc)
     {
@@ -258,8 +451,9 @@ private:
     void* ctx_ = nullptr;
     int numIterationFinal_ = 0, numIterationPreconditionerFinal_ = 0, outerIterations_ = 0, fallbackSwitches_ = 0;
     T_data errorFromIteration_ = -1, errorComputeOperator_ = -1;
-    double durationSolver_ = 0;
+    std::chrono::duration<double> durationSolver_{0};
     std::vector<double> history_;
+    int maxIteration_ = 0;
 };
 
 // ------------------------------------------------------------- solver types
@@ -344,6 +538,66 @@ struct LearnedPreconditioner {
         s.stall_window = stallWindow;
         s.stall_ratio = stallRatioPercent / 100.0;
         v.push_back(s);
+    }
+};
+
+// ------------------------------------------------------- alpaka solver types
+//
+// The aliases of solverPoissonMPI_alpaka/include/inputParam.hpp:25-37 with
+// the alpaka template parameters.  Their constructors take the AlpakaHelper,
+// and a main solver's operator()(x, b) sets the problem itself.
+
+// NoneSolverAlpaka<DIM,T_data,T_data_noneSolver,tolerance,maxIteration> (noneSolverAlpaka.hpp:17)
+template <int DIM, typename T_data, typename T_data_noneSolver, int tolerance, int maxIteration>
+struct NoneSolverAlpaka {
+    static void stages(std::vector<pps_stage>& v)
+    {
+        v.push_back(detail::stage(PPS_NONE, detail::precision<T_data_noneSolver>(), 0, 1));
+    }
+};
+
+// BiCGstabAlpaka<DIM,T_data,T_data_BiCGStab,tolerance,maxIteration,isMainLoop,communicationON,T_Preconditioner>
+// (BiCGstabAlpaka.hpp:19)
+template <int DIM, typename T_data, typename T_data_BiCGStab, int tolerance, int maxIteration, bool isMainLoop,
+          bool communicationON, typename T_Preconditioner>
+class BiCGstabAlpaka
+    : public SolverFacade<BiCGstabAlpaka<DIM, T_data, T_data_BiCGStab, tolerance, maxIteration, isMainLoop,
+                                         communicationON, T_Preconditioner>,
+                          DIM, double> {
+public:
+    static constexpr bool kSetsProblemInside = isMainLoop;
+    using SolverFacade<BiCGstabAlpaka, DIM, double>::SolverFacade;
+    static void stages(std::vector<pps_stage>& v)
+    {
+        static_assert(std::is_same<T_data_BiCGStab, double>::value, "BiCGSTAB vectors are FP64");
+        v.push_back(detail::stage(PPS_BICGSTAB, 64, tolerance * tollScalingFactor, maxIteration));
+        T_Preconditioner::stages(v);
+    }
+};
+
+// ChebyshevIterationAlpaka<DIM,T_data,T_data_chebyshev,global,tolerance,maxIteration,isMainLoop,
+// communicationON,T_Preconditioner> (chebyshevIterationAlpaka.hpp:16).  global: the rescaled global
+// eigenvalue bounds (epsilon, rescaleEigMin/Max), local: the block's own bounds, not rescaled
+// (:26-30); T_data_chebyshev = float runs the sweeps in FP32 on a cast copy of the input (:149-196)
+template <int DIM, typename T_data, typename T_data_chebyshev, bool global, int tolerance, int maxIteration,
+          bool isMainLoop, bool communicationON, typename T_Preconditioner>
+class ChebyshevIterationAlpaka
+    : public SolverFacade<ChebyshevIterationAlpaka<DIM, T_data, T_data_chebyshev, global, tolerance, maxIteration,
+                                                   isMainLoop, communicationON, T_Preconditioner>,
+                          DIM, double> {
+public:
+    static constexpr bool kSetsProblemInside = isMainLoop;
+    using SolverFacade<ChebyshevIterationAlpaka, DIM, double>::SolverFacade;
+    static void stages(std::vector<pps_stage>& v)
+    {
+        pps_stage s = detail::stage(PPS_CHEBYSHEV, detail::precision<T_data_chebyshev>(), tolerance * tollScalingFactor,
+                                    maxIteration);
+        s.cheb_eps = alpakaEpsilon;
+        s.cheb_rescale_min = alpakaRescaleEigMin;
+        s.cheb_rescale_max = alpakaRescaleEigMax;
+        s.cheb_local = global ? 0 : 1;
+        v.push_back(s);
+        T_Preconditioner::stages(v);
     }
 };
 

@@ -12,6 +12,7 @@
 
 #include <chrono>
 #include <cstdio>
+#include <fstream>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -50,6 +51,7 @@ struct Opts {
     std::array<T_data, 3> origin{0, 0, 0};
     std::array<int, 6> bc{0, 0, 0, 0, 0, 0};
     std::string solver = "cg", precond = "none", rhs = "function", out;
+    std::string solutionDat, residualHistory;  // the alpaka build's output files
     int problem = 0, chebM = 11, nsrc = 64, specK = 16;
     unsigned long long seed = 1234;
     double tol = 1e-10, wmin = 0.01, wmax = 0.1;
@@ -165,6 +167,35 @@ static int run(const Opts& o, Grid& grid, Comm& comm, const Exact& exact, int ra
     const double tSolve = std::chrono::duration<double>(t1 - t0).count();
     if (rank == 0) std::cout << "Solver time: " << tSolve << " seconds" << std::endl;
 
+    if (!o.residualHistory.empty() && rank == 0) {
+        // writeResidualHistory (solverPoissonMPI_alpaka/include/iterativeSolverBaseAlpaka.hpp:620-638)
+        // applied to this solver's history.  The alpaka history array is zero
+        // filled past the last iteration (:75-76); the CPU one is not
+        // (iterativeSolverBase.hpp:43), so only the entries the solve wrote,
+        // 0..iters, are read.  Preconditioner iterations as the alpaka main
+        // loop counts them: Chebyshev adds maxIteration per application
+        // (BiCGstabAlpaka.hpp:214,290), two applications per BiCGSTAB iteration.
+        long precIters = 0;
+        if (o.precond == "cheb") precIters = (long)o.chebM * (o.solver == "bicgstab" ? 2L * iters : iters + 1L);
+        std::ofstream outFile(o.residualHistory);
+        outFile << solver.getDurationSolver().count() << "\n";
+        outFile << iters << "\n";
+        outFile << precIters << "\n";
+        for (int i = 0; i < REF_MAXIT && i <= iters && solver.history()[i] > 0; i++) outFile << solver.history()[i] << "\n";
+    }
+    if (!o.solutionDat.empty()) {
+        // solution.dat (solverPoissonMPI_alpaka/src/main.cpp:135-146): each rank's
+        // guarded array at offset sizeof(T_data) * ntotLocalGuards * rank
+        MPI_File fileSolution;
+        MPI_Status status;
+        MPI_Offset offset = sizeof(T_data) * (MPI_Offset)grid.getNtotLocalGuards() * rank;
+        MPI_File_open(MPI_COMM_WORLD, o.solutionDat.c_str(), MPI_MODE_CREATE | MPI_MODE_WRONLY, MPI_INFO_NULL,
+                      &fileSolution);
+        MPI_File_seek(fileSolution, offset, MPI_SEEK_SET);
+        MPI_File_write(fileSolution, x.data(), grid.getNtotLocalGuards(), getMPIType<T_data>(), &status);
+        MPI_File_close(&fileSolution);
+    }
+
     if (!o.out.empty()) {
         char path[4096];
         std::snprintf(path, sizeof path, "%s.x.r%d.bin", o.out.c_str(), rank);
@@ -262,6 +293,8 @@ int main(int argc, char** argv)
         else if (k == "--cheb-rescale") { need(2); rescaleEigMin = std::atof(argv[++a]); rescaleEigMax = std::atof(argv[++a]); }
         else if (k == "--tol") { need(1); o.tol = std::atof(argv[++a]); }
         else if (k == "--out") { need(1); o.out = argv[++a]; }
+        else if (k == "--solution-dat") { need(1); o.solutionDat = argv[++a]; }
+        else if (k == "--residual-history") { need(1); o.residualHistory = argv[++a]; }
         else { std::cerr << "unknown option " << k << std::endl; MPI_Abort(MPI_COMM_WORLD, 2); }
     }
     tollScalingFactor = o.tol;

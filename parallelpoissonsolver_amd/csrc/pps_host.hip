@@ -58,6 +58,12 @@ struct Stage {
     virtual bool pointwise(double* c) const { return false; }
     // needs host control per application (Krylov preconditioners)
     virtual bool host_driven() const { return false; }
+    // Preconditioner iterations one application adds to
+    // getNumIterationPreconditionerFinal.  Applications of a stage that is not
+    // host driven are queued ahead of the caller's convergence test and gated,
+    // so the caller adds calls * iters_per_call() once it knows how many of
+    // its applications ran.
+    virtual long long iters_per_call() const { return 0; }
     // operator()(out, in) as a preconditioner; kernels gated on *gate
     virtual int apply(Field out, Field in, const int* gate) = 0;
     // apply() that may also sum out . dot into the context's reduction
@@ -428,7 +434,6 @@ struct ChebStage : Stage {
         int rc = buffers();
         if (rc) return rc;
         calls++;
-        prec_iters += cfg.max_iter;
         // resetNeumanBCs on the input (chebyshevIteration.hpp:74), in place
         if (prec() == 64 && in.prec == 64) {
             C.ghost(in, false, nullptr, gate);
@@ -451,6 +456,9 @@ struct ChebStage : Stage {
         return PPS_OK;
     }
     int solve_main() override;
+    // maxIteration per call (BiCGstabAlpaka.hpp:214,290 add the Chebyshev
+    // stage's numIterationFinal_ = maxIteration, chebyshevIterationAlpaka.hpp:282)
+    long long iters_per_call() const override { return cfg.max_iter; }
 };
 
 // ------------------------------------------------------------------ Krylov base
@@ -475,6 +483,11 @@ struct KrylovStage : Stage {
         return pre->pointwise(c);
     }
     bool pre_host() const { return pre && pre->host_driven(); }
+    // `calls` applications of a queued (not host-driven) preconditioner ran
+    void count_pre(long long calls)
+    {
+        if (pre && !pre->host_driven()) pre->prec_iters += calls * pre->iters_per_call();
+    }
     int pre_apply(Field out, Field in, const int* gate)
     {
         if (!pre) {
@@ -577,12 +590,18 @@ struct CGStage : KrylovStage {
             return PPS_OK;
         };
         Scal h;
+        // preconditioner applications: the one before the loop, then one per
+        // iteration except the stopping one (gated), one per replacement
+        long long pre_calls = pw ? 0 : 1;
+        int it0 = 0;
         for (int repl = 0;; ++repl) {
             rc = run_loop(C, S, cfg.max_iter, pre_host(), body);
             if (rc) return rc;
             // the last executed iteration K-1 left x += alpha_{K-1} p_{K-1} pending; p_{K-1} is p[K&1]
             rc = scal_read(C, S, h);
             if (rc) return rc;
+            if (!pw) pre_calls += std::max(0, h.iter - it0 - 1);
+            it0 = h.iter;
             if (h.iter > 0) launch_cg_final_x<double>(g, C.stream, x.d(), p[h.iter & 1].d(), S);
             if (!main || repl >= cfg.true_finish) break;
             // true-residual finish: r = b - A x of the complete x replaces the
@@ -594,9 +613,11 @@ struct CGStage : KrylovStage {
                 rc = pre_apply(z, r, &S->done);
                 if (rc) return rc;
                 launch_cg_rz<double>(g, C.stream, C.rb, r.d(), z.d(), S);
+                ++pre_calls;
             }
             cur = h.iter & 1;
         }
+        count_pre(pre_calls);
         epilogue(main, x, b, r);
         return PPS_OK;
     }
@@ -684,6 +705,13 @@ struct BiCGStage : KrylovStage {
             if (h.done) break;
             launch_copy<double>(g, C.stream, r.d(), p.d());
             launch_copy<double>(g, C.stream, r.d(), r0.d());
+        }
+        {
+            // two applications in every iteration that ran (BiCGSTAB.hpp:170,190)
+            Scal h;
+            rc = scal_read(C, S, h);
+            if (rc) return rc;
+            if (!pw) count_pre(2ll * h.iter);
         }
         // resetNeumanBCs<isMainLoop,true>(fieldX) (BiCGSTAB.hpp:301) before the true residual
         C.ghost(x, main, S, nullptr);
@@ -780,6 +808,7 @@ struct FCGStage : KrylovStage {
             cur = h.iter & 1;
         }
         C.fallback_switches += h.fallback;
+        count_pre(h.iter);  // one application per iteration that ran
         epilogue(main, x, b, r);
         return PPS_OK;
     }
@@ -856,6 +885,7 @@ __global__ void k_widen(const float* __restrict__ a, double* __restrict__ b, lon
 
 struct NNStage : Stage {
     using Stage::Stage;
+    long long iters_per_call() const override { return 1; }
     int apply(Field out, Field in, const int* gate) override
     {
         int nb;
@@ -884,7 +914,6 @@ struct NNStage : Stage {
         if (!gate) return C.fail(PPS_ERR_STATE, "PPS_NN stage called outside a Krylov stage");
         if (out.prec != in.prec) return C.fail(PPS_ERR_ARG, "PPS_NN: input and output precision differ");
         calls++;
-        prec_iters++;
         Scal* caller = caller_of(gate);
         const Layout& L = C.lay(in.prec);
         NNFieldGeom g;
@@ -1020,6 +1049,12 @@ struct IRStage : KrylovStage {
         if (flexible) C.fallback_switches += h.fallback;
         in->prec_iters += h.iter;
         in->calls++;
+        // applications of M that ran: CG applies it once before its loop and in
+        // every iteration but the stopping one, FCG once per iteration
+        if (M && !M->host_driven()) {
+            const long long calls = flexible ? h.iter : (pw ? 0 : 1 + std::max(0, h.iter - 1));
+            M->prec_iters += calls * M->iters_per_call();
+        }
         return PPS_OK;
     }
     int solve_main() override
@@ -1492,6 +1527,7 @@ int pps_precondition(void* ctx, const pps_stage* chain, int nstages, int out, in
     if (out == in) return C->fail(PPS_ERR_ARG, "pps_precondition: out and in must differ");
     rc = st[0]->apply(fo, fi, nullptr);
     if (rc) return rc;
+    if (!st[0]->host_driven()) st[0]->prec_iters += st[0]->iters_per_call();
     rc = C->hip(hipStreamSynchronize(C->stream), "sync");
     if (!rc) rc = C->hip(hipGetLastError(), "kernel");
     if (rc) return rc;

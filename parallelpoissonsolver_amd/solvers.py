@@ -353,7 +353,9 @@ class Solver:
     def getDurationSolver(self): return self.durationSolver_
     def getErrorFromIterationHistory(self): return self.errorFromIterationHistory_
 
-    # iterativeSolverBase.hpp:283-408 (one rank)
+    # iterativeSolverBase.hpp:283-408 (one rank): prints the largest block-average
+    # and point errors against trueSolutionFxyz on the data region and returns
+    # the sum of |x - u| (errorLocal)
     def checkSolutionLocalGlobal(self, fieldX, verbose=True):
         g = self.blockGrid
         X = self._view(fieldX)
@@ -362,17 +364,20 @@ class Solver:
         sl = (slice(L[4], L[5]), slice(L[2], L[3]), slice(L[0], L[1]))
         Z, Y, Xc = np.meshgrid(ax[2][sl[0]], ax[1][sl[1]], ax[0][sl[2]], indexing="ij")
         err = np.abs(X[sl] - self.exact.trueSolutionFxyz(Xc, Y, Z))
-        block_avg = float(err.sum()) / g.getNtotLocalNoGuards()
-        point = float(err.max())
+        error_local = float(err.sum())
         if verbose:
-            print(f"Max error local block avg {block_avg:g} in rank 0")
-            print(f"Max error local point {point:g} in rank 0")
-        return block_avg, point
+            print(f"Max error local block avg {error_local / g.getNtotLocalNoGuards():g} in rank 0")
+            print(f"Max error local point {float(err.max()):g} in rank 0")
+        return error_local
 
-    # iterativeSolverBaseAlpaka.hpp:620-638
+    # iterativeSolverBaseAlpaka.hpp:620-638: duration, iterations, preconditioner
+    # iterations, then the history up to the first non-positive entry and at most
+    # maxIteration entries
     def writeResidualHistory(self, path="residualHistory.txt"):
+        maxit = self.spec.chain()[0].get("maxit", 0)
         with open(path, "w") as f:
             f.write(f"{self.durationSolver_:g}\n{self.numIterationFinal_}\n{self.numIterationPreconditionerFinal_}\n")
-            for v in self.errorFromIterationHistory_:
-                if v > 0:
-                    f.write(f"{v:g}\n")
+            for i, v in enumerate(self.errorFromIterationHistory_):
+                if i >= maxit or not v > 0:
+                    break
+                f.write(f"{v:g}\n")
