@@ -143,15 +143,40 @@ def nn_layers(C, reps):
     return out
 
 
+def host_cpu():
+    """CPU model and physical core count of this host (from /proc/cpuinfo)."""
+    model, cores = "", set()
+    phys = core = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and not model:
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and phys is not None and core is not None:
+                cores.add((phys, core))
+                phys = core = None
+    except OSError:
+        pass
+    if phys is not None and core is not None:
+        cores.add((phys, core))
+    return model, len(cores) or None
+
+
 def cpu_baseline(gpu_iterations):
     """Reference CPU solver (compiled from the reference sources into oracle/_ref)
-    on the host cores: 10 CG iterations of the same 512^3 workload under MPI,
-    extrapolated to the iteration count the solve needs."""
+    on the host cores: 100 CG iterations of the same 512^3 workload under MPI,
+    extrapolated to the iteration count the solve needs.  Fails loudly when the
+    reference build or mpirun is missing (pass --no-cpu to skip the baseline)."""
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_solver_sample")
     mpirun = shutil.which("mpirun") or "/opt/conda/bin/mpirun"
     if not (os.path.exists(exe) and os.path.exists(mpirun)):
-        log("CPU baseline skipped: oracle/_ref/ref_solver_sample or mpirun missing")
-        return None
+        raise RuntimeError(f"CPU baseline: {exe} or mpirun missing (build with __graft_entry__.build() where the "
+                           f"reference sources are present, or pass --no-cpu)")
     npg = N_INTERIOR + 2
     ds = 1.0 / (npg - 1)
     ranks = (2, 2, 2)  # 514 = 2 * 257: the reference needs ranks dividing npglobal (blockGrid.hpp:165)
@@ -161,13 +186,12 @@ def cpu_baseline(gpu_iterations):
         cmd = [mpirun, "-n", str(nproc), exe, "--ranks", *map(str, ranks), "--np", str(npg), str(npg), str(npg),
                "--ds", repr(ds), repr(ds), repr(ds), "--problem", "2", "--rhs", "gauss", "--seed", str(SEED),
                "--nsrc", str(NSRC), "--solver", "cg", "--precond", "none", "--tol", repr(TOL), "--out", prefix]
-        log("CPU baseline: " + " ".join(cmd[:3]) + " ... (reference CPU CG, 10 iterations)")
+        log("CPU baseline: " + " ".join(cmd[:3]) + " ... (reference CPU CG, 100 iterations)")
         t0 = time.perf_counter()
         p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
         wall = time.perf_counter() - t0
         if p.returncode != 0:
-            log(f"CPU baseline failed: {p.stderr[-500:]}")
-            return None
+            raise RuntimeError(f"CPU baseline failed: {p.stderr[-500:]}")
         meta = json.load(open(prefix + ".json"))
     it = meta["iterations"]
     per_it = meta["solver_in_function_time"] / it
@@ -175,19 +199,14 @@ def cpu_baseline(gpu_iterations):
     est = setup + per_it * gpu_iterations
     log(f"CPU baseline: {it} iterations in {meta['solver_in_function_time']:.2f} s ({per_it:.3f} s/it), "
         f"extrapolated {est:.1f} s for {gpu_iterations} iterations (run took {wall:.1f} s)")
-    cpu_model = ""
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu_model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+    cpu_model, phys = host_cpu()
     return {"value": est, "unit": "s", "cores": nproc, "kind": "reference",
             "sample": f"reference solverPoissonMPI_CPU BaseCG+NoneSolver (plain CG), 512^3 Gaussian-source rhs, "
-                      f"{nproc} MPI ranks (2x2x2), {it} iterations timed ({per_it:.3f} s/iteration) + setup, "
-                      f"extrapolated to the {gpu_iterations} iterations of the full plain-CG solve",
-            "cpu_model": cpu_model}
+                      f"{nproc} MPI ranks (2x2x2, one thread each: 514 = 2 * 257 admits no other layout), "
+                      f"{it} iterations timed ({per_it:.4f} s/iteration) + setup, extrapolated to the "
+                      f"{gpu_iterations} iterations of the full plain-CG solve",
+            "cpu_model": cpu_model, "host_physical_cores": phys, "host_logical_cpus": os.cpu_count(),
+            "iterations_timed": it, "seconds_per_iteration": round(per_it, 5)}
 
 
 def network():
@@ -282,11 +301,8 @@ def main():
 
     cpu = None
     if not a.no_cpu and a.n == N_INTERIOR and a.rhs == "gauss":
-        try:
-            ref_it = results["plain_fp64_cg"]["iterations"] if "plain_fp64_cg" in results else 2406
-            cpu = cpu_baseline(ref_it)
-        except Exception as e:  # the baseline is reported, never the product
-            log(f"CPU baseline error: {e}")
+        ref_it = results["plain_fp64_cg"]["iterations"] if "plain_fp64_cg" in results else 2406
+        cpu = cpu_baseline(ref_it)
     C.close()
 
     hv = results[headline]

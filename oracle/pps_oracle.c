@@ -1053,3 +1053,12 @@ void og_rhs_spectrum(const og_grid* g, uint64_t seed, int K, double* b)
     }
     free(C); free(Sx); free(Sy); free(Sz); free(T1); free(T2);
 }
+
+/* The recipes' random parameters, exported for the tests that check their laws */
+uint64_t og_mix64(uint64_t z) { return pps_mix64(z); }
+double og_rng_normal(uint64_t seed, uint64_t stream, uint64_t ctr) { return pps_rng_normal(seed, stream, ctr); }
+void og_rhs_gauss_source(uint64_t seed, int s, double wmin, double wmax, double c[3], double* w, double* a)
+{
+    pps_rhs_gauss_source(seed, s, wmin, wmax, c, w, a);
+}
+double og_rhs_spec_coef(uint64_t seed, int K, int kx, int ky, int kz) { return pps_rhs_spec_coef(seed, K, kx, ky, kz); }

@@ -129,6 +129,12 @@ int og_precondition(const og_grid* g, const og_stage* chain, int nstages, double
 void og_rhs_normal(const og_grid* g, uint64_t seed, double* b);
 void og_rhs_gauss(const og_grid* g, uint64_t seed, int nsrc, double wmin, double wmax, double* b);
 void og_rhs_spectrum(const og_grid* g, uint64_t seed, int K, double* b);
+/* the recipes' random parameters (pps_rhs.h): the splitmix64 finaliser, the
+ * normal draw, one Gaussian source, one spectrum coefficient */
+uint64_t og_mix64(uint64_t z);
+double og_rng_normal(uint64_t seed, uint64_t stream, uint64_t ctr);
+void og_rhs_gauss_source(uint64_t seed, int s, double wmin, double wmax, double c[3], double* w, double* a);
+double og_rhs_spec_coef(uint64_t seed, int K, int kx, int ky, int kz);
 
 #ifdef __cplusplus
 }
