@@ -67,6 +67,12 @@ def chains():
         "ir_nnfcg": [dict(kind=L.IR, tol=TOL, maxit=100, first_tol=2e-3),
                      dict(kind=L.FCG, precision=32, tol=1e-4, maxit=20000),
                      dict(kind=L.NN, precision=32, stall_window=50, stall_ratio=0.9)],
+        # the reference's default GPU method (solverPoissonMPI_alpaka/include/inputParam.hpp:36,
+        # solverSetup.hpp:58-73): BiCGSTAB + Chebyshev(24, global eigenvalues rescaled by
+        # 100 and 1 - 1e-4, epsilon 0) in FP64
+        "bicgstab_cheb24_fp64": [dict(kind=L.BICGSTAB, tol=TOL, maxit=20000, true_finish=FINISH),
+                                 dict(kind=L.CHEBYSHEV, maxit=24, cheb_eps=0.0, cheb_rmin=100.0,
+                                      cheb_rmax=1 - 1e-4)],
     }
 
 
@@ -104,7 +110,8 @@ def run_form(C, chain, steps, warmup, name):
 
 
 KERNELS = {"cg_stencil_f64": 0, "cg_update_f64": 1, "residual_f64": 2, "copy_f64": 3, "cheb_sweep_f64": 4,
-           "cg_stencil_f32": 5, "cg_update_f32": 6}  # PPS_K_* ids of include/pps_abi.h
+           "cg_stencil_f32": 5, "cg_update_f32": 6, "fcg_stencil_f64": 7, "fcg_stencil_f32": 8,
+           "copy_flat_f64": 9, "cheb_pair_f64": 10}  # PPS_K_* ids of include/pps_abi.h
 
 
 def profiled_traffic(kernel, n):
@@ -230,7 +237,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=N_INTERIOR, help="interior points per direction")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="ir_nnfcg,nn_fcg_fp64,plain_fp64_cg")
+    ap.add_argument("--forms", default="ir_nnfcg,nn_fcg_fp64,plain_fp64_cg,bicgstab_cheb24_fp64")
     ap.add_argument("--reps", type=int, default=20, help="launches per kernel timing")
     ap.add_argument("--kernels-only", action="store_true", help="only the per-kernel timings")
     ap.add_argument("--rhs", default="gauss", choices=("gauss", "multiscale"),
@@ -276,13 +283,13 @@ def main():
     layers = nn_layers(C, a.reps) if net is not None else {}
     # dominant kernel of the headline solve: largest device time per iteration.
     # Per learned-FCG iteration: one network apply (each layer once; the output
-    # layer also sums gamma and the lift forms the residual update) and the FCG
-    # p-stencil pass (the CG stencil pass is its timing proxy); per plain-CG
-    # iteration: the fused stencil and update.
+    # layer also sums gamma and the lift forms the residual update) and the
+    # fused FCG p-stencil pass; per plain-CG iteration: the fused stencil and
+    # update.
     if headline in ("ir_nnfcg", "nn_fcg_fp64"):
         per_it = {f"nn_{k}": v for k, v in layers.items()}
         sfx = "f32" if headline == "ir_nnfcg" else "f64"
-        per_it[f"cg_stencil_{sfx}"] = kern[f"cg_stencil_{sfx}"]
+        per_it[f"fcg_stencil_{sfx}"] = kern[f"fcg_stencil_{sfx}"]
     elif headline == "ir_fp32cg_fp64":
         per_it = {k: kern[k] for k in ("cg_stencil_f32", "cg_update_f32")}
     else:
@@ -296,8 +303,8 @@ def main():
         roof["mfma_tflops"] = round(k["tflops"], 2)
         roof["mfma_frac"] = round(k["tflops"] / PEAK_BF16_TFLOPS, 4)
     # the spec peak next to a measured one: this box's plain copy kernel
-    if "copy_f64" in kern:
-        roof["peak_measured_copy"] = round(kern["copy_f64"]["gbs"], 1)
+    if "copy_flat_f64" in kern:
+        roof["peak_measured_copy"] = round(kern["copy_flat_f64"]["gbs"], 1)
 
     cpu = None
     if not a.no_cpu and a.n == N_INTERIOR and a.rhs == "gauss":
@@ -332,6 +339,13 @@ def main():
         "nn_layers": {k2: {"gbs": round(v["gbs"], 1), "tflops": round(v["tflops"], 2), "ms": round(v["ms"], 4)}
                       for k2, v in layers.items()},
         "roofline": roof,
+        # Chebyshev sweeps at this size: one per launch (4 fields) and two fused per launch (5 fields)
+        "chebyshev": ({"single_sweep_ms": round(kern["cheb_sweep_f64"]["ms"], 4),
+                       "single_sweep_gbs": round(kern["cheb_sweep_f64"]["gbs"], 1),
+                       "pair_ms": round(kern["cheb_pair_f64"]["ms"], 4),
+                       "pair_gbs": round(kern["cheb_pair_f64"]["gbs"], 1),
+                       "per_sweep_ms_fused": round(kern["cheb_pair_f64"]["ms"] / 2, 4)}
+                      if "cheb_pair_f64" in kern else None),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

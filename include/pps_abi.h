@@ -192,14 +192,28 @@ enum {
     PPS_K_COPY = 3,         /* plain copy of the solver region (reference) */
     PPS_K_CHEB_SWEEP = 4,   /* Chebyshev sweep                            */
     PPS_K_CG_STENCIL_F32 = 5,
-    PPS_K_CG_UPDATE_F32 = 6
+    PPS_K_CG_UPDATE_F32 = 6,
+    PPS_K_FCG_STENCIL = 7,     /* FCG p-stencil as the learned FCG runs it: p = z - beta p_old,
+                                  Ap, p.Ap, p.r, r.Ap, Ap.Ap, r.r, x += alpha_prev p_old */
+    PPS_K_FCG_STENCIL_F32 = 8,
+    PPS_K_COPY_FLAT = 9,       /* plain 1-D copy of a whole field allocation (16-byte
+                                  non-temporal lanes): the measured HBM ceiling */
+    PPS_K_CHEB_PAIR = 10       /* two fused Chebyshev sweeps: read Y, Z, B ; write y_{c+1}, y_{c+2} */
 };
 /* Average device time (ms) of `reps` back-to-back launches of one kernel on
  * the context stream, measured with HIP events; *bytes = algorithmic bytes
  * moved per launch. */
 int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes);
-/* z-chunk lengths of the stencil and vector kernels (0 keeps the default) */
-int pps_set_tiling(void* ctx, int kz_stencil, int kz_vector);
+/* Launch shape: the z-chunk of the stencil kernels (0 keeps the current one)
+ * and the stores of the field outputs: nontemporal = 0 automatic (non-temporal
+ * once a field exceeds 32 MiB), 1 always non-temporal, -1 never. */
+int pps_set_tiling(void* ctx, int kz_stencil, int nontemporal);
+/* Chebyshev sweeps: pairs = 1 runs them two per pass (one fused kernel reads
+ * y_c, y_{c-1} and B once and writes y_{c+1} and y_{c+2}; the default), 0 one
+ * sweep per pass; kz = z-chunk of the fused kernel (0 keeps the current one).
+ * No counterpart in the reference: its sweeps are one kernel each
+ * (kernelsAlpakaChebyshev.hpp:191-273). */
+int pps_set_cheb_pairs(void* ctx, int pairs, int kz);
 
 /* ---- learned preconditioner (new: the reference has no counterpart) ------
  * A 3-level convolutional network (16 channels, 3^3 kernels, ReLU, BF16

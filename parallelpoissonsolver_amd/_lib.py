@@ -17,7 +17,8 @@ ERRORS = {-1: "PPS_ERR_ARG", -2: "PPS_ERR_HIP", -3: "PPS_ERR_NOMEM", -4: "PPS_ER
 
 FIELD_X, FIELD_B, FIELD_T0, FIELD_T1, FIELD_T2, FIELD_T3 = range(6)
 NONE, CG, BICGSTAB, CHEBYSHEV, JACOBI, FCG, IR, NN = range(8)
-K_CG_STENCIL, K_CG_UPDATE, K_RESIDUAL, K_COPY, K_CHEB_SWEEP, K_CG_STENCIL_F32, K_CG_UPDATE_F32 = range(7)
+(K_CG_STENCIL, K_CG_UPDATE, K_RESIDUAL, K_COPY, K_CHEB_SWEEP, K_CG_STENCIL_F32, K_CG_UPDATE_F32, K_FCG_STENCIL,
+ K_FCG_STENCIL_F32, K_COPY_FLAT, K_CHEB_PAIR) = range(11)
 
 
 class PPSError(RuntimeError):
@@ -128,6 +129,7 @@ SIGNATURES = {
     "pps_rhs_spectrum": ([_VP, ctypes.c_uint64, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "pps_time_kernel": ([_VP, ctypes.c_int, ctypes.c_int, _DP, _DP], ctypes.c_int),
     "pps_set_tiling": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "pps_set_cheb_pairs": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "pps_nn_create": ([ctypes.c_uint64, ctypes.POINTER(_VP)], ctypes.c_int),
     "pps_nn_destroy": ([_VP], None),
     "pps_nn_last_error": ([_VP], ctypes.c_char_p),
@@ -333,8 +335,13 @@ class Context:
                                           ctypes.byref(fl)))
         return ms.value, nb.value, fl.value
 
-    def set_tiling(self, kz_stencil=0, kz_vector=0):
-        self._chk(lib().pps_set_tiling(self.h, kz_stencil, kz_vector))
+    def set_cheb_pairs(self, pairs=1, kz=0):
+        """Chebyshev sweeps two per pass (1, the default) or one per pass (0); kz: z-chunk of the fused kernel."""
+        self._chk(lib().pps_set_cheb_pairs(self.h, int(pairs), int(kz)))
+
+    def set_tiling(self, kz_stencil=0, nontemporal=0):
+        """Stencil z-chunk (0 keeps it); output stores: 0 automatic, 1 non-temporal, -1 plain."""
+        self._chk(lib().pps_set_tiling(self.h, kz_stencil, nontemporal))
 
     def synchronize(self):
         self._chk(lib().pps_synchronize(self.h))

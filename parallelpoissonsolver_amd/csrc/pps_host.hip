@@ -114,8 +114,11 @@ struct Ctx {
     int hist_cap = 0;
     int hist_len = 0;
     int kz_st = 16;   // z-chunk of stencil launches
-    int kz_vec = 32;  // z-chunk of vector launches
-    int nt = 0;       // non-temporal stores (set when a field exceeds the caches)
+    int nt = 0;       // non-temporal stores of field outputs
+    int nt_mode = 0;  // 0: automatic (nt_auto), 1: always, -1: never
+    int nt_auto = 0;  // the fields' working set exceeds the last-level cache
+    int cheb_pairs = 1;  // Chebyshev sweeps run two per pass (k_cheb_pair)
+    int cheb_kz = 32;    // z-chunk of the fused Chebyshev pair
     std::string err;
     std::shared_ptr<NNPrecond> nn;  // learned preconditioner of PPS_NN stages (pps_nn_attach)
     int fallback_switches = 0;      // FCG stages that switched to the fallback, this solve
@@ -136,10 +139,25 @@ struct Ctx {
         return fail(e == hipErrorOutOfMemory ? PPS_ERR_NOMEM : PPS_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
     }
     const Layout& lay(int prec) const { return prec == 32 ? L32 : L64; }
-    Geo geo(const Region& R, int prec) const { return make_geo(lay(prec), R, coef, kz_st, kz_vec, nt); }
+    Geo geo(const Region& R, int prec) const { return make_geo(lay(prec), R, coef, kz_st, nt); }
     Geo gsolver(int prec) const { return geo(solver, prec); }
     Geo gbox(int prec) const { return geo(box, prec); }
     Geo gdata(int prec) const { return geo(data, prec); }
+    ChebPairGeo cheb_pair_geo() const
+    {
+        ChebPairGeo cg{};
+        for (int d = 0; d < 3; ++d) {
+            cg.lo_d[d] = lim_data[2 * d];
+            cg.hi_d[d] = lim_data[2 * d + 1];
+            cg.ng[d] = ng[d];
+        }
+        for (int f = 0; f < 6; ++f)
+            if (grid.bcs_type[f] == 1) cg.neu |= 1 << f;
+        cg.kz = cheb_kz;
+        return cg;
+    }
+    // the fused pair needs the Neumann mirror point (two cells inward) inside the data region
+    bool cheb_pairs_ok() const { return cheb_pairs && np[0] >= 2 && np[1] >= 2 && np[2] >= 2; }
 
     int alloc_field(Field& f, int prec)
     {
@@ -362,7 +380,7 @@ struct JacobiStage : Stage {
 
 struct ChebStage : Stage {
     double theta = 0, delta = 0, sigma = 0;
-    Field Y, Z, W, Bc;  // Bc: FP32 copy of the input when the sweeps run in FP32
+    Field Y, Z, W, V, Bc;  // V: fourth iterate buffer of fused pairs; Bc: FP32 copy of the input
     ChebStage(Ctx& c, const pps_stage& s, int l) : Stage(c, s, l)
     {
         // chebyshevIteration.hpp:22-26 (global, rescaled); alpaka local form
@@ -383,16 +401,19 @@ struct ChebStage : Stage {
         int rc = C.get_work(k + ".Y", P, Y);
         if (!rc) rc = C.get_work(k + ".Z", P, Z);
         if (!rc) rc = C.get_work(k + ".W", P, W);
+        if (!rc && C.cheb_pairs_ok()) rc = C.get_work(k + ".V", P, V);
         if (!rc && P == 32) rc = C.get_work(k + ".B", 32, Bc);
         return rc;
     }
     template <typename T, typename TO>
-    void sweeps(const T* B, TO* X, const int* gate, Field Yf, Field Zf, Field Wf)
+    void sweeps(const T* B, TO* X, const int* gate, Field Yf, Field Zf, Field Wf, Field Vf)
     {
         // y1 = B/theta, y2 = 2 rho1/delta (2B + A B/theta), y_{c+1} from the
         // three-term recurrence; the reference returns -W after its final
         // rotation, which is -y_{m-1} (chebyshevIteration.hpp:93-128): the two
         // sweeps computing y_m and y_{m+1} do not reach the output and are skipped.
+        // Sweeps c = 2 .. m-2 run two per pass (k_cheb_pair) where the grid
+        // allows it, one first when their number is odd.
         const int m = cfg.max_iter;
         const Geo g = C.gsolver(sizeof(T) == 4 ? 32 : 64);
         double rhoOld = 1 / sigma;
@@ -409,23 +430,51 @@ struct ChebStage : Stage {
         T* Yp = static_cast<T*>(Yf.p);
         T* Zp = static_cast<T*>(Zf.p);
         T* Wp = static_cast<T*>(Wf.p);
+        T* Vp = static_cast<T*>(Vf.p);
+        const bool pairs = C.cheb_pairs_ok() && Vp != nullptr;
+        const ChebPairGeo cg = C.cheb_pair_geo();
         launch_cheb_first<T>(g, C.stream, B, Zp, Yp, c0, c1, gate);
-        for (int c = 2; c <= m - 2; ++c) {
+        Field yf;
+        yf.prec = sizeof(T) == 4 ? 32 : 64;
+        double a[2], b[2], cz[2];
+        auto next = [&](int q) {
             rhoOld = rhoCurr;
             rhoCurr = 1 / (2 * sigma - rhoOld);
-            Field yf;
+            a[q] = rhoCurr * 2 * sigma;
+            b[q] = rhoCurr * 2 / delta;
+            cz[q] = rhoCurr * rhoOld;
+        };
+        for (int c = 2; c <= m - 2;) {
+            const int left = m - 2 - c + 1;
             yf.p = Yp;
-            yf.prec = sizeof(T) == 4 ? 32 : 64;
             C.ghost(yf, false, nullptr, gate);
-            const double a = rhoCurr * 2 * sigma, b = rhoCurr * 2 / delta, cz = rhoCurr * rhoOld;
-            if (c < m - 2) {
-                launch_cheb_sweep<T, T>(g, C.stream, Yp, B, Zp, Wp, a, b, cz, 1.0, gate);
-                T* t = Zp;  // swap(Z,Y); swap(W,Y)
-                Zp = Yp;
-                Yp = Wp;
-                Wp = t;
+            if (pairs && left % 2 == 0) {
+                next(0);
+                next(1);
+                if (left > 2) {
+                    launch_cheb_pair<T, T>(g, cg, C.stream, Yp, Zp, B, Wp, Vp, a, b, cz, 1.0, false, gate);
+                    T* y = Yp;  // y_{c+2} -> Y, y_{c+1} -> Z
+                    T* z = Zp;
+                    Yp = Vp;
+                    Zp = Wp;
+                    Wp = y;
+                    Vp = z;
+                } else {
+                    launch_cheb_pair<T, TO>(g, cg, C.stream, Yp, Zp, B, Wp, X, a, b, cz, -1.0, true, gate);
+                }
+                c += 2;
             } else {
-                launch_cheb_sweep<T, TO>(g, C.stream, Yp, B, Zp, X, a, b, cz, -1.0, gate);
+                next(0);
+                if (c < m - 2) {
+                    launch_cheb_sweep<T, T>(g, C.stream, Yp, B, Zp, Wp, a[0], b[0], cz[0], 1.0, gate);
+                    T* t = Zp;  // swap(Z,Y); swap(W,Y)
+                    Zp = Yp;
+                    Yp = Wp;
+                    Wp = t;
+                } else {
+                    launch_cheb_sweep<T, TO>(g, C.stream, Yp, B, Zp, X, a[0], b[0], cz[0], -1.0, gate);
+                }
+                c += 1;
             }
         }
     }
@@ -437,7 +486,7 @@ struct ChebStage : Stage {
         // resetNeumanBCs on the input (chebyshevIteration.hpp:74), in place
         if (prec() == 64 && in.prec == 64) {
             C.ghost(in, false, nullptr, gate);
-            if (out.prec == 64) sweeps<double, double>(in.d(), out.d(), gate, Y, Z, W);
+            if (out.prec == 64) sweeps<double, double>(in.d(), out.d(), gate, Y, Z, W, V);
             else return C.fail(PPS_ERR_UNSUPPORTED, "FP64 Chebyshev into an FP32 field");
         } else {
             // CastPrecisionFieldKernel (kernelsAlpakaChebyshev.hpp:7-18) then FP32 sweeps
@@ -450,8 +499,8 @@ struct ChebStage : Stage {
                 C.ghost(in, false, nullptr, gate);
                 B = in.f();
             }
-            if (out.prec == 64) sweeps<float, double>(B, out.d(), gate, Y, Z, W);
-            else sweeps<float, float>(B, out.f(), gate, Y, Z, W);
+            if (out.prec == 64) sweeps<float, double>(B, out.d(), gate, Y, Z, W, V);
+            else sweeps<float, float>(B, out.f(), gate, Y, Z, W, V);
         }
         return PPS_OK;
     }
@@ -1133,7 +1182,7 @@ int ChebStage::solve_main()
     hipLaunchKernelGGL(k_adjust_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, C.stream, Bt.d(), b.d(), n);
     adjust_b_field(C.gdata(64), C.stream, x.d(), Bt.d(), C.grid.bcs_type, C.lim_data, C.flux_d, C.fng, C.grid.ds);
     C.ghost(Bt, false, nullptr, nullptr);
-    sweeps<double, double>(Bt.d(), x.d(), nullptr, Y, Z, W);
+    sweeps<double, double>(Bt.d(), x.d(), nullptr, Y, Z, W, V);
     C.ghost(x, true, S, nullptr);
     launch_residual(C.gsolver(64), C.stream, C.rb, x.d(), b.d(), r.d(), S, 1);
     scal_op(C, S, SC_SET, S_ERR, 0.0);
@@ -1296,12 +1345,19 @@ int pps_create(const pps_grid* grid, void** out)
     rc = rc ? rc : X.hip(hipEventCreateWithFlags(&X.ev_batch[1], hipEventDisableTiming), "event");
     rc = rc ? rc : X.hip(hipEventCreate(&X.ev_t0), "event");
     rc = rc ? rc : X.hip(hipEventCreate(&X.ev_t1), "event");
-    // reduction workspace: enough partials for the largest launch
-    // partials for the largest launch: the whole box at the smallest kz allowed
-    const Geo gbig = make_geo(X.L64, X.box, X.coef, kMinKz, kMinKz, 0);
-    X.rb.cap = gbig.nblocks + 8;
-    // fields beyond the 256 MB last-level cache stream past the caches
-    X.nt = (X.L64.total * 8 > (256ll << 20)) ? 1 : 0;
+    // reduction workspace: partials for the largest launch (a vector kernel
+    // over the whole box, or a stencil at the smallest z-chunk allowed), and
+    // the span sums of the two-level epilogue
+    const Geo gbig = make_geo(X.L64, X.box, X.coef, kMinKz, 0);
+    const long long nvec = (long long)((X.ng[0] + kThreads - 1) / kThreads) * X.ng[1] * X.ng[2];
+    X.rb.cap = (int)std::max<long long>(gbig.nblocks, nvec) + 8;
+    X.rb.cap2 = X.rb.cap / kSpan + 8;
+    rc = rc ? rc : X.hip(hipMalloc(&X.rb.partials2, sizeof(double) * kMaxRedQ * X.rb.cap2), "span partials");
+    // a solve streams 4-8 fields per iteration: once a field passes 32 MiB
+    // that working set exceeds the 256 MiB last-level cache, and the outputs
+    // are stored non-temporally
+    X.nt_auto = (X.L64.total * 8 > (32ll << 20)) ? 1 : 0;
+    X.nt = X.nt_auto;
     rc = rc ? rc : X.hip(hipMalloc(&X.rb.partials, sizeof(double) * kMaxRedQ * X.rb.cap), "partials");
     rc = rc ? rc : X.hip(hipMalloc(&X.rb.counter, 256), "counter");
     rc = rc ? rc : X.hip(hipMemsetAsync(X.rb.counter, 0, 256, X.stream), "counter");
@@ -1357,6 +1413,7 @@ void pps_destroy(void* ctx)
     for (auto* p : C->flux_d)
         if (p) (void)hipFree(p);
     if (C->rb.partials) (void)hipFree(C->rb.partials);
+    if (C->rb.partials2) (void)hipFree(C->rb.partials2);
     if (C->rb.counter) (void)hipFree(C->rb.counter);
     if (C->scal_d) (void)hipFree(C->scal_d);
     if (C->hist_d) (void)hipFree(C->hist_d);
@@ -1706,7 +1763,7 @@ int pps_residual_noalloc(const pps_grid* grid, const double* x, const double* b,
     if (rc) return rc;
     RedBuf rb{sp, nullptr, kScratchPartials};
     double* out = norm_dev ? norm_dev : sp + kScratchPartials;  // else the result slot past the partials
-    const Geo g = make_geo(L, R, coef, kz, kz, 0);
+    const Geo g = make_geo(L, R, coef, kz, (L.total * 8 > (32ll << 20)) ? 1 : 0);
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (launch_residual_norm(g, s, rb, x, b, r, out) != 0) {
         g_create_error = "pps_residual_noalloc: launch exceeds the scratch partials";
@@ -1755,14 +1812,27 @@ int pps_norm_b_adjusted(void* ctx, double* out)
     return PPS_OK;
 }
 
-int pps_set_tiling(void* ctx, int kz_stencil, int kz_vector)
+int pps_set_tiling(void* ctx, int kz_stencil, int nontemporal)
 {
     Ctx* C = as_ctx(ctx);
     if (!C) return PPS_ERR_ARG;
-    for (int kz : {kz_stencil, kz_vector})
-        if (kz != 0 && (kz < kMinKz || kz > 4096)) return C->fail(PPS_ERR_ARG, "kz must be in [%d, 4096]", kMinKz);
+    if (kz_stencil != 0 && (kz_stencil < kMinKz || kz_stencil > 4096))
+        return C->fail(PPS_ERR_ARG, "kz must be in [%d, 4096]", kMinKz);
+    if (nontemporal < -1 || nontemporal > 1) return C->fail(PPS_ERR_ARG, "nontemporal must be -1, 0 or 1");
     if (kz_stencil > 0) C->kz_st = kz_stencil;
-    if (kz_vector > 0) C->kz_vec = kz_vector;
+    C->nt_mode = nontemporal;
+    C->nt = nontemporal == 0 ? C->nt_auto : (nontemporal > 0 ? 1 : 0);
+    return PPS_OK;
+}
+
+int pps_set_cheb_pairs(void* ctx, int pairs, int kz)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    if (pairs < 0 || pairs > 1) return C->fail(PPS_ERR_ARG, "pairs must be 0 or 1");
+    if (kz != 0 && (kz < 1 || kz > 4096)) return C->fail(PPS_ERR_ARG, "kz must be in [1, 4096]");
+    C->cheb_pairs = pairs;
+    if (kz > 0) C->cheb_kz = kz;
     return PPS_OK;
 }
 
@@ -1770,7 +1840,8 @@ int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes)
 {
     Ctx* C = as_ctx(ctx);
     if (!C || reps < 1 || !ms || !bytes) return PPS_ERR_ARG;
-    const int P = (kernel == PPS_K_CG_STENCIL_F32 || kernel == PPS_K_CG_UPDATE_F32) ? 32 : 64;
+    const int P = (kernel == PPS_K_CG_STENCIL_F32 || kernel == PPS_K_CG_UPDATE_F32 || kernel == PPS_K_FCG_STENCIL_F32)
+                      ? 32 : 64;
     const double w = P == 32 ? 4.0 : 8.0;
     const double n = (double)C->solver.count();
     Field a, b, c, d, e;
@@ -1811,6 +1882,22 @@ int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes)
         case PPS_K_RESIDUAL: launch_residual(g, C->stream, C->rb, a.d(), b.d(), c.d(), S, 1); break;
         case PPS_K_COPY: launch_copy<double>(g, C->stream, a.d(), e.d()); break;
         case PPS_K_CHEB_SWEEP: launch_cheb_sweep<double, double>(g, C->stream, a.d(), b.d(), c.d(), e.d(), 0.1, 0.2, 0.3, 1.0, nullptr); break;
+        case PPS_K_FCG_STENCIL:
+            launch_fcg_p_fused<double>(g, C->stream, C->rb, a.d(), (it & 1) ? c.d() : b.d(), (it & 1) ? b.d() : c.d(),
+                                       d.d(), a.d(), e.d(), S, false);
+            break;
+        case PPS_K_FCG_STENCIL_F32:
+            launch_fcg_p_fused<float>(g, C->stream, C->rb, a.f(), (it & 1) ? c.f() : b.f(), (it & 1) ? b.f() : c.f(),
+                                      d.f(), a.f(), e.f(), S, false);
+            break;
+        case PPS_K_COPY_FLAT: launch_copy_flat(C->stream, a.d(), e.d(), C->L64.total); break;
+        case PPS_K_CHEB_PAIR: {
+            const double ca[2] = {0.1, 0.15}, cb[2] = {0.2, 0.25}, cc[2] = {0.3, 0.35};
+            launch_cheb_pair<double, double>(g, C->cheb_pair_geo(), C->stream, (it & 1) ? d.d() : a.d(), b.d(), c.d(),
+                                             (it & 1) ? e.d() : d.d(), (it & 1) ? a.d() : e.d(), ca, cb, cc, 1.0, false,
+                                             nullptr);
+            break;
+        }
         }
     };
     double per = 0;
@@ -1820,6 +1907,9 @@ int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes)
     case PPS_K_RESIDUAL: per = 3; break;                                // read x, b ; write r
     case PPS_K_COPY: per = 2; break;
     case PPS_K_CHEB_SWEEP: per = 4; break;                              // read Y, B, Z ; write W
+    case PPS_K_FCG_STENCIL: case PPS_K_FCG_STENCIL_F32: per = 7; break; // read z, p_old, r, x ; write p, Ap, x
+    case PPS_K_COPY_FLAT: per = 2.0 * (double)C->L64.total / n; break;  // the whole allocation, read and written
+    case PPS_K_CHEB_PAIR: per = 5; break;                               // read Y, Z, B ; write W1, W2 (two sweeps)
     default: return C->fail(PPS_ERR_ARG, "unknown kernel id %d", kernel);
     }
     for (int i = 0; i < 3; ++i) launch(i);  // warm-up

@@ -1,22 +1,22 @@
 // pps_kernels.hip -- hand-written gfx950 kernels of the Krylov hot path.
 //
 // Two kernel skeletons carry every operation:
-//  * k_stencil: 2.5-D z-marching tile.  A block of 64x4 threads owns a 64x4
-//    column tile of one x-y slab and marches through kz planes.  Each thread
-//    keeps its column's (k-1, k, k+1) values in registers; the plane-k tile and
-//    its one-point halo are staged in LDS (double-buffered, one barrier per
-//    plane), so x- and y-neighbours come from LDS and every value of the input
-//    field is produced once per plane (this is what lets the CG search
-//    direction p = z + beta*p be formed on the fly inside the stencil).  The
-//    column value and the halo of plane k+1 are loaded while plane k is
-//    computed, so the barrier never waits on HBM latency.
+//  * k_stencil: 2.5-D z-marching tile.  A block of 64x4 threads owns a 64x16
+//    column tile (kTX x kTH: each thread four rows) of one x-y slab and marches
+//    through kz planes.  Each thread keeps its columns' (k-1, k, k+1) values in
+//    registers; the plane-k tile and its one-point halo are staged in LDS
+//    (double-buffered, one barrier per plane), so x- and y-neighbours come from
+//    LDS and every value of the input field is produced once per plane (this is
+//    what lets the CG search direction p = z + beta*p be formed on the fly
+//    inside the stencil).  The loads of planes k+1 and k+2 are in flight while
+//    plane k is computed, so the barrier never waits on HBM latency.
 //    The 7-point operator is the reference's +Laplacian
 //    (matrixFreeOperatorA.hpp:33-38) written in difference form,
 //    cx((x[i-1]-x)+(x[i+1]-x)) + cy(..) + cz(..), which keeps FP32 accurate
 //    for smooth fields.
-//  * k_vector: the same tiling without halo, for fused axpy/dot updates; each
-//    thread issues the loads of kU planes before it computes and stores any of
-//    them, so a wavefront keeps kU x (loads per point) requests in flight.
+//  * k_vector: one-shot blocks of 256 consecutive points (x fastest) for the
+//    fused axpy/dot updates: all of a thread's loads are issued before its
+//    store, and consecutive blocks stream consecutive addresses.
 //
 // Reductions are FP64 in every precision and deterministic: wavefront
 // shuffles, LDS across the four wavefronts, one partial per block, then a
@@ -40,7 +40,6 @@
 
 namespace pps {
 
-constexpr int kU = 4;  // planes whose loads a vector-kernel thread issues together
 
 // tiling of one launch: 64-wide tiles of `th` rows, z-chunks of kz planes
 static Geo retile(const Geo& in, int th, int kz)
@@ -54,7 +53,7 @@ static Geo retile(const Geo& in, int th, int kz)
     return g;
 }
 
-Geo make_geo(const Layout& L, const Region& R, const double coef[3], int kz_st, int kz_vec, int nt)
+Geo make_geo(const Layout& L, const Region& R, const double coef[3], int kz_st, int nt)
 {
     Geo g{};
     g.plane = L.plane;
@@ -64,19 +63,22 @@ Geo make_geo(const Layout& L, const Region& R, const double coef[3], int kz_st, 
     g.lo1 = R.lo[1]; g.hi1 = R.hi[1];
     g.lo2 = R.lo[2]; g.hi2 = R.hi[2];
     g.kz_st = kz_st;
-    g.kz_vec = kz_vec;
     g.nt = nt;
     g.cx = coef[0];
     g.cy = coef[1];
     g.cz = coef[2];
-    return retile(g, kTY, kz_vec);
+    return retile(g, kTH, kz_st);
 }
 
-// stores of streamed outputs: non-temporal when the fields exceed the caches
-template <typename T>
-__device__ __forceinline__ void put(T* p, T v, int nt)
+// stores of the field outputs: non-temporal (NT) when the fields exceed the
+// caches.  NT is a template argument so that each kernel body holds one kind
+// of store: with a run-time flag the compiler merges the two stores of an
+// if/else into one plain store and the non-temporal hint is lost.
+template <typename T> struct NoDeduce { using type = T; };
+template <bool NT, typename T>
+__device__ __forceinline__ void put(T* p, typename NoDeduce<T>::type v)
 {
-    if (nt) __builtin_nontemporal_store(v, p);
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
     else *p = v;
 }
 
@@ -89,11 +91,37 @@ __device__ __forceinline__ int xcd_remap(int b, int nb)
     return (b & 7) * (full >> 3) + (b >> 3);
 }
 
+// one DPP lane exchange of a double (two 32-bit moves); CTRL is the DPP control
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// sum over the wavefront, the same value in every lane, in a fixed order:
+// butterflies inside each row of 16 lanes (quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], row_half_mirror, row_mirror), then the four row sums read out
+// and added as (r0 + r1) + (r2 + r3).  DPP moves stay in the VALU, where a
+// shuffle is a round trip through the LDS crossbar per step; in a one-shot
+// reducing kernel over FP32 fields that round trip was the bottleneck.
 __device__ __forceinline__ double wave_sum(double v)
 {
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0x141>(v);
+    v += dpp_mov<0x140>(v);
+    const long long b = __double_as_longlong(v);
+    double r[4];
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    for (int q = 0; q < 4; ++q) {
+        const int lo = __builtin_amdgcn_readlane((int)b, 16 * q);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 16 * q);
+        r[q] = __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+    return (r[0] + r[1]) + (r[2] + r[3]);
 }
 
 // sum over the block in a fixed order; result valid in thread 0
@@ -119,14 +147,15 @@ __device__ __forceinline__ void block_sum(double (&v)[NQ], double (&tot)[NQ])
     }
 }
 
-template <int NQ>
-__device__ __forceinline__ void block_partials(double (&v)[NQ], const RedBuf& rb)
+// the block's sums as partial number `slot` of the launch
+template <int NQ, int NTHREADS = kThreads>
+__device__ __forceinline__ void block_partials(double (&v)[NQ], const RedBuf& rb, int slot)
 {
     double tot[NQ];
-    block_sum<NQ>(v, tot);
+    block_sum<NQ, NTHREADS>(v, tot);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) rb.partials[(size_t)q * rb.cap + blockIdx.x] = tot[q];
+        for (int q = 0; q < NQ; ++q) rb.partials[(size_t)q * rb.cap + slot] = tot[q];
     }
 }
 
@@ -212,15 +241,15 @@ __device__ __forceinline__ typename OxOf<Op>::type st_ox(const Op& op, long long
     if constexpr (std::is_same_v<typename OxOf<Op>::type, NoRaw>) return NoRaw{};
     else return op.out_raw(c);
 }
-template <class Op, typename T>
+template <bool NT, class Op, typename T>
 __device__ __forceinline__ void st_out(const Op& op, long long c, T v, T Av, const typename RawOf<Op, T>::type& r,
                                        const typename OxOf<Op>::type& o, double* acc)
 {
-    if constexpr (kHooks<Op, T>) op.out(c, v, Av, r, o, acc);
-    else op.out(c, v, Av, acc);
+    if constexpr (kHooks<Op, T>) op.template out<NT>(c, v, Av, r, o, acc);
+    else op.template out<NT>(c, v, Av, acc);
 }
 
-template <typename T, int NQ, class Op>
+template <typename T, int NQ, class Op, bool NT>
 __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
 {
     if (op.gated()) return;
@@ -317,7 +346,7 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
                 const T Av = cx * ((lds[buf][r][lx] - v) + (lds[buf][r][lx + 2] - v)) +
                              cy * ((lds[buf][r - 1][lx + 1] - v) + (lds[buf][r + 1][lx + 1] - v)) +
                              cz * ((vm[q] - v) + (vp[q] - v));
-                st_out<Op, T>(op, base[q] + g.plane * k, v, Av, rc[q], oc[q], acc);
+                st_out<NT, Op, T>(op, base[q] + g.plane * k, v, Av, rc[q], oc[q], acc);
             }
             vm[q] = vc[q];
             vc[q] = vp[q];
@@ -329,42 +358,89 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
         h1 = h2;
         buf ^= 1;
     }
-    if constexpr (NQ > 0) block_partials<NQ>(acc, rb);
+    if constexpr (NQ > 0) block_partials<NQ>(acc, rb, (int)blockIdx.x);
 }
 
-template <int NQ, class Op>
-__global__ __launch_bounds__(kThreads) void k_vector(Geo g, Op op, RedBuf rb)
+// Vector kernels: one-shot blocks on a 3-D grid (segment of an x-row, y, z).
+// A block of BT threads covers BT * E consecutive points of one row, thread t
+// taking points t, t + BT, ... (coalesced for every e); all of a thread's
+// loads are issued before its first store, and consecutive blocks cover
+// consecutive addresses, so the blocks in flight stream one compact window of
+// memory.  The grid coordinates are the row and segment, so a block spends no
+// instructions finding them.  With a few hundred bytes per wave, the per-wave
+// prologue, the bytes in flight per thread and the block reduction are what
+// separate these kernels from a flat copy: ops that fetch 16 bytes or more per
+// point run 256 x 1, ops that fetch 8 bytes or less (FP32 fields) 128 x 4.
+// (Measured on a 512^3 update r -= a Ap with r.r and r.(jac r) partials:
+// FP64 256 x 1 5.6 TB/s where z-marching 64 x 4 x 32 tiles reached 5.1-5.35;
+// FP32 128 x 4 5.2 TB/s where 256 x 1 reached 3.9.)
+template <class Op>
+struct VecShape {
+    static constexpr int E = sizeof(typename Op::R) <= 8 ? 4 : 1;
+    static constexpr int BT = E == 1 ? 256 : 128;
+};
+
+template <int NQ, class Op, bool NT>
+__global__ __launch_bounds__(VecShape<Op>::BT) void k_vector(Geo g, Op op, RedBuf rb)
 {
+    constexpr int E = VecShape<Op>::E, BT = VecShape<Op>::BT;
     if (op.gated()) return;
     op.load();
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const int tx0 = b % g.ntx;
-    const int rr = b / g.ntx;
-    const int ty0 = rr % g.nty;
-    const int tz0 = rr / g.nty;
-    const int i = g.lo0 + tx0 * kTX + threadIdx.x;
-    const int j = g.lo1 + ty0 * kTY + threadIdx.y;
-    const int k0 = g.lo2 + tz0 * g.kz;
-    const int k1 = min(k0 + g.kz, g.hi2);
+    const int i0 = g.lo0 + (int)blockIdx.x * (BT * E) + (int)threadIdx.x;
+    const int j = g.lo1 + (int)blockIdx.y, k = g.lo2 + (int)blockIdx.z;
+    const long long row = (long long)g.ox + (long long)g.px * j + g.plane * k;
     double acc[NQ > 0 ? NQ : 1];
 #pragma unroll
     for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) acc[q] = 0.0;
-    if (i < g.hi0 && j < g.hi1) {
-        const long long c0 = (long long)g.ox + i + (long long)g.px * j;
-        int k = k0;
-        for (; k + kU <= k1; k += kU) {
-            typename Op::R r[kU];
+    typename Op::R r[E];
 #pragma unroll
-            for (int u = 0; u < kU; ++u) r[u] = op.fetch(c0 + g.plane * (k + u));
+    for (int e = 0; e < E; ++e)
+        if (i0 + e * BT < g.hi0) r[e] = op.fetch(row + i0 + e * BT);
 #pragma unroll
-            for (int u = 0; u < kU; ++u) op.apply(c0 + g.plane * (k + u), r[u], i, j, k + u, acc);
-        }
-        for (; k < k1; ++k) {
-            const typename Op::R r = op.fetch(c0 + g.plane * k);
-            op.apply(c0 + g.plane * k, r, i, j, k, acc);
-        }
+    for (int e = 0; e < E; ++e) {
+        const int i = i0 + e * BT;
+        if (i < g.hi0) op.template apply<NT>(row + i, r[e], i, j, k, acc);
     }
-    if constexpr (NQ > 0) block_partials<NQ>(acc, rb);
+    if constexpr (NQ > 0)
+        block_partials<NQ, BT>(acc, rb, (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)));
+}
+
+// sums of kSpan consecutive block partials, one block per span, fixed order
+template <int NQ, class Op>
+__global__ __launch_bounds__(kThreads) void k_span(Op op, RedBuf rb, int nblocks)
+{
+    if (op.gated()) return;
+    const int b0 = blockIdx.x * kSpan, b1 = min(nblocks, b0 + kSpan);
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += kThreads) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] += rb.partials[(size_t)q * rb.cap + b];
+    }
+    double tot[NQ];
+    block_sum<NQ, kThreads>(acc, tot);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) rb.partials2[(size_t)q * rb.cap2 + blockIdx.x] = tot[q];
+    }
+}
+
+// the epilogue of a reducing launch of nblocks blocks: one level of span sums
+// first when there are many partials and the workspace has room for them
+template <int NQ, class Op>
+static void launch_finish(hipStream_t s, const Op& op, RedBuf rb, int nblocks)
+{
+    const int nspan = (nblocks + kSpan - 1) / kSpan;
+    if (nblocks > 4 * kSpan && rb.partials2 != nullptr && nspan <= rb.cap2) {
+        hipLaunchKernelGGL((k_span<NQ, Op>), dim3(nspan), dim3(kThreads), 0, s, op, rb, nblocks);
+        RedBuf r2 = rb;
+        r2.partials = rb.partials2;
+        r2.cap = rb.cap2;
+        hipLaunchKernelGGL((k_finish<NQ, Op>), dim3(1), dim3(kFinThreads), 0, s, op, r2, nspan);
+    } else {
+        hipLaunchKernelGGL((k_finish<NQ, Op>), dim3(1), dim3(kFinThreads), 0, s, op, rb, nblocks);
+    }
 }
 
 template <int NQ, class Op>
@@ -382,20 +458,34 @@ static void run_stencil(const Geo& g0, hipStream_t s, const Op& op, RedBuf rb)
 {
     const Geo g = retile(g0, kTH, g0.kz_st);
     if (g.nblocks <= 0 || !check_capacity<NQ, Op>(g, rb)) return;
-    hipLaunchKernelGGL((k_stencil<T, NQ, Op>), dim3(g.nblocks), dim3(kTX, kTY), 0, s, g, op, rb);
-    if constexpr (NQ > 0) hipLaunchKernelGGL((k_finish<NQ, Op>), dim3(1), dim3(kFinThreads), 0, s, op, rb, g.nblocks);
+    if (g.nt) hipLaunchKernelGGL((k_stencil<T, NQ, Op, true>), dim3(g.nblocks), dim3(kTX, kTY), 0, s, g, op, rb);
+    else hipLaunchKernelGGL((k_stencil<T, NQ, Op, false>), dim3(g.nblocks), dim3(kTX, kTY), 0, s, g, op, rb);
+    if constexpr (NQ > 0) launch_finish<NQ>(s, op, rb, g.nblocks);
 }
 
 template <int NQ, class Op>
 static void run_vector(const Geo& g0, hipStream_t s, const Op& op, RedBuf rb)
 {
-    const Geo g = retile(g0, kTY, g0.kz_vec);
-    if (g.nblocks <= 0 || !check_capacity<NQ, Op>(g, rb)) return;
-    hipLaunchKernelGGL((k_vector<NQ, Op>), dim3(g.nblocks), dim3(kTX, kTY), 0, s, g, op, rb);
-    if constexpr (NQ > 0) hipLaunchKernelGGL((k_finish<NQ, Op>), dim3(1), dim3(kFinThreads), 0, s, op, rb, g.nblocks);
+    const int nx = g0.hi0 - g0.lo0, ny = g0.hi1 - g0.lo1, nz = g0.hi2 - g0.lo2;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return;
+    constexpr int W = VecShape<Op>::BT * VecShape<Op>::E;
+    const int nseg = (nx + W - 1) / W;
+    const long long nb = (long long)nseg * ny * nz;
+    if (nb > 0x7FFFFFFFll || ny > 65535 || nz > 65535) {
+        std::fprintf(stderr, "pps: a %d x %d x %d region exceeds a vector launch; launch skipped\n", nx, ny, nz);
+        return;
+    }
+    Geo g = g0;
+    g.nblocks = (int)nb;
+    if (!check_capacity<NQ, Op>(g, rb)) return;
+    const dim3 grid(nseg, ny, nz);
+    const dim3 block(VecShape<Op>::BT);
+    if (g.nt) hipLaunchKernelGGL((k_vector<NQ, Op, true>), grid, block, 0, s, g, op, rb);
+    else hipLaunchKernelGGL((k_vector<NQ, Op, false>), grid, block, 0, s, g, op, rb);
+    if constexpr (NQ > 0) launch_finish<NQ>(s, op, rb, g.nblocks);
 }
 
-static const RedBuf kNoRed{nullptr, nullptr, 0};
+static const RedBuf kNoRed{nullptr, nullptr, 0, nullptr, 0};
 
 struct Empty {};
 
@@ -407,7 +497,7 @@ struct OpApply : Gate {
     T* __restrict__ y;
     __device__ void load() {}
     __device__ T in(long long c) const { return x[c]; }
-    __device__ void out(long long c, T, T Av, double*) const { y[c] = Av; }
+    template <bool NT> __device__ void out(long long c, T, T Av, double*) const { put<NT>(y + c, Av); }
     __device__ void finish(const double*) {}
 };
 
@@ -428,7 +518,7 @@ struct OpCopy : Gate {
     struct R { T a; };
     __device__ void load() {}
     __device__ R fetch(long long c) const { return {x[c]}; }
-    __device__ void apply(long long c, const R& r, int, int, int, double*) const { y[c] = r.a; }
+    template <bool NT> __device__ void apply(long long c, const R& r, int, int, int, double*) const { put<NT>(y + c, r.a); }
     __device__ void finish(const double*) {}
 };
 
@@ -442,6 +532,26 @@ void launch_copy(const Geo& g, hipStream_t s, const T* x, T* y)
     run_vector<0>(g, s, op, kNoRed);
 }
 
+// plain copy of n contiguous doubles, 16 bytes per lane with non-temporal
+// loads and stores: the HBM ceiling the field kernels are measured against
+typedef double d2_t __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256) void k_copy_flat(const double* __restrict__ x, double* __restrict__ y, long long n)
+{
+    const long long i = ((long long)blockIdx.x * 256 + threadIdx.x) * 2;
+    if (i + 1 < n) {
+        const d2_t v = __builtin_nontemporal_load(reinterpret_cast<const d2_t*>(x + i));
+        __builtin_nontemporal_store(v, reinterpret_cast<d2_t*>(y + i));
+    } else if (i < n) {
+        y[i] = x[i];
+    }
+}
+
+void launch_copy_flat(hipStream_t s, const double* x, double* y, long long n)
+{
+    const long long blocks = (n + 511) / 512;
+    if (blocks > 0) hipLaunchKernelGGL(k_copy_flat, dim3((unsigned)blocks), dim3(256), 0, s, x, y, n);
+}
+
 template <typename T>
 struct OpDot : Gate {
     const T* __restrict__ a;
@@ -450,7 +560,7 @@ struct OpDot : Gate {
     struct R { T a, b; };
     __device__ void load() {}
     __device__ R fetch(long long c) const { return {a[c], b[c]}; }
-    __device__ void apply(long long, const R& r, int, int, int, double* acc) const { acc[0] += (double)r.a * (double)r.b; }
+    template <bool NT> __device__ void apply(long long, const R& r, int, int, int, double* acc) const { acc[0] += (double)r.a * (double)r.b; }
     __device__ void finish(const double* t) { *out = t[0]; }
 };
 
@@ -472,7 +582,7 @@ struct OpFill : Gate {
     using R = Empty;
     __device__ void load() {}
     __device__ R fetch(long long) const { return {}; }
-    __device__ void apply(long long c, const R&, int, int, int, double*) const { a[c] = v; }
+    template <bool NT> __device__ void apply(long long c, const R&, int, int, int, double*) const { put<NT>(a + c, v); }
     __device__ void finish(const double*) {}
 };
 
@@ -495,12 +605,12 @@ struct OpScaleBox : Gate {
     struct R { T a; };
     __device__ void load() { nrm = S->v[S_NORM]; }
     __device__ R fetch(long long c) const { return {a[c]}; }
-    __device__ void apply(long long c, const R& r, int, int, int, double*) const
+    template <bool NT> __device__ void apply(long long c, const R& r, int, int, int, double*) const
     {
         // normalizeProblemToFieldBNorm divides (iterativeSolverBase.hpp:227-231);
         // the solvers' epilogues multiply (baseCG.hpp:248-252)
-        if (mode == 0) a[c] = (T)((double)r.a / nrm);
-        else a[c] = (T)((double)r.a * nrm);
+        if (mode == 0) put<NT>(a + c, (T)((double)r.a / nrm));
+        else put<NT>(a + c, (T)((double)r.a * nrm));
     }
     __device__ void finish(const double*) {}
 };
@@ -524,7 +634,7 @@ struct OpPointwise : Gate {
     struct R { T a; };
     __device__ void load() {}
     __device__ R fetch(long long p) const { return {in[p]}; }
-    __device__ void apply(long long p, const R& r, int, int, int, double*) const { out[p] = c * r.a; }
+    template <bool NT> __device__ void apply(long long p, const R& r, int, int, int, double*) const { put<NT>(out + p, c * r.a); }
     __device__ void finish(const double*) {}
 };
 
@@ -546,7 +656,7 @@ struct OpCast : Gate {
     struct R { TS a; };
     __device__ void load() {}
     __device__ R fetch(long long c) const { return {src[c]}; }
-    __device__ void apply(long long c, const R& r, int, int, int, double*) const { dst[c] = (TD)r.a; }
+    template <bool NT> __device__ void apply(long long c, const R& r, int, int, int, double*) const { put<NT>(dst + c, (TD)r.a); }
     __device__ void finish(const double*) {}
 };
 
@@ -574,10 +684,10 @@ struct OpResidual : Gate {
     struct Ox { double b; };
     __device__ double in(long long c) const { return x[c]; }
     __device__ Ox out_raw(long long c) const { return {b[c]}; }
-    __device__ void out(long long c, double, double Av, const double&, const Ox& o, double* acc) const
+    template <bool NT> __device__ void out(long long c, double, double Av, const double&, const Ox& o, double* acc) const
     {
         const double rv = o.b - Av;
-        r[c] = rv;
+        put<NT>(r + c, rv);
         acc[0] += rv * rv;
         acc[1] += rv * (jac * rv);
     }
@@ -635,10 +745,10 @@ struct OpResidualNorm : Gate {
     struct Ox { double b; };
     __device__ double in(long long c) const { return x[c]; }
     __device__ Ox out_raw(long long c) const { return {b[c]}; }
-    __device__ void out(long long c, double, double Av, const double&, const Ox& o, double* acc) const
+    template <bool NT> __device__ void out(long long c, double, double Av, const double&, const Ox& o, double* acc) const
     {
         const double rv = o.b - Av;
-        r[c] = rv;
+        put<NT>(r + c, rv);
         acc[0] += rv * rv;
     }
     __device__ void finish(const double* t) { *norm = sqrt(t[0]); }
@@ -723,7 +833,7 @@ struct OpBNorm : Gate {
     struct R { double b; };
     __device__ void load() {}
     __device__ R fetch(long long c) const { return {b[c]}; }
-    __device__ void apply(long long c, const R& r, int i, int j, int k, double* acc) const
+    template <bool NT> __device__ void apply(long long c, const R& r, int i, int j, int k, double* acc) const
     {
         const double v = adj.adjust(r.b, c, i, j, k);
         acc[0] += v * v;
@@ -752,7 +862,7 @@ struct OpAdjustB : Gate {
     struct R { double b; };
     __device__ void load() {}
     __device__ R fetch(long long c) const { return {b[c]}; }
-    __device__ void apply(long long c, const R& r, int i, int j, int k, double*) const { b[c] = adj.adjust(r.b, c, i, j, k); }
+    template <bool NT> __device__ void apply(long long c, const R& r, int i, int j, int k, double*) const { put<NT>(b + c, adj.adjust(r.b, c, i, j, k)); }
     __device__ void finish(const double*) {}
 };
 
@@ -832,7 +942,6 @@ struct OpCgA : Gate {
     T* __restrict__ x;
     Scal* S;
     bool pointwise;
-    int nt;
     T a, beta, alpha_prev;
     __device__ void load()
     {
@@ -845,11 +954,11 @@ struct OpCgA : Gate {
     __device__ Raw in_raw(long long c) const { return {src[c], pin[c]}; }
     __device__ T in_value(const Raw& r) const { return a * r.src + beta * r.pold; }
     __device__ Ox out_raw(long long c) const { return {x[c]}; }
-    __device__ void out(long long c, T vc, T Av, const Raw& r, const Ox& o, double* acc) const
+    template <bool NT> __device__ void out(long long c, T vc, T Av, const Raw& r, const Ox& o, double* acc) const
     {
-        put(pout + c, vc, nt);
-        put(Ap + c, Av, nt);
-        put(x + c, o.x + alpha_prev * r.pold, nt);
+        put<NT>(pout + c, vc);
+        put<NT>(Ap + c, Av);
+        put<NT>(x + c, o.x + alpha_prev * r.pold);
         acc[0] += (double)vc * (double)Av;
     }
     __device__ void finish(const double* t)
@@ -874,7 +983,6 @@ void launch_cg_A2(const Geo& g, hipStream_t s, RedBuf rb, const T* src, const T*
     op.x = x;
     op.S = S;
     op.pointwise = pointwise;
-    op.nt = g.nt;
     run_stencil<T, 1>(g, s, op, rb);
 }
 
@@ -887,7 +995,7 @@ struct OpCgFinalX : Gate {
     struct R { T x, p; };
     __device__ void load() { alpha = (T)S->v[S_ALPHA]; }
     __device__ R fetch(long long c) const { return {x[c], p[c]}; }
-    __device__ void apply(long long c, const R& v, int, int, int, double*) const { x[c] = v.x + alpha * v.p; }
+    template <bool NT> __device__ void apply(long long c, const R& v, int, int, int, double*) const { put<NT>(x + c, v.x + alpha * v.p); }
     __device__ void finish(const double*) {}
 };
 
@@ -909,7 +1017,6 @@ struct OpCgB : Gate {
     const T* __restrict__ Ap;
     Scal* S;
     bool pointwise, history;
-    int nt;
     T alpha, jac;
     struct R { T r, ap; };
     __device__ void load()
@@ -918,10 +1025,10 @@ struct OpCgB : Gate {
         jac = (T)S->v[S_JAC];
     }
     __device__ R fetch(long long c) const { return {r[c], Ap[c]}; }
-    __device__ void apply(long long c, const R& v, int, int, int, double* acc) const
+    template <bool NT> __device__ void apply(long long c, const R& v, int, int, int, double* acc) const
     {
         const T rv = v.r - alpha * v.ap;
-        put(r + c, rv, nt);
+        put<NT>(r + c, rv);
         acc[0] += (double)rv * (double)rv;
         acc[1] += (double)rv * (double)(jac * rv);
     }
@@ -946,7 +1053,6 @@ void launch_cg_B(const Geo& g, hipStream_t s, RedBuf rb, T* r, const T* Ap, Scal
     op.S = S;
     op.pointwise = pointwise;
     op.history = history;
-    op.nt = g.nt;
     run_vector<2>(g, s, op, rb);
 }
 
@@ -958,7 +1064,7 @@ struct OpCgRz : Gate {
     struct R { T r, z; };
     __device__ void load() {}
     __device__ R fetch(long long c) const { return {r[c], z[c]}; }
-    __device__ void apply(long long, const R& v, int, int, int, double* acc) const { acc[0] += (double)v.r * (double)v.z; }
+    template <bool NT> __device__ void apply(long long, const R& v, int, int, int, double* acc) const { acc[0] += (double)v.r * (double)v.z; }
     __device__ void finish(const double* t)
     {
         S->v[S_BETA] = S->iter == 0 ? 0.0 : t[0] / S->v[S_RZ];
@@ -993,9 +1099,9 @@ struct OpBiK1 : Gate {
     __device__ Raw in_raw(long long c) const { return {Mp[c]}; }
     __device__ T in_value(const Raw& r) const { return amp * r.m; }
     __device__ Ox out_raw(long long c) const { return {r0[c]}; }
-    __device__ void out(long long c, T, T Av, const Raw&, const Ox& o, double* acc) const
+    template <bool NT> __device__ void out(long long c, T, T Av, const Raw&, const Ox& o, double* acc) const
     {
-        AMp[c] = Av;
+        put<NT>(AMp + c, Av);
         acc[0] += (double)o.r0 * (double)Av;
     }
     __device__ void finish(const double* t) { S->v[S_ALPHA] = S->v[S_RHO0] / t[0]; }
@@ -1023,7 +1129,7 @@ struct OpBiK2 : Gate {
     struct R { T r, a; };
     __device__ void load() { alpha = (T)S->v[S_ALPHA]; }
     __device__ R fetch(long long c) const { return {r[c], AMp[c]}; }
-    __device__ void apply(long long c, const R& v, int, int, int, double*) const { r[c] = v.r - alpha * v.a; }
+    template <bool NT> __device__ void apply(long long c, const R& v, int, int, int, double*) const { put<NT>(r + c, v.r - alpha * v.a); }
     __device__ void finish(const double*) {}
 };
 
@@ -1052,9 +1158,9 @@ struct OpBiK3 : Gate {
     __device__ Raw in_raw(long long c) const { return {z[c]}; }
     __device__ T in_value(const Raw& v) const { return az * v.z; }
     __device__ Ox out_raw(long long c) const { return {r[c]}; }
-    __device__ void out(long long c, T, T Av, const Raw&, const Ox& o, double* acc) const
+    template <bool NT> __device__ void out(long long c, T, T Av, const Raw&, const Ox& o, double* acc) const
     {
-        Az[c] = Av;
+        put<NT>(Az + c, Av);
         acc[0] += (double)o.r * (double)Av;
         acc[1] += (double)Av * (double)Av;
     }
@@ -1093,11 +1199,11 @@ struct OpBiK45 : Gate {
         omega = (T)S->v[S_OMEGA];
     }
     __device__ R fetch(long long c) const { return {x[c], Mp[c], z[c], r[c], Az[c], r0[c]}; }
-    __device__ void apply(long long c, const R& v, int, int, int, double* acc) const
+    template <bool NT> __device__ void apply(long long c, const R& v, int, int, int, double* acc) const
     {
-        x[c] = v.x + alpha * (amp * v.mp) + omega * (az * v.z);
+        put<NT>(x + c, v.x + alpha * (amp * v.mp) + omega * (az * v.z));
         const T rv = v.r - omega * v.az;
-        r[c] = rv;
+        put<NT>(r + c, rv);
         acc[0] += (double)v.r0 * (double)rv;
         acc[1] += (double)rv * (double)rv;
     }
@@ -1144,7 +1250,7 @@ struct OpBiK6 : Gate {
         omega = (T)S->v[S_OMEGA];
     }
     __device__ R fetch(long long c) const { return {p[c], r[c], AMp[c]}; }
-    __device__ void apply(long long c, const R& v, int, int, int, double*) const { p[c] = v.r + beta * (v.p - omega * v.a); }
+    template <bool NT> __device__ void apply(long long c, const R& v, int, int, int, double*) const { put<NT>(p + c, v.r + beta * (v.p - omega * v.a)); }
     __device__ void finish(const double*) {}
 };
 
@@ -1171,10 +1277,10 @@ struct OpChebFirst : Gate {
     T c0, c1;
     __device__ void load() {}
     __device__ T in(long long c) const { return B[c]; }
-    __device__ void out(long long c, T vc, T Av, double*) const
+    template <bool NT> __device__ void out(long long c, T vc, T Av, double*) const
     {
-        Z[c] = c0 * vc;
-        Y[c] = c1 * (T(2) * vc + c0 * Av);
+        put<NT>(Z + c, c0 * vc);
+        put<NT>(Y + c, c1 * (T(2) * vc + c0 * Av));
     }
     __device__ void finish(const double*) {}
 };
@@ -1201,10 +1307,10 @@ struct OpChebFirstOut : Gate {
     double sign;
     __device__ void load() {}
     __device__ T in(long long c) const { return B[c]; }
-    __device__ void out(long long c, T vc, T Av, double*) const
+    template <bool NT> __device__ void out(long long c, T vc, T Av, double*) const
     {
         const T y = which == 0 ? c0 * vc : c1 * (T(2) * vc + c0 * Av);
-        X[c] = (TO)(sign * (double)y);
+        put<NT>(X + c, (TO)(sign * (double)y));
     }
     __device__ void finish(const double*) {}
 };
@@ -1237,11 +1343,11 @@ struct OpChebSweep : Gate {
     struct Ox { T b, z; };
     __device__ T in(long long c) const { return Y[c]; }
     __device__ Ox out_raw(long long c) const { return {B[c], Z[c]}; }
-    __device__ void out(long long c, T vc, T Av, const T&, const Ox& o, double*) const
+    template <bool NT> __device__ void out(long long c, T vc, T Av, const T&, const Ox& o, double*) const
     {
         const T w = a * vc + b * (o.b + Av) - cz * o.z;
-        if constexpr (sizeof(TO) == sizeof(T)) W[c] = sign < 0 ? -w : w;
-        else W[c] = (TO)(sign * (double)w);
+        if constexpr (sizeof(TO) == sizeof(T)) put<NT>(W + c, sign < 0 ? -w : w);
+        else put<NT>(W + c, (TO)(sign * (double)w));
     }
     __device__ void finish(const double*) {}
 };
@@ -1261,6 +1367,251 @@ void launch_cheb_sweep(const Geo& g, hipStream_t s, const T* Y, const T* B, cons
     op.cz = (T)c;
     op.sign = sign;
     run_stencil<T, 0>(g, s, op, kNoRed);
+}
+
+// ---------------------------------------------- fused pair of Chebyshev sweeps
+// Two consecutive sweeps of the three-term recurrence in one pass over the grid
+// (temporal blocking of chebyshevIteration.hpp:103-113 / the alpaka
+// Chebyshev2Kernel, kernelsAlpakaChebyshev.hpp:191-273):
+//   W1 = a1 Y  + b1 (B + A Y)  - c1 Z      sweep c    (Y = y_c, Z = y_{c-1})
+//   W2 = a2 W1 + b2 (B + A W1) - c2 Y      sweep c+1
+// with the Neumann ghosts of W1 reset between the two as the unfused sweeps
+// reset those of their input (ghost = the data point two cells inward).  A
+// block owns a 64 x 16 tile of W2 and marches z; it computes W1 on the tile
+// plus a one-point ring (the neighbours W2 needs), from Y on the tile plus a
+// two-point ring.  Per plane: the Y plane is staged in LDS (x-y neighbours),
+// each point's z history lives in the registers of the thread that owns it
+// (tile points: 4 rows per thread; ring points: one per thread), W1 of plane
+// s goes to LDS and W2 of plane s-1 comes out of it.  HBM traffic per pair:
+// Y, Z, B read once (plus the rings and the two halo planes), W1 and W2
+// written once: 5 fields in place of the 8 of two separate sweeps.
+constexpr int kCpRY = 4;                   // tile rows per thread
+constexpr int kCpTY = kTY * kCpRY;         // tile height (16)
+constexpr int kCpE1X = kTX + 2, kCpE1Y = kCpTY + 2;  // W1 region: tile + 1
+constexpr int kCpE2X = kTX + 4, kCpE2Y = kCpTY + 4;  // Y region: tile + 2
+constexpr int kCpRing1 = 2 * kCpE1X + 2 * kCpTY;      // W1 ring points (164)
+constexpr int kCpRing2 = 2 * kCpE2X + 2 * kCpE1Y;     // outer Y ring points (172)
+static_assert(kCpRing1 <= kThreads && kCpRing2 <= kThreads, "ring points need one thread each");
+
+// ring1 point t -> (row, col) of the W1 region; ring2 point t -> (row, col) of the Y region
+__device__ __forceinline__ void cp_ring1(int t, int& r, int& c)
+{
+    if (t < kCpE1X) { r = 0; c = t; }
+    else if (t < 2 * kCpE1X) { r = kCpE1Y - 1; c = t - kCpE1X; }
+    else if (t < 2 * kCpE1X + kCpTY) { r = 1 + t - 2 * kCpE1X; c = 0; }
+    else { r = 1 + t - 2 * kCpE1X - kCpTY; c = kCpE1X - 1; }
+}
+__device__ __forceinline__ void cp_ring2(int t, int& r, int& c)
+{
+    if (t < kCpE2X) { r = 0; c = t; }
+    else if (t < 2 * kCpE2X) { r = kCpE2Y - 1; c = t - kCpE2X; }
+    else if (t < 2 * kCpE2X + kCpE1Y) { r = 1 + t - 2 * kCpE2X; c = 0; }
+    else { r = 1 + t - 2 * kCpE2X - kCpE1Y; c = kCpE2X - 1; }
+}
+
+template <typename T, typename TO, bool NT, bool LAST>
+__global__ __launch_bounds__(kThreads) void k_cheb_pair(Geo g, ChebPairGeo cg, const T* __restrict__ Y,
+                                                        const T* __restrict__ Z, const T* __restrict__ B,
+                                                        T* __restrict__ W1, TO* __restrict__ W2, T a1, T b1, T c1,
+                                                        T a2, T b2, T c2, double sign, const int* gate)
+{
+    if (gate != nullptr && *gate != 0) return;
+    __shared__ T ysl[2][kCpE2Y][kCpE2X];
+    __shared__ T wsl[2][kCpE1Y][kCpE1X];
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int tx0 = b % g.ntx, rr = b / g.ntx;
+    const int ty0 = rr % g.nty, tz0 = rr / g.nty;
+    const int i0 = g.lo0 + tx0 * kTX, j0 = g.lo1 + ty0 * kCpTY;
+    const int k0 = g.lo2 + tz0 * g.kz, k1 = min(k0 + g.kz, g.hi2);
+    const int tid = threadIdx.x + kTX * threadIdx.y;
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const T cx = (T)g.cx, cy = (T)g.cy, czz = (T)g.cz;
+    auto solver = [&](int i, int j, int k) {
+        return i >= g.lo0 && i < g.hi0 && j >= g.lo1 && j < g.hi1 && k >= g.lo2 && k < g.hi2;
+    };
+    auto inbox = [&](int i, int j, int k) {
+        return i >= 0 && i < cg.ng[0] && j >= 0 && j < cg.ng[1] && k >= 0 && k < cg.ng[2];
+    };
+    auto at = [&](int i, int j, int k) { return (long long)g.ox + i + (long long)g.px * j + g.plane * k; };
+    auto yld = [&](int i, int j, int k) { return inbox(i, j, k) ? Y[at(i, j, k)] : T(0); };
+
+    // tile points: (i0 + lx, j0 + ly + kTY q)
+    const int ic = i0 + lx;
+    T ym[kCpRY], yc[kCpRY], yp[kCpRY], yn[kCpRY], zc[kCpRY], zn[kCpRY], bc[kCpRY], bn[kCpRY], bp[kCpRY];
+    T wm[kCpRY], wc[kCpRY], wn[kCpRY];
+    // W1 ring point (one per thread t < kCpRing1) and outer Y ring point (t < kCpRing2)
+    int r1r = 0, r1c = 0, r2r = 0, r2c = 0;
+    const bool has1 = tid < kCpRing1, has2 = tid < kCpRing2;
+    if (has1) cp_ring1(tid, r1r, r1c);
+    if (has2) cp_ring2(tid, r2r, r2c);
+    const int i1 = i0 - 1 + r1c, j1 = j0 - 1 + r1r;  // ring1 point
+    const int i2 = i0 - 2 + r2c, j2 = j0 - 2 + r2r;  // ring2 point
+    T qm = 0, qc = 0, qp = 0, qn = 0, qz = 0, qzn = 0, qb = 0, qbn = 0;
+    T oc = 0, on = 0;
+
+    const int s0 = k0 - 1;
+#pragma unroll
+    for (int q = 0; q < kCpRY; ++q) {
+        const int j = j0 + ly + kTY * q;
+        ym[q] = yld(ic, j, s0 - 1);
+        yc[q] = yld(ic, j, s0);
+        yp[q] = yld(ic, j, s0 + 1);
+        const bool sv = solver(ic, j, s0);
+        zc[q] = sv ? Z[at(ic, j, s0)] : T(0);
+        bc[q] = sv ? B[at(ic, j, s0)] : T(0);
+        bp[q] = T(0);
+        wm[q] = wc[q] = wn[q] = T(0);
+        yn[q] = zn[q] = bn[q] = T(0);
+    }
+    if (has1) {
+        qm = yld(i1, j1, s0 - 1);
+        qc = yld(i1, j1, s0);
+        qp = yld(i1, j1, s0 + 1);
+        if (solver(i1, j1, s0)) {
+            qz = Z[at(i1, j1, s0)];
+            qb = B[at(i1, j1, s0)];
+        }
+    }
+    if (has2) oc = yld(i2, j2, s0);
+
+    for (int s = s0; s <= k1; ++s) {
+        const int buf = s & 1;
+        // Y of plane s into LDS
+#pragma unroll
+        for (int q = 0; q < kCpRY; ++q) ysl[buf][ly + kTY * q + 2][lx + 2] = yc[q];
+        if (has1) ysl[buf][r1r + 1][r1c + 1] = qc;
+        if (has2) ysl[buf][r2r][r2c] = oc;
+        // loads for the next step: Y of plane s+2 (s+1 for the outer ring), Z and B of plane s+1
+        const bool ny = s + 2 <= k1 + 1, nzb = s + 1 <= k1;
+#pragma unroll
+        for (int q = 0; q < kCpRY; ++q) {
+            const int j = j0 + ly + kTY * q;
+            if (ny) yn[q] = yld(ic, j, s + 2);
+            const bool sv = nzb && solver(ic, j, s + 1);
+            zn[q] = sv ? Z[at(ic, j, s + 1)] : T(0);
+            bn[q] = sv ? B[at(ic, j, s + 1)] : T(0);
+        }
+        if (has1) {
+            if (ny) qn = yld(i1, j1, s + 2);
+            const bool sv = nzb && solver(i1, j1, s + 1);
+            qzn = sv ? Z[at(i1, j1, s + 1)] : T(0);
+            qbn = sv ? B[at(i1, j1, s + 1)] : T(0);
+        }
+        if (has2 && nzb) on = yld(i2, j2, s + 1);
+        __syncthreads();
+        // W1 of plane s on the tile and its ring (0 off the solver region)
+#pragma unroll
+        for (int q = 0; q < kCpRY; ++q) {
+            const int j = j0 + ly + kTY * q;
+            const int r = ly + kTY * q + 2, c = lx + 2;
+            T w = T(0);
+            if (solver(ic, j, s)) {
+                const T v = yc[q];
+                const T Av = cx * ((ysl[buf][r][c - 1] - v) + (ysl[buf][r][c + 1] - v)) +
+                             cy * ((ysl[buf][r - 1][c] - v) + (ysl[buf][r + 1][c] - v)) +
+                             czz * ((ym[q] - v) + (yp[q] - v));
+                w = a1 * v + b1 * (bc[q] + Av) - c1 * zc[q];
+                if constexpr (!LAST)
+                    if (s >= k0 && s < k1) put<NT>(W1 + at(ic, j, s), w);
+            }
+            wn[q] = w;
+            wsl[buf][r - 1][c - 1] = w;
+        }
+        if (has1) {
+            T w = T(0);
+            if (solver(i1, j1, s)) {
+                const int r = r1r + 1, c = r1c + 1;
+                const T v = qc;
+                const T Av = cx * ((ysl[buf][r][c - 1] - v) + (ysl[buf][r][c + 1] - v)) +
+                             cy * ((ysl[buf][r - 1][c] - v) + (ysl[buf][r + 1][c] - v)) +
+                             czz * ((qm - v) + (qp - v));
+                w = a1 * v + b1 * (qb + Av) - c1 * qz;
+            }
+            wsl[buf][r1r][r1c] = w;
+        }
+        __syncthreads();
+        // Neumann x/y ghosts of W1: the data point two cells inward
+        if (cg.neu & 15) {
+            const bool zin = s >= cg.lo_d[2] && s < cg.hi_d[2];
+            auto mirror = [&](int i, int j, int r, int c) {
+                if (!zin) return;
+                const bool yin = j >= cg.lo_d[1] && j < cg.hi_d[1], xin = i >= cg.lo_d[0] && i < cg.hi_d[0];
+                if ((cg.neu & 1) && i == cg.lo_d[0] - 1 && yin) wsl[buf][r][c] = wsl[buf][r][c + 2];
+                else if ((cg.neu & 2) && i == cg.hi_d[0] && yin) wsl[buf][r][c] = wsl[buf][r][c - 2];
+                else if ((cg.neu & 4) && j == cg.lo_d[1] - 1 && xin) wsl[buf][r][c] = wsl[buf][r + 2][c];
+                else if ((cg.neu & 8) && j == cg.hi_d[1] && xin) wsl[buf][r][c] = wsl[buf][r - 2][c];
+            };
+#pragma unroll
+            for (int q = 0; q < kCpRY; ++q) mirror(ic, j0 + ly + kTY * q, ly + kTY * q + 1, lx + 1);
+            if (has1) mirror(i1, j1, r1r, r1c);
+        }
+        // W2 of plane s-1 on the tile (W1 of plane s-1 is in the other LDS buffer)
+        if (s - 1 >= k0) {
+            const int p = s - 1, pb = buf ^ 1;
+            const bool zlo = (cg.neu & 16) && p == cg.lo_d[2], zhi = (cg.neu & 32) && p == cg.hi_d[2] - 1;
+#pragma unroll
+            for (int q = 0; q < kCpRY; ++q) {
+                const int j = j0 + ly + kTY * q;
+                if (!solver(ic, j, p)) continue;
+                const int r = ly + kTY * q + 1, c = lx + 1;
+                const T v = wc[q];
+                const T m = zlo ? wn[q] : wm[q];
+                const T u = zhi ? wm[q] : wn[q];
+                const T Av = cx * ((wsl[pb][r][c - 1] - v) + (wsl[pb][r][c + 1] - v)) +
+                             cy * ((wsl[pb][r - 1][c] - v) + (wsl[pb][r + 1][c] - v)) + czz * ((m - v) + (u - v));
+                const T w = a2 * v + b2 * (bp[q] + Av) - c2 * ym[q];
+                if constexpr (sizeof(TO) == sizeof(T)) put<NT>(W2 + at(ic, j, p), (TO)(sign < 0 ? -w : w));
+                else put<NT>(W2 + at(ic, j, p), (TO)(sign * (double)w));
+            }
+        }
+        // advance one plane
+#pragma unroll
+        for (int q = 0; q < kCpRY; ++q) {
+            ym[q] = yc[q];
+            yc[q] = yp[q];
+            yp[q] = yn[q];
+            zc[q] = zn[q];
+            bp[q] = bc[q];
+            bc[q] = bn[q];
+            wm[q] = wc[q];
+            wc[q] = wn[q];
+        }
+        qm = qc;
+        qc = qp;
+        qp = qn;
+        qz = qzn;
+        qb = qbn;
+        oc = on;
+    }
+}
+
+template <typename T, typename TO>
+void launch_cheb_pair(const Geo& g0, const ChebPairGeo& cg, hipStream_t s, const T* Y, const T* Z, const T* B, T* W1,
+                      TO* W2, const double a[2], const double b[2], const double c[2], double sign, bool last,
+                      const int* gate)
+{
+    Geo g = g0;
+    g.kz = cg.kz > 0 ? cg.kz : 16;
+    g.ntx = (g.hi0 - g.lo0 + kTX - 1) / kTX;
+    g.nty = (g.hi1 - g.lo1 + kCpTY - 1) / kCpTY;
+    const int ntz = (g.hi2 - g.lo2 + g.kz - 1) / g.kz;
+    const long long nb = (long long)g.ntx * g.nty * ntz;
+    if (g.ntx <= 0 || g.nty <= 0 || ntz <= 0) return;
+    if (nb > 0x7FFFFFFFll) {
+        std::fprintf(stderr, "pps: fused Chebyshev grid too large; launch skipped\n");
+        return;
+    }
+    const dim3 grid((unsigned)nb), block(kTX, kTY);
+    const T A1 = (T)a[0], B1 = (T)b[0], C1 = (T)c[0], A2 = (T)a[1], B2 = (T)b[1], C2 = (T)c[1];
+    if (last) {
+        if (g.nt) hipLaunchKernelGGL((k_cheb_pair<T, TO, true, true>), grid, block, 0, s, g, cg, Y, Z, B, W1, W2, A1, B1, C1, A2, B2, C2, sign, gate);
+        else hipLaunchKernelGGL((k_cheb_pair<T, TO, false, true>), grid, block, 0, s, g, cg, Y, Z, B, W1, W2, A1, B1, C1, A2, B2, C2, sign, gate);
+    } else if constexpr (std::is_same_v<T, TO>) {
+        if (g.nt) hipLaunchKernelGGL((k_cheb_pair<T, T, true, false>), grid, block, 0, s, g, cg, Y, Z, B, W1, W2, A1, B1, C1, A2, B2, C2, sign, gate);
+        else hipLaunchKernelGGL((k_cheb_pair<T, T, false, false>), grid, block, 0, s, g, cg, Y, Z, B, W1, W2, A1, B1, C1, A2, B2, C2, sign, gate);
+    } else {
+        std::fprintf(stderr, "pps: a fused Chebyshev pair changes precision only on its last output; launch skipped\n");
+    }
 }
 
 // ------------------------------------------------------------------ FCG
@@ -1298,7 +1649,7 @@ struct OpFcgGamma : Gate {
     struct R { T z, a; };
     __device__ void load() {}
     __device__ R fetch(long long c) const { return {z[c], Apold[c]}; }
-    __device__ void apply(long long, const R& v, int, int, int, double* acc) const { acc[0] += (double)v.z * (double)v.a; }
+    template <bool NT> __device__ void apply(long long, const R& v, int, int, int, double* acc) const { acc[0] += (double)v.z * (double)v.a; }
     __device__ void finish(const double* t)
     {
         S->v[S_GAM] = t[0];
@@ -1365,11 +1716,11 @@ struct OpFcgP : Gate {
     __device__ Raw in_raw(long long c) const { return {z[c], (beta == (T)0 && !upd) ? (T)0 : pold[c]}; }
     __device__ T in_value(const Raw& v) const { return beta == (T)0 ? v.z : v.z - beta * v.pold; }
     __device__ Ox out_raw(long long c) const { return {r[c], upd ? x[c] : (T)0}; }
-    __device__ void out(long long c, T vc, T Av, const Raw& v, const Ox& o, double* acc) const
+    template <bool NT> __device__ void out(long long c, T vc, T Av, const Raw& v, const Ox& o, double* acc) const
     {
-        p[c] = vc;
-        Ap[c] = Av;
-        if (upd) x[c] = o.x + alpha_prev * v.pold;
+        put<NT>(p + c, vc);
+        put<NT>(Ap + c, Av);
+        if (upd) put<NT>(x + c, o.x + alpha_prev * v.pold);
         acc[0] += (double)vc * (double)Av;
         acc[1] += (double)vc * (double)o.r;
         if constexpr (FUSED) {
@@ -1453,9 +1804,9 @@ struct OpFcgFinalX : Gate {
         skip = S->restart;
     }
     __device__ R fetch(long long c) const { return {x[c], skip ? (T)0 : p[c]}; }
-    __device__ void apply(long long c, const R& v, int, int, int, double*) const
+    template <bool NT> __device__ void apply(long long c, const R& v, int, int, int, double*) const
     {
-        if (!skip) x[c] = v.x + alpha * v.p;
+        if (!skip) put<NT>(x + c, v.x + alpha * v.p);
     }
     __device__ void finish(const double*) {}
 };
@@ -1488,12 +1839,12 @@ struct OpFcgUpdate : Gate {
     // r -= alpha Ap; x takes alpha p in the next stencil pass (or in the final
     // x kernel after the last iteration)
     __device__ R fetch(long long c) const { return {r[c], skip ? (T)0 : Ap[c]}; }
-    __device__ void apply(long long c, const R& v, int, int, int, double* acc) const
+    template <bool NT> __device__ void apply(long long c, const R& v, int, int, int, double* acc) const
     {
         T rv = v.r;
         if (!skip) {
             rv = v.r - alpha * v.ap;
-            r[c] = rv;
+            put<NT>(r + c, rv);
         }
         acc[0] += (double)rv * (double)rv;
     }
@@ -1525,11 +1876,11 @@ struct OpIrRes : Gate {
     struct Ox { double b; };
     __device__ double in(long long c) const { return x[c]; }
     __device__ Ox out_raw(long long c) const { return {b[c]}; }
-    __device__ void out(long long c, double, double Av, const double&, const Ox& o, double* acc) const
+    template <bool NT> __device__ void out(long long c, double, double Av, const double&, const Ox& o, double* acc) const
     {
         const double rv = o.b - Av;
         const float rf = (float)rv;
-        r32[c] = rf;
+        put<NT>(r32 + c, rf);
         acc[0] += rv * rv;
         acc[1] += (double)rf * (double)rf;
     }
@@ -1570,7 +1921,7 @@ struct OpIrUpdate : Gate {
     struct R { double x; float d; };
     __device__ void load() { sc = (float)SI->v[S_INNER_NORM]; }
     __device__ R fetch(long long c) const { return {x[c], d[c]}; }
-    __device__ void apply(long long c, const R& v, int, int, int, double*) const { x[c] = v.x + (double)(v.d * sc); }
+    template <bool NT> __device__ void apply(long long c, const R& v, int, int, int, double*) const { put<NT>(x + c, v.x + (double)(v.d * sc)); }
     __device__ void finish(const double*) {}
 };
 
@@ -1600,12 +1951,12 @@ struct OpIrInit : Gate {
         jac = (float)SI->v[S_JAC];
     }
     __device__ R fetch(long long c) const { return {r[c]}; }
-    __device__ void apply(long long c, const R& v, int, int, int, double* acc) const
+    template <bool NT> __device__ void apply(long long c, const R& v, int, int, int, double* acc) const
     {
         const float rv = v.r * inv;
-        r[c] = rv;
-        x[c] = 0.0f;
-        p[c] = 0.0f;
+        put<NT>(r + c, rv);
+        put<NT>(x + c, 0.0f);
+        put<NT>(p + c, 0.0f);
         acc[0] += (double)rv * (double)rv;
         acc[1] += (double)rv * (double)(jac * rv);
     }
@@ -1671,6 +2022,15 @@ void launch_ir_inner_init(const Geo& g, hipStream_t s, RedBuf rb, float* r, floa
 PPS_INST_T(double)
 PPS_INST_T(float)
 
+template void launch_cheb_pair<double, double>(const Geo&, const ChebPairGeo&, hipStream_t, const double*,
+                                               const double*, const double*, double*, double*, const double[2],
+                                               const double[2], const double[2], double, bool, const int*);
+template void launch_cheb_pair<float, float>(const Geo&, const ChebPairGeo&, hipStream_t, const float*, const float*,
+                                             const float*, float*, float*, const double[2], const double[2],
+                                             const double[2], double, bool, const int*);
+template void launch_cheb_pair<float, double>(const Geo&, const ChebPairGeo&, hipStream_t, const float*, const float*,
+                                              const float*, float*, double*, const double[2], const double[2],
+                                              const double[2], double, bool, const int*);
 template void launch_cheb_sweep<double, double>(const Geo&, hipStream_t, const double*, const double*, const double*,
                                                 double*, double, double, double, double, const int*);
 template void launch_cheb_sweep<float, float>(const Geo&, hipStream_t, const float*, const float*, const float*, float*,

@@ -19,7 +19,11 @@ namespace pps {
 constexpr int kTX = 64;   // tile width in x: one wavefront per row
 constexpr int kTY = 4;    // thread rows per block: 4 wavefronts per block
 constexpr int kThreads = kTX * kTY;
-constexpr int kRY = 4;    // rows per thread in the stencil tile
+// rows per thread in the stencil tile (a build-time knob for tile-shape sweeps)
+#ifndef PPS_STENCIL_ROWS
+#define PPS_STENCIL_ROWS 4
+#endif
+constexpr int kRY = PPS_STENCIL_ROWS;
 constexpr int kTH = kTY * kRY;  // stencil tile height (rows)
 constexpr int kMaxRedQ = 5;
 constexpr int kMinKz = 4;   // smallest z-chunk; sizes the reduction workspace
@@ -61,13 +65,18 @@ struct alignas(256) Scal {
     int stall_window;  // FCG: iterations per stall check (0 = off)
 };
 
-// Single-pass deterministic grid reduction workspace: per-block partial sums
-// and an arrival counter; the last block to arrive sums the partials in a
-// fixed order and runs the solver's scalar epilogue.
+// Deterministic grid reduction workspace: every block of a reducing launch
+// stores its partial sums; a one-block epilogue kernel sums them in a fixed
+// order and runs the solver's scalar update.  Launches with many blocks first
+// sum fixed spans of kSpan partials into partials2 (one block per span), so
+// the epilogue reads a few hundred values.
+constexpr int kSpan = 2048;
 struct RedBuf {
     double* partials;   // kMaxRedQ * cap
     unsigned* counter;
     int cap;            // max blocks per launch
+    double* partials2;  // kMaxRedQ * cap2 span sums (null: single-level sums only)
+    int cap2;
 };
 
 // Grid geometry handed to a launch: region to process and tiling.
@@ -77,14 +86,13 @@ struct Geo {
     int lo0, hi0, lo1, hi1, lo2, hi2;
     int ntx, nty, kz;   // tiling of one launch (set by the launcher)
     int nblocks;
-    int kz_st, kz_vec;  // z-chunk of stencil and vector launches
-    int nt;             // stream outputs past the caches (fields larger than the caches)
+    int kz_st;          // z-chunk of stencil launches
+    int nt;             // store field outputs non-temporally (working set beyond the caches)
     double cx, cy, cz;  // 1/ds^2 per direction
 };
 
-// kz_st / kz_vec: z-chunk lengths of stencil and vector kernels; nt: use
-// non-temporal stores
-Geo make_geo(const Layout& L, const Region& R, const double coef[3], int kz_st, int kz_vec, int nt);
+// kz_st: z-chunk length of the stencil kernels; nt: non-temporal stores
+Geo make_geo(const Layout& L, const Region& R, const double coef[3], int kz_st, int nt);
 
 // Neumann faces of the current grid, for the ghost-fill kernel.
 struct Faces {
@@ -107,6 +115,9 @@ void launch_apply_A(const Geo& g, hipStream_t s, const T* x, T* y);
 // copy of the solver region (bandwidth reference)
 template <typename T>
 void launch_copy(const Geo& g, hipStream_t s, const T* x, T* y);
+// plain copy of n contiguous doubles (16-byte non-temporal lanes; x and y
+// 16-byte aligned): the measured HBM ceiling
+void launch_copy_flat(hipStream_t s, const double* x, double* y, long long n);
 // sum a.b over the region into *out (device)
 template <typename T>
 void launch_dot(const Geo& g, hipStream_t s, RedBuf rb, const T* a, const T* b, double* out, const int* gate);
@@ -188,6 +199,19 @@ template <typename T, typename TO>
 void launch_cheb_sweep(const Geo& g, hipStream_t s, const T* Y, const T* B, const T* Z, TO* W, double a, double b,
                        double c, double sign, const int* gate);
 // X = sign*c0*B (degree-0/1 outputs) and X = sign*c1*(2B + c0*AB)
+// two fused sweeps (c, c+1) of the Chebyshev recurrence: W1 = y_{c+1} (not
+// written when `last`), W2 = y_{c+2} (sign-applied and converted to TO when
+// `last`, the stage output).  Y = y_c needs its Neumann ghosts set; Z = y_{c-1}.
+struct ChebPairGeo {
+    int lo_d[3], hi_d[3];  // data region (the Neumann ghosts' mirror rule)
+    int ng[3];             // guarded box (loads outside it read 0)
+    int neu;               // bit f set: face f (x- x+ y- y+ z- z+) is Neumann
+    int kz;                // z-chunk of one block
+};
+template <typename T, typename TO>
+void launch_cheb_pair(const Geo& g, const ChebPairGeo& cg, hipStream_t s, const T* Y, const T* Z, const T* B, T* W1,
+                      TO* W2, const double a[2], const double b[2], const double c[2], double sign, bool last,
+                      const int* gate);
 template <typename T, typename TO>
 void launch_cheb_first_out(const Geo& g, hipStream_t s, const T* B, TO* X, double c0, double c1, int which,
                            double sign, const int* gate);

@@ -285,5 +285,9 @@ def test_reference_main_runs_on_gpu():
     # 152), so the bar is the 5 % of the other BiCGSTAB runs of this size
     # (measured 101 against 104)
     assert abs(it - fx["iterations"]) <= round(0.05 * fx["iterations"])
-    assert abs(avg - fx["max_block_avg"]) <= 0.02 * fx["max_block_avg"]
-    assert abs(pt - fx["max_point"]) <= 0.02 * fx["max_point"]
+    # errors against the exact solution: at a 1e-8 residual the solver's own
+    # error is a few per cent of them and depends on the iterate the solve
+    # stops at (the recorded 64-rank run and its rerun here differ by 2.5 % and
+    # 0.8 %; the GPU's two summation orders gave 0.3 % / 0.1 % and 2.0 % / 2.6 %)
+    assert abs(avg - fx["max_block_avg"]) <= 0.05 * fx["max_block_avg"]
+    assert abs(pt - fx["max_point"]) <= 0.05 * fx["max_point"]
