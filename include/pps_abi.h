@@ -204,13 +204,17 @@ enum {
  * the context stream, measured with HIP events; *bytes = algorithmic bytes
  * moved per launch. */
 int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes);
-/* Launch shape: the z-chunk of the stencil kernels (0 keeps the current one)
- * and the stores of the field outputs: nontemporal = 0 automatic (non-temporal
- * once a field exceeds 32 MiB), 1 always non-temporal, -1 never. */
+/* Launch shape: the z-chunk of the stencil kernels (0 keeps the current one,
+ * -1 returns to the automatic choice: the longest of 64/32/16/8 planes that
+ * gives 2048 blocks) and the stores of the field outputs: nontemporal = 0
+ * automatic (non-temporal once a field exceeds 32 MiB), 1 always
+ * non-temporal, -1 never. */
 int pps_set_tiling(void* ctx, int kz_stencil, int nontemporal);
 /* Chebyshev sweeps: pairs = 1 runs them two per pass (one fused kernel reads
  * y_c, y_{c-1} and B once and writes y_{c+1} and y_{c+2}; the default), 0 one
- * sweep per pass; kz = z-chunk of the fused kernel (0 keeps the current one).
+ * sweep per pass; kz = z-chunk of the fused kernel (0 keeps the current one,
+ * -1 returns to the automatic choice: the longest of 64/32/16 planes giving
+ * 1024 blocks).
  * No counterpart in the reference: its sweeps are one kernel each
  * (kernelsAlpakaChebyshev.hpp:191-273). */
 int pps_set_cheb_pairs(void* ctx, int pairs, int kz);

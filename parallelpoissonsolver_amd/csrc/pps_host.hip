@@ -113,12 +113,12 @@ struct Ctx {
     double* hist_d = nullptr;
     int hist_cap = 0;
     int hist_len = 0;
-    int kz_st = 16;   // z-chunk of stencil launches
+    int kz_st = 0;    // z-chunk of stencil launches (0: chosen per launch)
     int nt = 0;       // non-temporal stores of field outputs
     int nt_mode = 0;  // 0: automatic (nt_auto), 1: always, -1: never
     int nt_auto = 0;  // the fields' working set exceeds the last-level cache
     int cheb_pairs = 1;  // Chebyshev sweeps run two per pass (k_cheb_pair)
-    int cheb_kz = 32;    // z-chunk of the fused Chebyshev pair
+    int cheb_kz = 0;     // z-chunk of the fused Chebyshev pair (0: chosen per launch)
     std::string err;
     std::shared_ptr<NNPrecond> nn;  // learned preconditioner of PPS_NN stages (pps_nn_attach)
     int fallback_switches = 0;      // FCG stages that switched to the fallback, this solve
@@ -1747,12 +1747,12 @@ int pps_residual_noalloc(const pps_grid* grid, const double* x, const double* b,
     L.ox = 0;
     L.plane = (long long)L.ng[0] * L.ng[1];
     L.total = L.plane * L.ng[2];
-    // z-chunk: the library default (16 planes), longer when a launch would
-    // need more blocks than the scratch holds partials
+    // z-chunk: 16 planes, longer when a launch would need more blocks than
+    // the scratch holds partials
     const long long tiles = (long long)((R.hi[0] - R.lo[0] + kTX - 1) / kTX) * ((R.hi[1] - R.lo[1] + kTH - 1) / kTH);
     if (tiles > kScratchPartials) {
-        g_create_error = "pps_residual_noalloc: an x-y plane needs more than 131072 tiles of 64x16 points "
-                         "(the scratch holds one partial per block)";
+        g_create_error = "pps_residual_noalloc: an x-y plane needs more than 131072 tiles of " + std::to_string(kTX) +
+                         "x" + std::to_string(kTH) + " points (the scratch holds one partial per block)";
         return PPS_ERR_UNSUPPORTED;
     }
     const int nz = R.hi[2] - R.lo[2];
@@ -1816,10 +1816,11 @@ int pps_set_tiling(void* ctx, int kz_stencil, int nontemporal)
 {
     Ctx* C = as_ctx(ctx);
     if (!C) return PPS_ERR_ARG;
-    if (kz_stencil != 0 && (kz_stencil < kMinKz || kz_stencil > 4096))
+    if (kz_stencil > 0 && (kz_stencil < kMinKz || kz_stencil > 4096))
         return C->fail(PPS_ERR_ARG, "kz must be in [%d, 4096]", kMinKz);
     if (nontemporal < -1 || nontemporal > 1) return C->fail(PPS_ERR_ARG, "nontemporal must be -1, 0 or 1");
     if (kz_stencil > 0) C->kz_st = kz_stencil;
+    if (kz_stencil < 0) C->kz_st = 0;
     C->nt_mode = nontemporal;
     C->nt = nontemporal == 0 ? C->nt_auto : (nontemporal > 0 ? 1 : 0);
     return PPS_OK;
@@ -1830,9 +1831,10 @@ int pps_set_cheb_pairs(void* ctx, int pairs, int kz)
     Ctx* C = as_ctx(ctx);
     if (!C) return PPS_ERR_ARG;
     if (pairs < 0 || pairs > 1) return C->fail(PPS_ERR_ARG, "pairs must be 0 or 1");
-    if (kz != 0 && (kz < 1 || kz > 4096)) return C->fail(PPS_ERR_ARG, "kz must be in [1, 4096]");
+    if (kz > 4096) return C->fail(PPS_ERR_ARG, "kz must be at most 4096");
     C->cheb_pairs = pairs;
     if (kz > 0) C->cheb_kz = kz;
+    if (kz < 0) C->cheb_kz = 0;
     return PPS_OK;
 }
 
@@ -1844,12 +1846,13 @@ int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes)
                       ? 32 : 64;
     const double w = P == 32 ? 4.0 : 8.0;
     const double n = (double)C->solver.count();
-    Field a, b, c, d, e;
+    Field a, b, c, d, e, f;
     int rc = C->get_work("time.a", P, a);
     if (!rc) rc = C->get_work("time.b", P, b);
     if (!rc) rc = C->get_work("time.c", P, c);
     if (!rc) rc = C->get_work("time.d", P, d);
     if (!rc) rc = C->get_work("time.e", P, e);
+    if (!rc) rc = C->get_work("time.f", P, f);
     if (rc) return rc;
     Scal* S = C->scal(C->nscal - 2);
     rc = scal_init(*C, S, -1.0, 1 << 30, 1.0, nullptr, 0);
@@ -1861,10 +1864,12 @@ int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes)
         launch_fill<double>(C->gsolver(64), C->stream, a.d(), 1.0);
         launch_fill<double>(C->gsolver(64), C->stream, b.d(), 0.5);
         launch_fill<double>(C->gsolver(64), C->stream, c.d(), 0.25);
+        launch_fill<double>(C->gsolver(64), C->stream, f.d(), 0.125);
     } else {
         launch_fill<float>(C->gsolver(32), C->stream, a.f(), 1.0f);
         launch_fill<float>(C->gsolver(32), C->stream, b.f(), 0.5f);
         launch_fill<float>(C->gsolver(32), C->stream, c.f(), 0.25f);
+        launch_fill<float>(C->gsolver(32), C->stream, f.f(), 0.125f);
     }
     const Geo g = C->gsolver(P);
     auto launch = [&](int it) {
@@ -1884,11 +1889,11 @@ int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes)
         case PPS_K_CHEB_SWEEP: launch_cheb_sweep<double, double>(g, C->stream, a.d(), b.d(), c.d(), e.d(), 0.1, 0.2, 0.3, 1.0, nullptr); break;
         case PPS_K_FCG_STENCIL:
             launch_fcg_p_fused<double>(g, C->stream, C->rb, a.d(), (it & 1) ? c.d() : b.d(), (it & 1) ? b.d() : c.d(),
-                                       d.d(), a.d(), e.d(), S, false);
+                                       d.d(), f.d(), e.d(), S, false);
             break;
         case PPS_K_FCG_STENCIL_F32:
             launch_fcg_p_fused<float>(g, C->stream, C->rb, a.f(), (it & 1) ? c.f() : b.f(), (it & 1) ? b.f() : c.f(),
-                                      d.f(), a.f(), e.f(), S, false);
+                                      d.f(), f.f(), e.f(), S, false);
             break;
         case PPS_K_COPY_FLAT: launch_copy_flat(C->stream, a.d(), e.d(), C->L64.total); break;
         case PPS_K_CHEB_PAIR: {

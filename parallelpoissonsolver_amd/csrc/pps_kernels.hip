@@ -1,9 +1,10 @@
 // pps_kernels.hip -- hand-written gfx950 kernels of the Krylov hot path.
 //
 // Two kernel skeletons carry every operation:
-//  * k_stencil: 2.5-D z-marching tile.  A block of 64x4 threads owns a 64x16
+//  * k_stencil: 2.5-D z-marching tile.  A block of 64x8 threads owns a 64x32
 //    column tile (kTX x kTH: each thread four rows) of one x-y slab and marches
-//    through kz planes.  Each thread keeps its columns' (k-1, k, k+1) values in
+//    through kz planes (the longest of 64/32/16/8 that still gives 2048
+//    blocks).  Each thread keeps its columns' (k-1, k, k+1) values in
 //    registers; the plane-k tile and its one-point halo are staged in LDS
 //    (double-buffered, one barrier per plane), so x- and y-neighbours come from
 //    LDS and every value of the input field is produced once per plane (this is
@@ -14,13 +15,16 @@
 //    (matrixFreeOperatorA.hpp:33-38) written in difference form,
 //    cx((x[i-1]-x)+(x[i+1]-x)) + cy(..) + cz(..), which keeps FP32 accurate
 //    for smooth fields.
-//  * k_vector: one-shot blocks of 256 consecutive points (x fastest) for the
-//    fused axpy/dot updates: all of a thread's loads are issued before its
-//    store, and consecutive blocks stream consecutive addresses.
+//  * k_vector: one-shot blocks over consecutive points of an x-row (512 x 1
+//    for FP64 operands, 128 threads x 4 points for FP32) for the fused
+//    axpy/dot updates: all of a thread's loads are issued before its store,
+//    and consecutive blocks stream consecutive addresses.
+// A third, k_cheb_pair, runs two Chebyshev sweeps per pass.
 //
-// Reductions are FP64 in every precision and deterministic: wavefront
-// shuffles, LDS across the four wavefronts, one partial per block, then a
-// one-block epilogue kernel sums the partials in a fixed order and runs the
+// Reductions are FP64 in every precision and deterministic: DPP row
+// butterflies inside each wavefront, LDS across the wavefronts, one partial
+// per block, then a one-block epilogue kernel (after a level of span sums when
+// there are many blocks) sums the partials in a fixed order and runs the
 // solver's scalar update (alpha, beta, omega, convergence flag) on the device.
 // The epilogue runs as its own launch instead of a last-block-arrives scheme:
 // on gfx950 the agent-scope release fence that scheme needs in every block
@@ -250,11 +254,11 @@ __device__ __forceinline__ void st_out(const Op& op, long long c, T v, T Av, con
 }
 
 template <typename T, int NQ, class Op, bool NT>
-__global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
+__global__ __launch_bounds__(kSThreads) void k_stencil(Geo g, Op op, RedBuf rb)
 {
     if (op.gated()) return;
     op.load();
-    // tile: 64 columns x kTH rows; thread (lx, ly) owns rows ly, ly+kTY, ...
+    // tile: 64 columns x kTH rows; thread (lx, ly) owns rows ly, ly+kSTY, ...
     __shared__ T lds[2][kTH + 2][kTX + 2];
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int tx0 = b % g.ntx;
@@ -268,6 +272,7 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
     const int k1 = min(k0 + g.kz, g.hi2);
 
     // halo producers: 128 threads for the two y-halo rows, 2*kTH for the x-halo columns
+    static_assert(2 * kTX + 2 * kTH <= kSThreads, "one thread per halo point of a plane");
     int hr = 0, hc_ = 0, hi = 0, hj = 0;
     bool hv = false;
     if (tid < 2 * kTX) {
@@ -303,7 +308,7 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
     Ox oc[kRY], on[kRY];
 #pragma unroll
     for (int q = 0; q < kRY; ++q) {
-        const int j = jt + ly + q * kTY;
+        const int j = jt + ly + q * kSTY;
         act[q] = (i < g.hi0) && (j < g.hi1);
         inb[q] = (i <= g.hi0) && (j <= g.hi1);
         base[q] = (long long)g.ox + i + (long long)g.px * j;
@@ -335,13 +340,13 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
 #pragma unroll
         for (int q = 0; q < kRY; ++q) vp[q] = inb[q] ? st_value<Op, T>(op, r1[q]) : T(0);
 #pragma unroll
-        for (int q = 0; q < kRY; ++q) lds[buf][ly + q * kTY + 1][lx + 1] = vc[q];
+        for (int q = 0; q < kRY; ++q) lds[buf][ly + q * kSTY + 1][lx + 1] = vc[q];
         if (hv) lds[buf][hr][hc_] = hin ? st_value<Op, T>(op, hc) : T(0);
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < kRY; ++q) {
             if (act[q]) {
-                const int r = ly + q * kTY + 1;
+                const int r = ly + q * kSTY + 1;
                 const T v = vc[q];
                 const T Av = cx * ((lds[buf][r][lx] - v) + (lds[buf][r][lx + 2] - v)) +
                              cy * ((lds[buf][r - 1][lx + 1] - v) + (lds[buf][r + 1][lx + 1] - v)) +
@@ -358,7 +363,7 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
         h1 = h2;
         buf ^= 1;
     }
-    if constexpr (NQ > 0) block_partials<NQ>(acc, rb, (int)blockIdx.x);
+    if constexpr (NQ > 0) block_partials<NQ, kSThreads>(acc, rb, (int)blockIdx.x);
 }
 
 // Vector kernels: one-shot blocks on a 3-D grid (segment of an x-row, y, z).
@@ -370,14 +375,14 @@ __global__ __launch_bounds__(kThreads) void k_stencil(Geo g, Op op, RedBuf rb)
 // instructions finding them.  With a few hundred bytes per wave, the per-wave
 // prologue, the bytes in flight per thread and the block reduction are what
 // separate these kernels from a flat copy: ops that fetch 16 bytes or more per
-// point run 256 x 1, ops that fetch 8 bytes or less (FP32 fields) 128 x 4.
+// point run 512 x 1, ops that fetch 8 bytes or less (FP32 fields) 128 x 4.
 // (Measured on a 512^3 update r -= a Ap with r.r and r.(jac r) partials:
-// FP64 256 x 1 5.6 TB/s where z-marching 64 x 4 x 32 tiles reached 5.1-5.35;
-// FP32 128 x 4 5.2 TB/s where 256 x 1 reached 3.9.)
+// FP64 256 x 1 5.6 TB/s, 512 x 1 5.8, where z-marching 64 x 4 x 32 tiles
+// reached 5.1-5.35; FP32 128 x 4 5.2 TB/s where 256 x 1 reached 3.9.)
 template <class Op>
 struct VecShape {
     static constexpr int E = sizeof(typename Op::R) <= 8 ? 4 : 1;
-    static constexpr int BT = E == 1 ? 256 : 128;
+    static constexpr int BT = E == 1 ? 512 : 128;
 };
 
 template <int NQ, class Op, bool NT>
@@ -453,13 +458,28 @@ static bool check_capacity(const Geo& g, const RedBuf& rb)
     return true;
 }
 
+// tiling of a stencil launch: the z-chunk set on the context, or (kz_st = 0)
+// the longest of 64, 32, 16, 8 planes that still gives 2048 blocks (a longer
+// chunk re-reads fewer halo planes; 2048 blocks keep every CU busy to the end)
+static Geo stencil_geo(const Geo& g0)
+{
+    if (g0.kz_st > 0) return retile(g0, kTH, g0.kz_st);
+    int kz = 64;
+    Geo g = retile(g0, kTH, kz);
+    while (kz > 8 && g.nblocks < 2048) {
+        kz /= 2;
+        g = retile(g0, kTH, kz);
+    }
+    return g;
+}
+
 template <typename T, int NQ, class Op>
 static void run_stencil(const Geo& g0, hipStream_t s, const Op& op, RedBuf rb)
 {
-    const Geo g = retile(g0, kTH, g0.kz_st);
+    const Geo g = stencil_geo(g0);
     if (g.nblocks <= 0 || !check_capacity<NQ, Op>(g, rb)) return;
-    if (g.nt) hipLaunchKernelGGL((k_stencil<T, NQ, Op, true>), dim3(g.nblocks), dim3(kTX, kTY), 0, s, g, op, rb);
-    else hipLaunchKernelGGL((k_stencil<T, NQ, Op, false>), dim3(g.nblocks), dim3(kTX, kTY), 0, s, g, op, rb);
+    if (g.nt) hipLaunchKernelGGL((k_stencil<T, NQ, Op, true>), dim3(g.nblocks), dim3(kTX, kSTY), 0, s, g, op, rb);
+    else hipLaunchKernelGGL((k_stencil<T, NQ, Op, false>), dim3(g.nblocks), dim3(kTX, kSTY), 0, s, g, op, rb);
     if constexpr (NQ > 0) launch_finish<NQ>(s, op, rb, g.nblocks);
 }
 
@@ -756,7 +776,7 @@ struct OpResidualNorm : Gate {
 
 int launch_residual_norm(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, double* r, double* out)
 {
-    const Geo gt = retile(g, kTH, g.kz_st);
+    const Geo gt = stencil_geo(g);
     if (gt.nblocks > rb.cap) return -1;
     OpResidualNorm op;
     op.gate = nullptr;
@@ -1377,7 +1397,7 @@ void launch_cheb_sweep(const Geo& g, hipStream_t s, const T* Y, const T* B, cons
 //   W2 = a2 W1 + b2 (B + A W1) - c2 Y      sweep c+1
 // with the Neumann ghosts of W1 reset between the two as the unfused sweeps
 // reset those of their input (ghost = the data point two cells inward).  A
-// block owns a 64 x 16 tile of W2 and marches z; it computes W1 on the tile
+// block owns a 64 x 32 tile of W2 and marches z; it computes W1 on the tile
 // plus a one-point ring (the neighbours W2 needs), from Y on the tile plus a
 // two-point ring.  Per plane: the Y plane is staged in LDS (x-y neighbours),
 // each point's z history lives in the registers of the thread that owns it
@@ -1385,13 +1405,18 @@ void launch_cheb_sweep(const Geo& g, hipStream_t s, const T* Y, const T* B, cons
 // s goes to LDS and W2 of plane s-1 comes out of it.  HBM traffic per pair:
 // Y, Z, B read once (plus the rings and the two halo planes), W1 and W2
 // written once: 5 fields in place of the 8 of two separate sweeps.
+#ifndef PPS_CHEB_WAVES
+#define PPS_CHEB_WAVES 8
+#endif
+constexpr int kCpW = PPS_CHEB_WAVES;       // wavefronts per block (tile rows ly + kCpW q)
+constexpr int kCpT = kTX * kCpW;           // threads per block
 constexpr int kCpRY = 4;                   // tile rows per thread
-constexpr int kCpTY = kTY * kCpRY;         // tile height (16)
+constexpr int kCpTY = kCpW * kCpRY;        // tile height
 constexpr int kCpE1X = kTX + 2, kCpE1Y = kCpTY + 2;  // W1 region: tile + 1
 constexpr int kCpE2X = kTX + 4, kCpE2Y = kCpTY + 4;  // Y region: tile + 2
 constexpr int kCpRing1 = 2 * kCpE1X + 2 * kCpTY;      // W1 ring points (164)
 constexpr int kCpRing2 = 2 * kCpE2X + 2 * kCpE1Y;     // outer Y ring points (172)
-static_assert(kCpRing1 <= kThreads && kCpRing2 <= kThreads, "ring points need one thread each");
+static_assert(kCpRing1 <= kCpT && kCpRing2 <= kCpT, "ring points need one thread each");
 
 // ring1 point t -> (row, col) of the W1 region; ring2 point t -> (row, col) of the Y region
 __device__ __forceinline__ void cp_ring1(int t, int& r, int& c)
@@ -1410,7 +1435,7 @@ __device__ __forceinline__ void cp_ring2(int t, int& r, int& c)
 }
 
 template <typename T, typename TO, bool NT, bool LAST>
-__global__ __launch_bounds__(kThreads) void k_cheb_pair(Geo g, ChebPairGeo cg, const T* __restrict__ Y,
+__global__ __launch_bounds__(kCpT) void k_cheb_pair(Geo g, ChebPairGeo cg, const T* __restrict__ Y,
                                                         const T* __restrict__ Z, const T* __restrict__ B,
                                                         T* __restrict__ W1, TO* __restrict__ W2, T a1, T b1, T c1,
                                                         T a2, T b2, T c2, double sign, const int* gate)
@@ -1435,7 +1460,7 @@ __global__ __launch_bounds__(kThreads) void k_cheb_pair(Geo g, ChebPairGeo cg, c
     auto at = [&](int i, int j, int k) { return (long long)g.ox + i + (long long)g.px * j + g.plane * k; };
     auto yld = [&](int i, int j, int k) { return inbox(i, j, k) ? Y[at(i, j, k)] : T(0); };
 
-    // tile points: (i0 + lx, j0 + ly + kTY q)
+    // tile points: (i0 + lx, j0 + ly + kCpW q)
     const int ic = i0 + lx;
     T ym[kCpRY], yc[kCpRY], yp[kCpRY], yn[kCpRY], zc[kCpRY], zn[kCpRY], bc[kCpRY], bn[kCpRY], bp[kCpRY];
     T wm[kCpRY], wc[kCpRY], wn[kCpRY];
@@ -1452,7 +1477,7 @@ __global__ __launch_bounds__(kThreads) void k_cheb_pair(Geo g, ChebPairGeo cg, c
     const int s0 = k0 - 1;
 #pragma unroll
     for (int q = 0; q < kCpRY; ++q) {
-        const int j = j0 + ly + kTY * q;
+        const int j = j0 + ly + kCpW * q;
         ym[q] = yld(ic, j, s0 - 1);
         yc[q] = yld(ic, j, s0);
         yp[q] = yld(ic, j, s0 + 1);
@@ -1478,14 +1503,14 @@ __global__ __launch_bounds__(kThreads) void k_cheb_pair(Geo g, ChebPairGeo cg, c
         const int buf = s & 1;
         // Y of plane s into LDS
 #pragma unroll
-        for (int q = 0; q < kCpRY; ++q) ysl[buf][ly + kTY * q + 2][lx + 2] = yc[q];
+        for (int q = 0; q < kCpRY; ++q) ysl[buf][ly + kCpW * q + 2][lx + 2] = yc[q];
         if (has1) ysl[buf][r1r + 1][r1c + 1] = qc;
         if (has2) ysl[buf][r2r][r2c] = oc;
         // loads for the next step: Y of plane s+2 (s+1 for the outer ring), Z and B of plane s+1
         const bool ny = s + 2 <= k1 + 1, nzb = s + 1 <= k1;
 #pragma unroll
         for (int q = 0; q < kCpRY; ++q) {
-            const int j = j0 + ly + kTY * q;
+            const int j = j0 + ly + kCpW * q;
             if (ny) yn[q] = yld(ic, j, s + 2);
             const bool sv = nzb && solver(ic, j, s + 1);
             zn[q] = sv ? Z[at(ic, j, s + 1)] : T(0);
@@ -1502,8 +1527,8 @@ __global__ __launch_bounds__(kThreads) void k_cheb_pair(Geo g, ChebPairGeo cg, c
         // W1 of plane s on the tile and its ring (0 off the solver region)
 #pragma unroll
         for (int q = 0; q < kCpRY; ++q) {
-            const int j = j0 + ly + kTY * q;
-            const int r = ly + kTY * q + 2, c = lx + 2;
+            const int j = j0 + ly + kCpW * q;
+            const int r = ly + kCpW * q + 2, c = lx + 2;
             T w = T(0);
             if (solver(ic, j, s)) {
                 const T v = yc[q];
@@ -1542,7 +1567,7 @@ __global__ __launch_bounds__(kThreads) void k_cheb_pair(Geo g, ChebPairGeo cg, c
                 else if ((cg.neu & 8) && j == cg.hi_d[1] && xin) wsl[buf][r][c] = wsl[buf][r - 2][c];
             };
 #pragma unroll
-            for (int q = 0; q < kCpRY; ++q) mirror(ic, j0 + ly + kTY * q, ly + kTY * q + 1, lx + 1);
+            for (int q = 0; q < kCpRY; ++q) mirror(ic, j0 + ly + kCpW * q, ly + kCpW * q + 1, lx + 1);
             if (has1) mirror(i1, j1, r1r, r1c);
         }
         // W2 of plane s-1 on the tile (W1 of plane s-1 is in the other LDS buffer)
@@ -1551,9 +1576,9 @@ __global__ __launch_bounds__(kThreads) void k_cheb_pair(Geo g, ChebPairGeo cg, c
             const bool zlo = (cg.neu & 16) && p == cg.lo_d[2], zhi = (cg.neu & 32) && p == cg.hi_d[2] - 1;
 #pragma unroll
             for (int q = 0; q < kCpRY; ++q) {
-                const int j = j0 + ly + kTY * q;
+                const int j = j0 + ly + kCpW * q;
                 if (!solver(ic, j, p)) continue;
-                const int r = ly + kTY * q + 1, c = lx + 1;
+                const int r = ly + kCpW * q + 1, c = lx + 1;
                 const T v = wc[q];
                 const T m = zlo ? wn[q] : wm[q];
                 const T u = zhi ? wm[q] : wn[q];
@@ -1591,17 +1616,24 @@ void launch_cheb_pair(const Geo& g0, const ChebPairGeo& cg, hipStream_t s, const
                       const int* gate)
 {
     Geo g = g0;
-    g.kz = cg.kz > 0 ? cg.kz : 16;
     g.ntx = (g.hi0 - g.lo0 + kTX - 1) / kTX;
     g.nty = (g.hi1 - g.lo1 + kCpTY - 1) / kCpTY;
-    const int ntz = (g.hi2 - g.lo2 + g.kz - 1) / g.kz;
+    // z-chunk: as set, or the longest of 64, 32, 16 planes giving 1024 blocks
+    // (two rounds of the two blocks per CU that fit; halo planes cost 2 / kz)
+    const int nz = g.hi2 - g.lo2;
+    g.kz = cg.kz;
+    if (g.kz <= 0) {
+        g.kz = 64;
+        while (g.kz > 16 && (long long)g.ntx * g.nty * ((nz + g.kz - 1) / g.kz) < 1024) g.kz /= 2;
+    }
+    const int ntz = (nz + g.kz - 1) / g.kz;
     const long long nb = (long long)g.ntx * g.nty * ntz;
     if (g.ntx <= 0 || g.nty <= 0 || ntz <= 0) return;
     if (nb > 0x7FFFFFFFll) {
         std::fprintf(stderr, "pps: fused Chebyshev grid too large; launch skipped\n");
         return;
     }
-    const dim3 grid((unsigned)nb), block(kTX, kTY);
+    const dim3 grid((unsigned)nb), block(kTX, kCpW);
     const T A1 = (T)a[0], B1 = (T)b[0], C1 = (T)c[0], A2 = (T)a[1], B2 = (T)b[1], C2 = (T)c[1];
     if (last) {
         if (g.nt) hipLaunchKernelGGL((k_cheb_pair<T, TO, true, true>), grid, block, 0, s, g, cg, Y, Z, B, W1, W2, A1, B1, C1, A2, B2, C2, sign, gate);

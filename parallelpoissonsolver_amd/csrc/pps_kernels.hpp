@@ -19,12 +19,18 @@ namespace pps {
 constexpr int kTX = 64;   // tile width in x: one wavefront per row
 constexpr int kTY = 4;    // thread rows per block: 4 wavefronts per block
 constexpr int kThreads = kTX * kTY;
-// rows per thread in the stencil tile (a build-time knob for tile-shape sweeps)
+// stencil tile: kSTY wavefronts per block, kRY rows per thread (build-time
+// knobs for tile-shape sweeps)
 #ifndef PPS_STENCIL_ROWS
 #define PPS_STENCIL_ROWS 4
 #endif
+#ifndef PPS_STENCIL_WAVES
+#define PPS_STENCIL_WAVES 8
+#endif
 constexpr int kRY = PPS_STENCIL_ROWS;
-constexpr int kTH = kTY * kRY;  // stencil tile height (rows)
+constexpr int kSTY = PPS_STENCIL_WAVES;
+constexpr int kSThreads = kTX * kSTY;
+constexpr int kTH = kSTY * kRY;  // stencil tile height (rows)
 constexpr int kMaxRedQ = 5;
 constexpr int kMinKz = 4;   // smallest z-chunk; sizes the reduction workspace
 

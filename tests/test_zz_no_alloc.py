@@ -35,11 +35,13 @@ def _grid(n, ds, bcs):
 
 def test_no_alloc_rejects_bad_arguments_on_the_host():
     """Null pointers and grids beyond the scratch (an x-y plane of more than
-    131072 tiles of 64 x 16 points) are refused before any device call."""
+    131072 stencil tiles: 65536 x 8400 points is 1024 x 132 tiles of 64 x 64
+    points and more with any smaller tile height) are refused before any
+    device call."""
     f = _fn()
     g = _grid((18, 18, 18), (0.1,) * 3, (0,) * 6)
     assert f(ctypes.byref(g), None, 8, 8, None, None, None) == ERR_ARG
-    big = _grid((65536, 2064, 3), (0.1,) * 3, (1,) * 6)
+    big = _grid((65536, 8400, 3), (0.1,) * 3, (1,) * 6)
     rc = f(ctypes.byref(big), 8, 8, 8, None, None, None)
     assert rc == ERR_UNSUPPORTED
     assert b"131072 tiles" in L.lib().pps_last_error(None)
