@@ -35,6 +35,7 @@
 // the y-halo rows a tile reads from its neighbour are then L2 hits.
 #include "pps_kernels.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 #include <hip/hip_runtime.h>
@@ -385,18 +386,13 @@ struct VecShape {
     static constexpr int BT = E == 1 ? 512 : 128;
 };
 
-template <int NQ, class Op, bool NT>
-__global__ __launch_bounds__(VecShape<Op>::BT) void k_vector(Geo g, Op op, RedBuf rb)
+// segment `seg` of row (j, k): the block's BT * E points
+template <bool NT, class Op>
+__device__ __forceinline__ void vector_segment(const Geo& g, const Op& op, int seg, int j, int k, double* acc)
 {
     constexpr int E = VecShape<Op>::E, BT = VecShape<Op>::BT;
-    if (op.gated()) return;
-    op.load();
-    const int i0 = g.lo0 + (int)blockIdx.x * (BT * E) + (int)threadIdx.x;
-    const int j = g.lo1 + (int)blockIdx.y, k = g.lo2 + (int)blockIdx.z;
+    const int i0 = g.lo0 + seg * (BT * E) + (int)threadIdx.x;
     const long long row = (long long)g.ox + (long long)g.px * j + g.plane * k;
-    double acc[NQ > 0 ? NQ : 1];
-#pragma unroll
-    for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) acc[q] = 0.0;
     typename Op::R r[E];
 #pragma unroll
     for (int e = 0; e < E; ++e)
@@ -406,8 +402,41 @@ __global__ __launch_bounds__(VecShape<Op>::BT) void k_vector(Geo g, Op op, RedBu
         const int i = i0 + e * BT;
         if (i < g.hi0) op.template apply<NT>(row + i, r[e], i, j, k, acc);
     }
+}
+
+template <int NQ, class Op, bool NT>
+__global__ __launch_bounds__(VecShape<Op>::BT) void k_vector(Geo g, Op op, RedBuf rb)
+{
+    if (op.gated()) return;
+    op.load();
+    double acc[NQ > 0 ? NQ : 1];
+#pragma unroll
+    for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) acc[q] = 0.0;
+    vector_segment<NT>(g, op, (int)blockIdx.x, g.lo1 + (int)blockIdx.y, g.lo2 + (int)blockIdx.z, acc);
     if constexpr (NQ > 0)
-        block_partials<NQ, BT>(acc, rb, (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)));
+        block_partials<NQ, VecShape<Op>::BT>(acc, rb,
+                                             (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)));
+}
+
+// The same pass on a fixed grid of at most kStridedBlocks blocks, each taking
+// segments blockIdx.x, blockIdx.x + gridDim.x, ...: for passes that are
+// usually gated off (a gated one-shot launch still dispatches one block per
+// segment, ~0.1 ms at 512^3 for nothing), at some cost when they do run.
+constexpr int kStridedBlocks = 2048;
+template <int NQ, class Op, bool NT>
+__global__ __launch_bounds__(VecShape<Op>::BT) void k_vector_strided(Geo g, Op op, RedBuf rb, int nseg, int ny,
+                                                                     long long nb)
+{
+    if (op.gated()) return;
+    op.load();
+    double acc[NQ > 0 ? NQ : 1];
+#pragma unroll
+    for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) acc[q] = 0.0;
+    for (long long b = blockIdx.x; b < nb; b += gridDim.x) {
+        const long long rw = b / nseg;
+        vector_segment<NT>(g, op, (int)(b - rw * nseg), g.lo1 + (int)(rw % ny), g.lo2 + (int)(rw / ny), acc);
+    }
+    if constexpr (NQ > 0) block_partials<NQ, VecShape<Op>::BT>(acc, rb, (int)blockIdx.x);
 }
 
 // sums of kSpan consecutive block partials, one block per span, fixed order
@@ -484,7 +513,7 @@ static void run_stencil(const Geo& g0, hipStream_t s, const Op& op, RedBuf rb)
 }
 
 template <int NQ, class Op>
-static void run_vector(const Geo& g0, hipStream_t s, const Op& op, RedBuf rb)
+static void run_vector(const Geo& g0, hipStream_t s, const Op& op, RedBuf rb, bool strided = false)
 {
     const int nx = g0.hi0 - g0.lo0, ny = g0.hi1 - g0.lo1, nz = g0.hi2 - g0.lo2;
     if (nx <= 0 || ny <= 0 || nz <= 0) return;
@@ -496,6 +525,15 @@ static void run_vector(const Geo& g0, hipStream_t s, const Op& op, RedBuf rb)
         return;
     }
     Geo g = g0;
+    if (strided) {
+        g.nblocks = (int)std::min<long long>(nb, kStridedBlocks);
+        if (!check_capacity<NQ, Op>(g, rb)) return;
+        const dim3 grid(g.nblocks), block(VecShape<Op>::BT);
+        if (g.nt) hipLaunchKernelGGL((k_vector_strided<NQ, Op, true>), grid, block, 0, s, g, op, rb, nseg, ny, nb);
+        else hipLaunchKernelGGL((k_vector_strided<NQ, Op, false>), grid, block, 0, s, g, op, rb, nseg, ny, nb);
+        if constexpr (NQ > 0) launch_finish<NQ>(s, op, rb, g.nblocks);
+        return;
+    }
     g.nblocks = (int)nb;
     if (!check_capacity<NQ, Op>(g, rb)) return;
     const dim3 grid(nseg, ny, nz);
@@ -1699,7 +1737,7 @@ void launch_fcg_gamma(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const 
     op.Apold = Apold;
     op.S = S;
     if (fallback_only) op.run_if = &S->fallback;
-    run_vector<1>(g, s, op, rb);
+    run_vector<1>(g, s, op, rb, fallback_only);
 }
 
 template <typename T>
