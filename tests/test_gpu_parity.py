@@ -381,6 +381,35 @@ def test_ir_first_step_tolerance_matches_oracle():
 ALP_CHEB = dict(cheb_eps=0.0, cheb_rmin=100.0, cheb_rmax=1 - 1e-4)  # solverSetup.hpp (alpaka):59-62
 
 
+@pytest.mark.parametrize("n", [34, 50])
+@pytest.mark.parametrize("m", [7, 24])
+def test_ir_fp32_cg_with_fp32_chebyshev_matches_oracle(n, m):
+    """Refinement with an FP32 inner CG preconditioned by FP32 Chebyshev(m)
+    (the fused pairs in FP32 throughout, the last pair writing an FP32 field)
+    against the oracle's refinement with its FP32 Chebyshev (og_cheb_f), on the
+    reference test problem with Dirichlet faces (with the reference's Neumann
+    CG, whose p ghosts are never refreshed, the inner CG does not converge in
+    either): same outer step count, inner counts within 5 % (at least 2: FP32
+    sweeps in another summation order), both at the FP64 true residual 1e-10."""
+    bcs = (0,) * 6
+    G = O.OracleGrid((n,) * 3, (1 / (n - 1),) * 3, (3, 2.5, 10), bcs)
+    x, b, flux = G.set_problem(0)
+    C = ctx_for(G, bcs)
+    upload_problem(C, G, x, b, flux, bcs)
+    res = C.solve([dict(kind=L.IR, tol=1e-10, maxit=50), dict(kind=L.CG, precision=32, tol=1e-5, maxit=5000),
+                   dict(kind=L.CHEBYSHEV, precision=32, maxit=m)])
+    C.close()
+    xo, bo = x.copy(), b.copy()
+    ro = G.solve([dict(kind=O.IR, tol=1e-10, maxit=50), dict(kind=O.CG, tol=1e-5, maxit=5000),
+                  dict(kind=O.CHEB, maxit=m)], xo, bo, flux)
+    print(f"[ir-cheb32] n {n} m {m}: GPU {res['iterations']} inner ({res['outer_iterations']} outer), "
+          f"oracle {ro['iterations']} ({ro['outer_iterations']}); true residual "
+          f"{res['error_compute_operator']:.2e} / {ro['error_compute_operator']:.2e}", flush=True)
+    assert res["outer_iterations"] == ro["outer_iterations"]
+    assert abs(res["iterations"] - ro["iterations"]) <= max(2, round(0.05 * ro["iterations"]))
+    assert res["error_compute_operator"] < 1e-10 and ro["error_compute_operator"] < 1e-10
+
+
 @pytest.mark.parametrize("prec", [64, 32])
 @pytest.mark.parametrize("bcs", [(0, 0, 0, 0, 0, 0), (0, 1, 1, 0, 1, 0)])
 @pytest.mark.parametrize("m", [4, 11, 24])
