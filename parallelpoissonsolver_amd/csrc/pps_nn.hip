@@ -937,8 +937,10 @@ constexpr int kFG = kXH / 4;                      // x-rows (groups) per wavefro
 static_assert(kXW == 16 && kXH % 4 == 0, "one 16-voxel x-row per group, equal rows per wavefront");
 
 // (FP64 input: three blocks per CU, the raw FP64 prefetch does not fit the
-// register budget of four)
-template <bool F64>
+// register budget of four).  O64: the output field (and the dot operand) is
+// FP64; a template argument, so the plane loop holds one store and one
+// conversion.
+template <bool F64, bool O64>
 __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
 {
     if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
@@ -1075,8 +1077,8 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     const bool mine = tid < kFW * kFH && x < W && y < H;
     const int q0 = oy * kXW + ox + 1;  // X region voxel of the output's (x, y - 1) neighbour
     float am1 = 0.0f, a0 = 0.0f, ap1 = 0.0f;
-    const bool o64 = a.fout.is64;
-    const int oesz = o64 ? 8 : 4;
+    constexpr bool o64 = O64;
+    constexpr int oesz = o64 ? 8 : 4;
     const unsigned ooff = mine ? (unsigned)((x * a.fout.sx + y * a.fout.sy) * oesz) : kOOB;
     const char* obase = (const char*)a.fout.p + a.fout.off * oesz;
     const char* dbase = a.dot ? (const char*)a.dot + a.fout.off * oesz : nullptr;
@@ -1111,7 +1113,8 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
         store_plane(buf, s + 1);
         {
             const double d64 = __builtin_bit_cast(double, dbuf), d32 = (double)__uint_as_float(dbuf[0]);
-            dcur = o64 ? d64 : d32;  // the operand of output plane s-2
+            if constexpr (O64) dcur = d64;  // the operand of output plane s-2
+            else dcur = d32;
         }
         fs.fetch(buf, s + 3, D);  // zeros beyond the domain; unused beyond the chunk
         {
@@ -1141,7 +1144,7 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
         if (zo >= z0 && zo < z1) {  // uniform
             const float v = am1 + bout;
             const __amdgpu_buffer_rsrc_t r = buf_rsrc(obase + zo * oszb, kInRange);
-            if (o64) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (double)v), r, ooff, 0, 0);
+            if constexpr (O64) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (double)v), r, ooff, 0, 0);
             else __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, ooff, 0, 0);
             // threads without an output hold a dot operand of 0 but may hold a
             // non-finite v (gathered from the padding)
@@ -1189,8 +1192,13 @@ void launch_outf(const ConvArgs& a, hipStream_t s)
 {
     int zc;
     const dim3 grid = outf_grid(a.out.D, a.out.H, a.out.W, &zc);
-    if (a.fin.is64) hipLaunchKernelGGL(k_outf<true>, grid, dim3(256), 0, s, a, zc);
-    else hipLaunchKernelGGL(k_outf<false>, grid, dim3(256), 0, s, a, zc);
+    if (a.fin.is64) {
+        if (a.fout.is64) hipLaunchKernelGGL((k_outf<true, true>), grid, dim3(256), 0, s, a, zc);
+        else hipLaunchKernelGGL((k_outf<true, false>), grid, dim3(256), 0, s, a, zc);
+    } else {
+        if (a.fout.is64) hipLaunchKernelGGL((k_outf<false, true>), grid, dim3(256), 0, s, a, zc);
+        else hipLaunchKernelGGL((k_outf<false, false>), grid, dim3(256), 0, s, a, zc);
+    }
 }
 
 // ------------------------------------------------------------ weight gradient
