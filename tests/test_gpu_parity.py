@@ -474,6 +474,31 @@ def test_cheb_pairs_match_single_sweeps(prec, bcs, m):
     assert diff <= (1e-14 if prec == 64 else 1e-6), diff
 
 
+@pytest.mark.parametrize("n", [(4, 4, 4), (5, 7, 6), (3, 66, 9)])
+@pytest.mark.parametrize("bcs", [(1, 1, 1, 1, 1, 1), (0, 1, 1, 0, 1, 0)])
+def test_cheb_pairs_on_tiny_grids(n, bcs):
+    """Fused pairs against single sweeps where one tile holds both Neumann
+    ghosts of a direction and the mirror points sit next to them: 2 to 5
+    interior points per direction (plus one direction of two tiles), FP64,
+    m = 24.  Bar: 1e-14 relative max-norm."""
+    ds = tuple(1.0 / (v - 1) for v in n)
+    G = O.OracleGrid(n, ds, (0, 0, 0), bcs)
+    inp = np.random.default_rng(13).standard_normal(G.shape)
+    chain = [dict(kind=L.CHEBYSHEV, maxit=24, **ALP_CHEB)]
+    out = {}
+    for pairs in (1, 0):
+        C = ctx_for(G, bcs)
+        C.set_cheb_pairs(pairs)
+        C.upload(L.FIELD_T0, inp)
+        C.fill(L.FIELD_T1, 0.0)
+        assert C.precondition(chain, L.FIELD_T1, L.FIELD_T0) == 24
+        out[pairs] = C.download(L.FIELD_T1)
+        C.close()
+    diff = relmax(out[1], out[0])
+    print(f"[cheb-pairs-tiny] n {n} bcs {bcs}: max rel diff {diff:.3e}", flush=True)
+    assert diff <= 1e-14, diff
+
+
 @pytest.mark.parametrize("n,bcs", [(34, (0, 1, 1, 0, 1, 0)), (66, (0, 0, 0, 0, 0, 0))])
 def test_bicgstab_fp32_chebyshev_solve_matches_oracle(n, bcs):
     """The reference's default GPU composition with T_data_chebyshev = float:
