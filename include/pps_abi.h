@@ -168,7 +168,7 @@ int pps_residual(void* ctx, int r, int x, int b, double* norm);
  * `stream`).  stream: a hipStream_t, NULL for the null stream.  The reduction
  * partials live in a static device array of the library (1 MiB per GPU, kept
  * until the process ends): an x-y plane may need at most 131072 tiles of
- * 64 x 16 points, larger grids are refused with PPS_ERR_UNSUPPORTED.  Calls
+ * 64 x 32 points, larger grids are refused with PPS_ERR_UNSUPPORTED.  Calls
  * that overlap on one GPU share that array and race.  Errors: pps_last_error(NULL). */
 int pps_residual_noalloc(const pps_grid* grid, const double* x, const double* b, double* r, double* norm_dev,
                          double* norm_host, void* stream);

@@ -17,7 +17,7 @@
 namespace pps {
 
 constexpr int kTX = 64;   // tile width in x: one wavefront per row
-constexpr int kTY = 4;    // thread rows per block: 4 wavefronts per block
+constexpr int kTY = 4;    // wavefronts of the 256-thread blocks (span sums, the reductions' default)
 constexpr int kThreads = kTX * kTY;
 // stencil tile: kSTY wavefronts per block, kRY rows per thread (build-time
 // knobs for tile-shape sweeps)
