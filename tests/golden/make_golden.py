@@ -93,6 +93,11 @@ LARGE = [
     ("cg_none_normal_258_r8", (2, 2, 2), unit(258) + BC_DIR + ["--problem", "2", "--rhs", "normal", "--seed", "2024", "--solver", "cg", "--precond", "none"], False),
     ("bicgstab_none_normal_258_r8", (2, 2, 2), unit(258) + BC_DIR + ["--problem", "2", "--rhs", "normal", "--seed", "2024", "--solver", "bicgstab", "--precond", "none"], False),
     ("cg_none_gauss_130_r8", (2, 2, 2), unit(130) + BC_DIR + ["--problem", "2", "--rhs", "gauss", "--seed", "3", "--solver", "cg", "--precond", "none"], False),
+    # the alpaka default composition on its test problem at 130^3, one rank (the
+    # preconditioner runs without halo exchange, communicationOFF, so a
+    # multi-rank run would precondition block by block)
+    ("bicgstab_cheb24_alp_130", (1, 1, 1), grid(130, 0.1) + ["--origin", "3", "2.5", "10"] + BC_ALP
+     + ["--solver", "bicgstab", "--precond", "cheb", "--cheb-m", "24"] + ALP_CHEB, False),
 ]
 
 
@@ -145,25 +150,33 @@ def run_case(name, ranks, args, full, tmp):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--large", action="store_true", help="also run the 130^3/258^3 multi-rank cases")
+    ap.add_argument("--only", nargs="*", default=None, help="run only these cases and merge them into the fixtures")
     a = ap.parse_args()
     if not os.path.exists(REF):
         sys.exit(f"{REF} missing: run `make -C oracle ref` first")
     cases, vecs = [], {}
     with tempfile.TemporaryDirectory() as tmp:
         todo = SMALL + MEDIUM + (LARGE if a.large else [])
+        if a.only:
+            todo = [t for t in SMALL + MEDIUM + LARGE if t[0] in a.only]
         for name, ranks, args, full in todo:
             c, v = run_case(name, ranks, args, full, tmp)
             cases.append(c)
             if v is not None:
                 vecs[name] = v
     out_json = os.path.join(HERE, "ref_cases.json")
-    if not a.large and os.path.exists(out_json):
-        # keep previously generated large cases
+    if (a.only or not a.large) and os.path.exists(out_json):
+        # keep the previously generated cases not rerun
         old = json.load(open(out_json))
         have = {c["name"] for c in cases}
         cases += [c for c in old["cases"] if c["name"] not in have]
     json.dump({"generator": "tests/golden/make_golden.py", "reference": "solverPoissonMPI_CPU (compiled by oracle/Makefile)",
                "cases": cases}, open(out_json, "w"), indent=1)
+    if a.only:
+        # keep the stored vectors of the cases not rerun
+        old_vecs = dict(np.load(os.path.join(HERE, "ref_vectors.npz")))
+        old_vecs.update(vecs)
+        vecs = old_vecs
     np.savez_compressed(os.path.join(HERE, "ref_vectors.npz"), **vecs)
 
 

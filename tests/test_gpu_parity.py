@@ -534,6 +534,40 @@ def test_bicgstab_fp32_chebyshev_solve_matches_oracle(n, bcs):
     assert diff <= 10 * kappa * (res["error_compute_operator"] + ro["error_compute_operator"]), diff
 
 
+def test_large_bicgstab_chebyshev_matches_reference(golden):
+    """The reference's default GPU composition (BiCGSTAB + Chebyshev(24),
+    global eigenvalues, epsilon 0, rescale 100) on its test problem (origin
+    {3, 2.5, 10}, ds 0.1, bcsType {0,1,1,0,1,0}) at 130^3, one rank, against
+    the compiled reference's own run (bicgstab_cheb24_alp_130): the fused
+    Chebyshev pairs at a size where every block marches full z-chunks.  Bars
+    as for every BiCGSTAB case: iterations within 5 % (at least 2), the
+    solution sample within kappa (r_gpu + r_ref), both true residuals below
+    the tolerance when the reference's is."""
+    cases, _ = golden
+    name = "bicgstab_cheb24_alp_130"
+    if name not in cases:
+        pytest.skip("large reference cases not generated")
+    c = cases[name]
+    d = decode(c)
+    G, x, b, flux = oracle_problem(d)
+    C = ctx_for(G, d["bc"])
+    upload_problem(C, G, x, b, flux, d["bc"])
+    res = C.solve(d["chain"])
+    xg = C.download(L.FIELD_X)
+    C.close()
+    it_g, it_r = res["iterations"], c["iterations"]
+    samp = data_region(G, xg)[::c["x_sample_step"][0], ::c["x_sample_step"][1], ::c["x_sample_step"][2]].ravel()
+    diff = relmax(samp, np.array(c["x_sample"]))
+    e = G.g.eig_global
+    kappa = e[1] / e[0]
+    print(f"[{name}] GPU {it_g} it, reference {it_r} it, sample max rel dx {diff:.3e}, true residual "
+          f"{res['error_compute_operator']:.3e} / {c['error_compute_operator']:.3e}", flush=True)
+    assert abs(it_g - it_r) <= max(2, round(0.05 * it_r)), (it_g, it_r)
+    if c["error_compute_operator"] < 1e-10:
+        assert res["error_compute_operator"] < 1e-10
+    assert diff <= kappa * (res["error_compute_operator"] + c["error_compute_operator"]), diff
+
+
 @pytest.mark.parametrize("name", ["cg_none_normal_258_r8", "bicgstab_none_normal_258_r8", "cg_none_gauss_130_r8"])
 def test_large_iteration_counts_match_reference(golden, name):
     """config-2 grid (256^3 interior, seeded N(0,1) rhs): iteration counts of the

@@ -104,7 +104,8 @@ def test_multirank_bicgstab_converges_to_same_solution(golden):
     assert relmax(samp, np.array(c4["x_sample"])) < 1e-8
 
 
-@pytest.mark.parametrize("name", ["cg_none_normal_258_r8", "bicgstab_none_normal_258_r8", "cg_none_gauss_130_r8"])
+@pytest.mark.parametrize("name", ["cg_none_normal_258_r8", "bicgstab_none_normal_258_r8", "cg_none_gauss_130_r8",
+                                  "bicgstab_cheb24_alp_130"])
 def test_large_reference_runs_recorded(golden, name):
     """The large reference runs are parity targets for the GPU (iteration counts);
     here only check the records are sane."""
