@@ -601,12 +601,24 @@ __device__ __forceinline__ unsigned relu_bf2(unsigned h)
     return __builtin_bit_cast(unsigned, __builtin_elementwise_max(__builtin_bit_cast(s16x2, h), s16x2{0, 0}));
 }
 
+// c + h / 8 for the low (lo8) or high (hi8) BF16 h of a packed pair, in one
+// BF16 dot product: the products by 1/8 and 0 are exact, so the sum rounds
+// once, as c + h / 8 in FP32 does
+__device__ __forceinline__ float dot_lo8(unsigned h, float c)
+{
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, h), __builtin_bit_cast(bf16x2, 0x00003e00u), c, false);
+}
+__device__ __forceinline__ float dot_hi8(unsigned h, float c)
+{
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, h), __builtin_bit_cast(bf16x2, 0x3e000000u), c, false);
+}
+
 // Staging of the one-channel network input (an FP64 or FP32 field) into a
 // BF16 LDS plane: a thread owns kPer slots of the staged region.  Slots
 // outside the domain (and beyond the region) load 0 from an out-of-range
 // buffer offset; the loads have no control flow around them and stay in
 // flight until the plane is converted.
-template <bool F64, int kPer>
+template <bool F64, int kPer, int kThreads = 256>
 struct FieldStage {
     using Raw = std::conditional_t<F64, u32x2, unsigned>;
     const char* base;  // element (0, 0, 0) of sample b
@@ -622,16 +634,27 @@ struct FieldStage {
         scale = field_scale(f);
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-            const int t = tid + 256 * k;
+            const int t = tid + kThreads * k;
             const int yy = ry + t / rw, xx = rx + t % rw;
             const bool in = t < np && xx >= 0 && yy >= 0 && xx < W && yy < H;
             soff[k] = in ? (unsigned)((xx * f.sx + yy * f.sy) * esz) : kOOB;
         }
     }
-    __device__ void fetch(Raw* pre, int zi, int D) const
+    // planes at or beyond zend (the domain's or the chunk's end) read zeros
+    __device__ void fetch(Raw* pre, int zi, int zend) const
     {
-        const bool zin = zi >= 0 && zi < D;  // uniform
+        const bool zin = zi >= 0 && zi < zend;  // uniform
         const __amdgpu_buffer_rsrc_t r = buf_rsrc(base + (zin ? zi : 0) * szb, zin ? kInRange : 0);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if constexpr (F64) pre[k] = __builtin_amdgcn_raw_buffer_load_b64(r, soff[k], 0, 0);
+            else pre[k] = __builtin_amdgcn_raw_buffer_load_b32(r, soff[k], 0, 0);
+        }
+    }
+    // the plane at byte offset off from element (0, 0, 0), or zeros (!in)
+    __device__ void fetch_at(Raw* pre, long long off, bool in) const
+    {
+        const __amdgpu_buffer_rsrc_t r = buf_rsrc(base + (in ? off : 0), in ? kInRange : 0);
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             if constexpr (F64) pre[k] = __builtin_amdgcn_raw_buffer_load_b64(r, soff[k], 0, 0);
@@ -648,7 +671,9 @@ struct FieldStage {
     }
 };
 
-// Lift (1 -> 16 channels): the 27 taps of one channel as three K = 16 MFMAs
+// Lift (1 -> 16 channels), training form: writes the full-resolution output
+// and its pooled copy (inference, which keeps only the pooled copy, runs the
+// wide-tile k_liftw below).  The 27 taps of one channel as three K = 16 MFMAs
 // per 16 voxels over the z-packed quad ring; lane (voxel n, block j)
 // receives channels 4j .. 4j+3.  The accumulator starts at the bias, or at
 // -1e30 outside the domain, so relu leaves zeros there.  Ring entry q mod 2
@@ -697,9 +722,9 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
     constexpr int kPer = (kP + 255) / 256;
     unsigned hist[kPer] = {};  // BF16 of the slot's planes p-2 (low) and p-1 (high)
     using T = std::conditional_t<F64, double, float>;
+    using Raw = typename FieldStage<F64, kPer>::Raw;
     FieldStage<F64, kPer> fs, fa;
     fs.init(a.fin, b, x0 - 1, y0 - 1, kLW, kP, W, H, tid);
-    using Raw = typename FieldStage<F64, kPer>::Raw;
     // residual update: A p staged like the input, and the byte offsets of the
     // slots on this block's own tile (the others are kOOB: their store is dropped)
     T alpha = (T)0;
@@ -721,19 +746,20 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
             coff[k] = own ? fs.soff[k] : kOOB;
         }
     }
-    Raw preA[kPer], preB[kPer], apA[kPer], apB[kPer];
-    auto fetch = [&](Raw* pre, Raw* ap, int zi) {
-        fs.fetch(pre, zi, D);
-        if constexpr (UPD) fa.fetch(ap, zi, D);
+    // planes in flight: buffer i holds the raw loads of a plane issued NB - i
+    // steps ahead of its use (FP64 loads take twice the registers)
+    constexpr int NB = F64 ? 2 : 3;
+    Raw pre[NB][kPer], ap[NB][kPer];
+    // the chunk reads planes z0-1 .. z1 (z1 as the halo of its last plane)
+    const int zend = min(z1 + 1, D);
+    auto fetch = [&](Raw* p, Raw* q, int zi) {
+        fs.fetch(p, zi, zend);
+        if constexpr (UPD) fa.fetch(q, zi, zend);
     };
     // plane zi completes the quads of entry zi-1
-    auto store_plane = [&](const Raw* pre, const Raw* ap, int zi) {
+    // (rc: where its residual goes, with no records for a plane outside the chunk)
+    auto store_plane = [&](const Raw* p, const Raw* q, int zi, __amdgpu_buffer_rsrc_t rc) {
         u32x2* const dq = lq + ((zi - 1) & 1) * kE;
-        __amdgpu_buffer_rsrc_t rc;
-        if constexpr (UPD) {
-            const bool zown = zi >= z0 && zi < z1;  // uniform: a plane of this block's chunk
-            rc = buf_rsrc(cbase + (zown ? zi : z0) * fs.szb, zown ? kInRange : 0);
-        }
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             // slots beyond the region hold zeros (their offset is kOOB) and
@@ -741,41 +767,45 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
             const int t = tid + 256 * k;
             uint16_t h;
             if constexpr (UPD) {
-                T r, q;
+                T r, v;
                 if constexpr (F64) {
-                    r = __builtin_bit_cast(double, pre[k]);
-                    q = __builtin_bit_cast(double, ap[k]);
+                    r = __builtin_bit_cast(double, p[k]);
+                    v = __builtin_bit_cast(double, q[k]);
                 } else {
-                    r = __uint_as_float(pre[k]);
-                    q = __uint_as_float(ap[k]);
+                    r = __uint_as_float(p[k]);
+                    v = __uint_as_float(q[k]);
                 }
-                const T v = skip ? r : r - alpha * q;
+                const T rn = skip ? r : r - alpha * v;
                 if constexpr (F64) {
-                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rc, coff[k], 0, 0);
-                    h = (uint16_t)f2bf((float)(v * (double)fs.scale));
+                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, rn), rc, coff[k], 0, 0);
+                    h = (uint16_t)f2bf((float)(rn * (double)fs.scale));
                 } else {
-                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rc, coff[k], 0, 0);
-                    h = (uint16_t)f2bf(v * fs.scale);
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rn), rc, coff[k], 0, 0);
+                    h = (uint16_t)f2bf(rn * fs.scale);
                 }
             } else {
-                h = fs.bf(pre, k);
+                h = fs.bf(p, k);
             }
             dq[t] = u32x2{hist[k], (unsigned)h};
             hist[k] = (hist[k] >> 16) | ((unsigned)h << 16);
         }
     };
     // the two planes before the chunk fill the slots' history (their quads
-    // land in entries that are rewritten before they are read)
-    fetch(preA, apA, z0 - 1);
-    store_plane(preA, apA, z0 - 1);
-    fetch(preA, apA, z0);
-    store_plane(preA, apA, z0);
-    fetch(preA, apA, z0 + 1);
-    fetch(preB, apB, z0 + 2);
-    // pooled copy (2x2x2 average of the stored BF16 outputs) for the next level:
+    // land in entries that are rewritten before they are read); all loads of
+    // the prologue are issued before the first is used
+    {
+        Raw h0[kPer], h1[kPer], g0[kPer], g1[kPer];
+        fetch(h0, g0, z0 - 1);
+        fetch(h1, g1, z0);
+#pragma unroll
+        for (int i = 0; i < NB; ++i) fetch(pre[i], ap[i], z0 + 1 + i);
+        store_plane(h0, g0, z0 - 1, buf_rsrc(cbase, 0));
+        store_plane(h1, g1, z0, buf_rsrc(cbase + z0 * fs.szb, kInRange));
+    }
+    // pooled copy (2x2x2 average of the BF16 outputs) for the next level:
     // y pairs are neighbouring rows of this wave and z pairs consecutive planes
-    // (chunks start on even planes), summed per lane; x pairs are neighbouring
-    // lanes, added when the pooled voxel is written
+    // (chunks start on even planes), summed per lane (each output times 1/8);
+    // x pairs are neighbouring lanes, added when the pooled voxel is written
     float ps[2][4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
     // byte offset of this lane's pooled voxel in a pooled plane (row pair ip),
     // kOOB for odd lanes and voxels outside the level: their store is dropped
@@ -786,33 +816,46 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
         poff[ip] = (!(n & 1) && x < W && yf < H) ? (unsigned)(2 * (((yf >> 1) * a.pout.W + (x >> 1)) * kC + 4 * j))
                                                   : kOOB;
     }
-    // one plane: two planes of loads in flight, each buffer refilled once written
+    // one plane: NB planes of loads in flight, each buffer refilled once
+    // written.  The four rows have no branch between them, so their MFMA
+    // chains overlap.
+    // byte offset of the plane the next step fetches (plane z + 1 + NB)
+    long long foff = (long long)(z0 + 1 + NB) * fs.szb;
+    const long long back = (long long)NB * fs.szb;
     auto step = [&](int z, Raw* buf, Raw* abuf) {
-        store_plane(buf, abuf, z + 1);
-        fetch(buf, abuf, z + 3);  // zeros beyond the domain; unused beyond the chunk
+        const bool own = z + 1 < z1;  // uniform: plane z + 1 is one of the chunk's
+        store_plane(buf, abuf, z + 1, buf_rsrc(cbase + (foff - back), own ? kInRange : 0));
+        const bool in = z + 1 + NB < zend;  // zeros beyond the chunk's last halo plane
+        fs.fetch_at(buf, foff, in);
+        if constexpr (UPD) fa.fetch_at(abuf, foff, in);
+        foff += fs.szb;
         __syncthreads();
         const u32x2* const q = lq + (z & 1) * kE;  // entry of output plane z
         u32x2 qr[6];
 #pragma unroll
         for (int r = 0; r < 6; ++r) qr[r] = q[pq + r * kLW];
+        f32x4 acc[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = cinit[i];
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wq[dy], __builtin_bit_cast(s16x4, qr[i + dy]), acc[i], 0, 0, 0);
+        u32x2 o[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            f32x4 acc = cinit[i];
+            o[i][0] = relu_bf2(pack2bf(acc[i][0], acc[i][1]));
+            o[i][1] = relu_bf2(pack2bf(acc[i][2], acc[i][3]));
+            ps[i >> 1][0] = dot_lo8(o[i][0], ps[i >> 1][0]);
+            ps[i >> 1][1] = dot_hi8(o[i][0], ps[i >> 1][1]);
+            ps[i >> 1][2] = dot_lo8(o[i][1], ps[i >> 1][2]);
+            ps[i >> 1][3] = dot_hi8(o[i][1], ps[i >> 1][3]);
+        }
 #pragma unroll
-            for (int dy = 0; dy < 3; ++dy)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wq[dy], __builtin_bit_cast(s16x4, qr[i + dy]), acc, 0, 0, 0);
-            u32x2 o;
-            o[0] = relu_bf2(pack2bf(acc[0], acc[1]));
-            o[1] = relu_bf2(pack2bf(acc[2], acc[3]));
+        for (int i = 0; i < 4; ++i) {
             const int y = y0 + wave * 4 + i;
-            if (a.out.p && x < W && y < H)  // inference keeps only the pooled copy
-                *(u32x2*)(a.out.p + vox(a.out, b, x, y, z) + 4 * j) = o;
-            if (a.pout.p) {
-                ps[i >> 1][0] += bf2f(o[0] & 0xffffu);
-                ps[i >> 1][1] += __uint_as_float(o[0] & 0xffff0000u);
-                ps[i >> 1][2] += bf2f(o[1] & 0xffffu);
-                ps[i >> 1][3] += __uint_as_float(o[1] & 0xffff0000u);
-            }
+            if (x < W && y < H) *(u32x2*)(a.out.p + vox(a.out, b, x, y, z) + 4 * j) = o[i];
         }
         if (a.pout.p && ((z & 1) || z == D - 1)) {  // uniform
             const __amdgpu_buffer_rsrc_t rp = buf_rsrc(a.pout.p + vox(a.pout, b, 0, 0, z >> 1), kInRange);
@@ -820,20 +863,262 @@ __global__ __launch_bounds__(256, 4) void k_lift(ConvArgs a, int zc)
             for (int ip = 0; ip < 2; ++ip) {
                 float v[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) v[r] = 0.125f * (ps[ip][r] + xor1(ps[ip][r]));
-                u32x2 o;
-                o[0] = pack2bf(v[0], v[1]);
-                o[1] = pack2bf(v[2], v[3]);
-                __builtin_amdgcn_raw_buffer_store_b64(o, rp, poff[ip], 0, 0);
+                for (int r = 0; r < 4; ++r) v[r] = ps[ip][r] + xor1(ps[ip][r]);
+                u32x2 ov;
+                ov[0] = pack2bf(v[0], v[1]);
+                ov[1] = pack2bf(v[2], v[3]);
+                __builtin_amdgcn_raw_buffer_store_b64(ov, rp, poff[ip], 0, 0);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) ps[ip][r] = 0.0f;
             }
         }
     };
-    for (int z = z0; z < z1; z += 2) {
-        step(z, preA, apA);
-        if (z + 1 < z1) step(z + 1, preB, apB);
+    // the loop body runs all NB steps: a conditional step inside it would
+    // leave fewer loads behind each wait on the path that skips it, and the
+    // waits of every step would drain the prefetch.  The rest of the chunk
+    // follows the loop.
+    int z = z0;
+    for (; z + NB <= z1; z += NB) {
+#pragma unroll
+        for (int i = 0; i < NB; ++i) step(z + i, pre[i], ap[i]);
     }
+#pragma unroll
+    for (int i = 0; i < NB - 1; ++i)
+        if (z + i < z1) step(z + i, pre[i], ap[i]);
+}
+
+// Wide-tile lift for inference, which keeps only the pooled copy: a block
+// owns 64 x 16 output voxels per plane, its 8 wavefronts 4 rows x 2 segments
+// of 16 voxels each (wavefront w: rows 4 (w & 3) .., segments at x offsets
+// 32 (w >> 2) and 32 (w >> 2) + 16).  The staged rows are 66 values wide, so
+// the input loads and the residual stores cover whole cache lines, and the
+// halo adds 16 % to the loads (27 % with 16 x 16 tiles).  Operands, ring and
+// residual update as in k_lift; the ring entry holds 18 x 66 quads.
+constexpr int kWX = 64, kWY = 16, kWLW = kWX + 2, kWLH = kWY + 2, kWThreads = 512;
+
+template <bool F64, bool UPD>
+__global__ __launch_bounds__(kWThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_liftw(ConvArgs a, int zc)
+{
+    if ((a.gate && *a.gate) || (a.gate2 && *a.gate2)) return;
+    constexpr int kP = kWLH * kWLW;
+    constexpr int kPer = (kP + kWThreads - 1) / kWThreads;
+    // ring entry: every slot of the staging threads (those beyond the region
+    // store their zeros in the tail instead of branching)
+    constexpr int kE = kPer * kWThreads;
+    __shared__ __align__(16) u32x2 lq[2 * kE];
+    const int D = a.out.D, H = a.out.H, W = a.out.W;
+    const int nzc = (D + zc - 1) / zc;
+    const TileIdx ti = tile_of_block();
+    const int b = ti.z / nzc;
+    const int z0 = (ti.z - b * nzc) * zc;
+    const int z1 = min(z0 + zc, D);
+    const int x0 = ti.x * kWX, y0 = ti.y * kWY;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = lane >> 4, n = lane & 15;
+    const int wr = 4 * (wave & 3), sx = 32 * (wave >> 2);
+    s16x4 wq[3];  // per dy
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy) wq[dy] = __builtin_bit_cast(s16x4, ((const u32x2*)a.wl)[64 * dy + lane]);
+    // the accumulators start at the bias, or at -1e30 for voxels outside the
+    // domain (relu leaves zeros there); vin bit 4 s + i: voxel of segment s,
+    // row i inside
+    f32x4 cb;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) cb[r] = a.bias ? a.bias[4 * j + r] : 0.0f;
+    unsigned vin = 0u;
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (x0 + sx + 16 * s + n < W && y0 + wr + i < H) vin |= 1u << (4 * s + i);
+    // this lane's quad in ring row wr for segment 0 (segment 1: + 16)
+    const int pq = wr * kWLW + sx + n + lift_dx(j);
+    unsigned hist[kPer] = {};  // BF16 of the slot's planes p-2 (low) and p-1 (high)
+    using T = std::conditional_t<F64, double, float>;
+    using Raw = typename FieldStage<F64, kPer, kWThreads>::Raw;
+    FieldStage<F64, kPer, kWThreads> fs;
+    fs.init(a.fin, b, x0 - 1, y0 - 1, kWLW, kP, W, H, tid);
+    // residual update: A p staged like the input (same layout, so through the
+    // same slot offsets), and the slots on this block's own tile (bit k of
+    // ownm; the other slots' stores go to kOOB and are dropped)
+    T alpha = (T)0;
+    bool skip = true;
+    unsigned ownm = 0u;
+    const char* cbase = nullptr;
+    const char* abase = nullptr;
+    if constexpr (UPD) {
+        abase = (const char*)a.fap + (a.fin.off + b * a.fin.sb) * (F64 ? 8 : 4);
+        alpha = (T)*a.alpha;
+        skip = *a.skip != 0;
+        cbase = (const char*)a.fcur + (a.fin.off + b * a.fin.sb) * (F64 ? 8 : 4);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int t = tid + kWThreads * k;
+            const int yy = t / kWLW, xx = t - (t / kWLW) * kWLW;
+            if (t < kP && xx >= 1 && xx <= kWX && yy >= 1 && yy <= kWY) ownm |= 1u << k;
+        }
+    }
+    auto fetch_ap = [&](Raw* q, long long off, bool in) {
+        const __amdgpu_buffer_rsrc_t r = buf_rsrc(abase + (in ? off : 0), in ? kInRange : 0);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if constexpr (F64) q[k] = __builtin_amdgcn_raw_buffer_load_b64(r, fs.soff[k], 0, 0);
+            else q[k] = __builtin_amdgcn_raw_buffer_load_b32(r, fs.soff[k], 0, 0);
+        }
+    };
+    // two planes of loads in flight: buffer i holds plane z + 1 + i at step z
+    Raw pre[2][kPer], ap[2][kPer];
+    // the chunk reads planes z0-1 .. z1 (z1 as the halo of its last plane)
+    const int zend = min(z1 + 1, D);
+    // plane zi completes the quads of entry zi-1 (rc: where its residual goes,
+    // with no records for a plane outside the chunk)
+    auto store_plane = [&](const Raw* p, const Raw* q, int zi, __amdgpu_buffer_rsrc_t rc) {
+        u32x2* const dq = lq + ((zi - 1) & 1) * kE;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int t = tid + kWThreads * k;
+            uint16_t h;
+            if constexpr (UPD) {
+                T r, v;
+                if constexpr (F64) {
+                    r = __builtin_bit_cast(double, p[k]);
+                    v = __builtin_bit_cast(double, q[k]);
+                } else {
+                    r = __uint_as_float(p[k]);
+                    v = __uint_as_float(q[k]);
+                }
+                const T rn = skip ? r : r - alpha * v;
+                if constexpr (F64) {
+                    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, rn), rc, ((ownm >> k) & 1u) ? fs.soff[k] : kOOB, 0, 0);
+                    h = (uint16_t)f2bf((float)(rn * (double)fs.scale));
+                } else {
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(rn), rc, ((ownm >> k) & 1u) ? fs.soff[k] : kOOB, 0, 0);
+                    h = (uint16_t)f2bf(rn * fs.scale);
+                }
+            } else {
+                h = fs.bf(p, k);
+            }
+            dq[t] = u32x2{hist[k], (unsigned)h};
+            hist[k] = (hist[k] >> 16) | ((unsigned)h << 16);
+        }
+    };
+    // prologue: the two planes before the chunk fill the slots' history; all
+    // its loads are issued before the first is used
+    {
+        Raw h0[kPer], h1[kPer], g0[kPer], g1[kPer];
+        fs.fetch(h0, z0 - 1, zend);
+        fs.fetch(h1, z0, zend);
+        fs.fetch(pre[0], z0 + 1, zend);
+        fs.fetch(pre[1], z0 + 2, zend);
+        if constexpr (UPD) {
+            fetch_ap(g0, (long long)(z0 - 1) * fs.szb, z0 >= 1);
+            fetch_ap(g1, (long long)z0 * fs.szb, true);
+            fetch_ap(ap[0], (long long)(z0 + 1) * fs.szb, z0 + 1 < zend);
+            fetch_ap(ap[1], (long long)(z0 + 2) * fs.szb, z0 + 2 < zend);
+        }
+        store_plane(h0, g0, z0 - 1, buf_rsrc(cbase, 0));
+        store_plane(h1, g1, z0, buf_rsrc(cbase + z0 * fs.szb, kInRange));
+    }
+    // pooled copy (2x2x2 average of the BF16 outputs): per segment s and row
+    // pair ip, this lane's sum of the outputs times 1/8 over the pair's rows
+    // and the plane pair (chunks start on even planes); x pairs are
+    // neighbouring lanes, added when the pooled voxel is written (kOOB: odd
+    // lanes and voxels outside the level, whose stores are dropped)
+    float ps[2][2][4];
+    // pooled voxel of segment s, row pair ip: pbase + 256 s + prow ip (bytes);
+    // it exists where the segment's voxel of row 2 ip does (vin)
+    const unsigned pbase = (unsigned)(2 * ((((y0 + wr) >> 1) * a.pout.W + ((x0 + sx + n) >> 1)) * kC + 4 * j));
+    const unsigned prow = (unsigned)(2 * a.pout.W * kC);
+    long long foff = (long long)(z0 + 3) * fs.szb;  // byte offset of the plane the next step fetches
+    const long long back = 2 * fs.szb;
+    // one plane.  FIRST: the first plane of a pair (the pooled sums start
+    // over); EDGE: the block's tile reaches beyond the domain
+    auto step = [&](int z, Raw* buf, Raw* abuf, auto first_c, auto edge_c) {
+        constexpr bool FIRST = decltype(first_c)::value, EDGE = decltype(edge_c)::value;
+        const bool own = z + 1 < z1;  // uniform: plane z + 1 is one of the chunk's
+        store_plane(buf, abuf, z + 1, buf_rsrc(cbase + (foff - back), own ? kInRange : 0));
+        const bool in = z + 3 < zend;  // zeros beyond the chunk's last halo plane
+        fs.fetch_at(buf, foff, in);
+        if constexpr (UPD) fetch_ap(abuf, foff, in);
+        foff += fs.szb;
+        __syncthreads();
+        const u32x2* const q = lq + (z & 1) * kE;  // entry of output plane z
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            u32x2 qr[6];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) qr[r] = q[pq + 16 * s + r * kWLW];
+            f32x4 acc[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if constexpr (EDGE) acc[i] = ((vin >> (4 * s + i)) & 1u) ? cb : f32x4{-1e30f, -1e30f, -1e30f, -1e30f};
+                else acc[i] = cb;
+            }
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wq[dy], __builtin_bit_cast(s16x4, qr[i + dy]), acc[i], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const unsigned o0 = relu_bf2(pack2bf(acc[i][0], acc[i][1]));
+                const unsigned o1 = relu_bf2(pack2bf(acc[i][2], acc[i][3]));
+                float* const p = ps[s][i >> 1];
+                const bool fresh = FIRST && !(i & 1);  // compile-time
+                p[0] = dot_lo8(o0, fresh ? 0.0f : p[0]);
+                p[1] = dot_hi8(o0, fresh ? 0.0f : p[1]);
+                p[2] = dot_lo8(o1, fresh ? 0.0f : p[2]);
+                p[3] = dot_hi8(o1, fresh ? 0.0f : p[3]);
+            }
+            // one segment's operands and accumulators live at a time: its
+            // sums are formed here (not sunk into the pooled store's branch)
+#pragma unroll
+            for (int ip = 0; ip < 2; ++ip)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) asm volatile("" : "+v"(ps[s][ip][c]));
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if ((z & 1) || z == D - 1) {  // uniform
+            const __amdgpu_buffer_rsrc_t rp = buf_rsrc(a.pout.p + vox(a.pout, b, 0, 0, z >> 1), kInRange);
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int ip = 0; ip < 2; ++ip) {
+                    float v[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) v[r] = ps[s][ip][r] + xor1(ps[s][ip][r]);
+                    u32x2 ov;
+                    ov[0] = pack2bf(v[0], v[1]);
+                    ov[1] = pack2bf(v[2], v[3]);
+                    const bool pin = !(n & 1) && ((vin >> (4 * s + 2 * ip)) & 1u);
+                    __builtin_amdgcn_raw_buffer_store_b64(ov, rp, pin ? pbase + 256u * s + prow * ip : kOOB, 0, 0);
+                }
+        }
+    };
+    // the loop body runs both steps of a plane pair (a conditional step inside
+    // it would leave fewer loads behind the waits of the path that skips it,
+    // and every wait would drain the prefetch); an odd last plane follows it
+    auto run = [&](auto edge_c) {
+        int z = z0;
+        for (; z + 2 <= z1; z += 2) {
+            step(z, pre[0], ap[0], std::true_type{}, edge_c);
+            step(z + 1, pre[1], ap[1], std::false_type{}, edge_c);
+        }
+        if (z < z1) step(z, pre[0], ap[0], std::true_type{}, edge_c);
+    };
+    if (x0 + kWX > W || y0 + kWY > H) run(std::true_type{});
+    else run(std::false_type{});
+}
+
+// z-chunk of the wide-tile lift: at least 2048 blocks (4 per CU), chunks of
+// at least 16 planes, starting on even planes
+static int pick_zc_w(int D, int H, int W, int B)
+{
+    const long long tiles = (long long)((W + kWX - 1) / kWX) * ((H + kWY - 1) / kWY) * B;
+    int zc = D;
+    while (zc > 16 && tiles * ((D + zc - 1) / zc) < 2048) zc = (zc + 1) / 2;
+    if ((zc & 1) && zc < D) ++zc;
+    return std::max(zc, 1);
 }
 
 // z-chunk: enough blocks to fill the chip, at most one chunk per 16 planes
@@ -884,14 +1169,28 @@ void launch_conv(const ConvArgs& a, hipStream_t s)
 
 void launch_lift(const ConvArgs& a, hipStream_t s)
 {
-    const int zc = pick_zc(a.out.D, a.out.H, a.out.W, a.out.B);
-    const dim3 grid = conv_grid(a.out.D, a.out.H, a.out.W, a.out.B, zc);
+    if (a.out.p) {  // training: the full-resolution output and its pooled copy
+        const int zc = pick_zc(a.out.D, a.out.H, a.out.W, a.out.B);
+        const dim3 grid = conv_grid(a.out.D, a.out.H, a.out.W, a.out.B, zc);
+        if (a.fcur) {
+            if (a.fin.is64) hipLaunchKernelGGL((k_lift<true, true>), grid, dim3(256), 0, s, a, zc);
+            else hipLaunchKernelGGL((k_lift<false, true>), grid, dim3(256), 0, s, a, zc);
+        } else {
+            if (a.fin.is64) hipLaunchKernelGGL((k_lift<true, false>), grid, dim3(256), 0, s, a, zc);
+            else hipLaunchKernelGGL((k_lift<false, false>), grid, dim3(256), 0, s, a, zc);
+        }
+        return;
+    }
+    // inference keeps only the pooled copy: the wide-tile kernel
+    const int zw = pick_zc_w(a.out.D, a.out.H, a.out.W, a.out.B);
+    const dim3 gw((unsigned)((a.out.W + kWX - 1) / kWX), (unsigned)((a.out.H + kWY - 1) / kWY),
+                  (unsigned)(a.out.B * ((a.out.D + zw - 1) / zw)));
     if (a.fcur) {
-        if (a.fin.is64) hipLaunchKernelGGL((k_lift<true, true>), grid, dim3(256), 0, s, a, zc);
-        else hipLaunchKernelGGL((k_lift<false, true>), grid, dim3(256), 0, s, a, zc);
+        if (a.fin.is64) hipLaunchKernelGGL((k_liftw<true, true>), gw, dim3(kWThreads), 0, s, a, zw);
+        else hipLaunchKernelGGL((k_liftw<false, true>), gw, dim3(kWThreads), 0, s, a, zw);
     } else {
-        if (a.fin.is64) hipLaunchKernelGGL((k_lift<true, false>), grid, dim3(256), 0, s, a, zc);
-        else hipLaunchKernelGGL((k_lift<false, false>), grid, dim3(256), 0, s, a, zc);
+        if (a.fin.is64) hipLaunchKernelGGL((k_liftw<true, false>), gw, dim3(kWThreads), 0, s, a, zw);
+        else hipLaunchKernelGGL((k_liftw<false, false>), gw, dim3(kWThreads), 0, s, a, zw);
     }
 }
 
