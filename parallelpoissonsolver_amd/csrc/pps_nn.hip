@@ -1408,7 +1408,10 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     // partials buffer of X plane s.  One barrier per step: the ring entry of
     // plane s is staged; the partials of plane s - 1 are complete; the gather
     // of plane s - 2, which read the buffer that plane s fills, is done.
-    auto step = [&](int s, int es, Raw* buf, u32x2& dbuf) {
+    auto step = [&](int s, int es, Raw* buf, u32x2& dbuf, auto full_c) {
+        // FULL: a step of the chunk's interior (z0 + 2 <= s <= z1 - 2), which
+        // has every part below
+        constexpr bool FULL = decltype(full_c)::value;
         store_plane(buf, s + 1);
         {
             const double d64 = __builtin_bit_cast(double, dbuf), d32 = (double)__uint_as_float(dbuf[0]);
@@ -1422,10 +1425,10 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
             dbuf = __builtin_amdgcn_raw_buffer_load_b64(r, ooff, 0, 0);
         }
         __syncthreads();
-        if (s >= 0 && s < D && s <= z1) partials(ev, es);  // uniform
-        if (s + 1 <= z1 && s + 1 < D) fetch_e(ev, s + 1);
+        if (FULL || (s >= 0 && s < D && s <= z1)) partials(ev, es);  // uniform
+        if (FULL || (s + 1 <= z1 && s + 1 < D)) fetch_e(ev, s + 1);
         const int sg = s - 1;                        // X plane gathered
-        if (sg >= 0 && sg < D && sg >= z0 - 1) {      // uniform; every thread gathers,
+        if (FULL || (sg >= 0 && sg < D && sg >= z0 - 1)) {  // uniform; every thread gathers,
             const float* const lg = lp + (es ^ 1) * kLPS;  // those without an output read the padding
             float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
@@ -1440,7 +1443,7 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
             ap1 += s0;
         }
         const int zo = s - 2;
-        if (zo >= z0 && zo < z1) {  // uniform
+        if (FULL || (zo >= z0 && zo < z1)) {  // uniform
             const float v = am1 + bout;
             const __amdgpu_buffer_rsrc_t r = buf_rsrc(obase + zo * oszb, kInRange);
             if constexpr (O64) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, (double)v), r, ooff, 0, 0);
@@ -1454,10 +1457,24 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
         a0 = ap1;
         ap1 = 0.0f;
     };
-    for (int s = z0 - 1; s <= z1 + 1; s += 2) {
-        step(s, s & 1, preA, dA);
-        if (s + 1 <= z1 + 1) step(s + 1, (s + 1) & 1, preB, dB);
+    // steps s = z0 - 1 .. z1 + 1 alternate the buffers A, B from A.  The
+    // interior pairs run without a conditional step or part (a skipped step
+    // would leave fewer loads behind the waits of the path that skips it, and
+    // every wait would drain the prefetch); the steps at the chunk's ends
+    // check which parts apply.
+    step(z0 - 1, (z0 - 1) & 1, preA, dA, std::false_type{});
+    step(z0, z0 & 1, preB, dB, std::false_type{});
+    step(z0 + 1, (z0 + 1) & 1, preA, dA, std::false_type{});
+    int s = z0 + 2;
+    for (; s + 1 <= z1 - 2; s += 2) {
+        step(s, s & 1, preB, dB, std::true_type{});
+        step(s + 1, (s + 1) & 1, preA, dA, std::true_type{});
     }
+    // the steps left up to z1 + 1: 3 or 4, or 1 or 2 for a chunk of 1 or 2 planes
+    if (s <= z1 + 1) step(s, s & 1, preB, dB, std::false_type{});
+    if (s + 1 <= z1 + 1) step(s + 1, (s + 1) & 1, preA, dA, std::false_type{});
+    if (s + 2 <= z1 + 1) step(s + 2, (s + 2) & 1, preB, dB, std::false_type{});
+    if (s + 3 <= z1 + 1) step(s + 3, (s + 3) & 1, preA, dA, std::false_type{});
     if (a.dot) {
         __shared__ double red[4];
 #pragma unroll
