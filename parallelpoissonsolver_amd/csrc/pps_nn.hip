@@ -966,8 +966,10 @@ __global__ __launch_bounds__(kWThreads) __attribute__((amdgpu_waves_per_eu(4, 4)
             else q[k] = __builtin_amdgcn_raw_buffer_load_b32(r, fs.soff[k], 0, 0);
         }
     };
-    // two planes of loads in flight: buffer i holds plane z + 1 + i at step z
-    Raw pre[2][kPer], ap[2][kPer];
+    // planes of loads in flight: buffer i holds plane z + 1 + i at step z
+    // (FP64 with the update: one, as two do not fit the registers)
+    constexpr int NB = (F64 && UPD) ? 1 : 2;
+    Raw pre[NB][kPer], ap[NB][kPer];
     // the chunk reads planes z0-1 .. z1 (z1 as the halo of its last plane)
     const int zend = min(z1 + 1, D);
     // plane zi completes the quads of entry zi-1 (rc: where its residual goes,
@@ -1008,13 +1010,13 @@ __global__ __launch_bounds__(kWThreads) __attribute__((amdgpu_waves_per_eu(4, 4)
         Raw h0[kPer], h1[kPer], g0[kPer], g1[kPer];
         fs.fetch(h0, z0 - 1, zend);
         fs.fetch(h1, z0, zend);
-        fs.fetch(pre[0], z0 + 1, zend);
-        fs.fetch(pre[1], z0 + 2, zend);
+#pragma unroll
+        for (int i = 0; i < NB; ++i) fs.fetch(pre[i], z0 + 1 + i, zend);
         if constexpr (UPD) {
             fetch_ap(g0, (long long)(z0 - 1) * fs.szb, z0 >= 1);
             fetch_ap(g1, (long long)z0 * fs.szb, true);
-            fetch_ap(ap[0], (long long)(z0 + 1) * fs.szb, z0 + 1 < zend);
-            fetch_ap(ap[1], (long long)(z0 + 2) * fs.szb, z0 + 2 < zend);
+#pragma unroll
+            for (int i = 0; i < NB; ++i) fetch_ap(ap[i], (long long)(z0 + 1 + i) * fs.szb, z0 + 1 + i < zend);
         }
         store_plane(h0, g0, z0 - 1, buf_rsrc(cbase, 0));
         store_plane(h1, g1, z0, buf_rsrc(cbase + z0 * fs.szb, kInRange));
@@ -1029,15 +1031,15 @@ __global__ __launch_bounds__(kWThreads) __attribute__((amdgpu_waves_per_eu(4, 4)
     // it exists where the segment's voxel of row 2 ip does (vin)
     const unsigned pbase = (unsigned)(2 * ((((y0 + wr) >> 1) * a.pout.W + ((x0 + sx + n) >> 1)) * kC + 4 * j));
     const unsigned prow = (unsigned)(2 * a.pout.W * kC);
-    long long foff = (long long)(z0 + 3) * fs.szb;  // byte offset of the plane the next step fetches
-    const long long back = 2 * fs.szb;
+    long long foff = (long long)(z0 + 1 + NB) * fs.szb;  // byte offset of the plane the next step fetches
+    const long long back = NB * fs.szb;
     // one plane.  FIRST: the first plane of a pair (the pooled sums start
     // over); EDGE: the block's tile reaches beyond the domain
     auto step = [&](int z, Raw* buf, Raw* abuf, auto first_c, auto edge_c) {
         constexpr bool FIRST = decltype(first_c)::value, EDGE = decltype(edge_c)::value;
         const bool own = z + 1 < z1;  // uniform: plane z + 1 is one of the chunk's
         store_plane(buf, abuf, z + 1, buf_rsrc(cbase + (foff - back), own ? kInRange : 0));
-        const bool in = z + 3 < zend;  // zeros beyond the chunk's last halo plane
+        const bool in = z + 1 + NB < zend;  // zeros beyond the chunk's last halo plane
         fs.fetch_at(buf, foff, in);
         if constexpr (UPD) fetch_ap(abuf, foff, in);
         foff += fs.szb;
@@ -1102,7 +1104,7 @@ __global__ __launch_bounds__(kWThreads) __attribute__((amdgpu_waves_per_eu(4, 4)
         int z = z0;
         for (; z + 2 <= z1; z += 2) {
             step(z, pre[0], ap[0], std::true_type{}, edge_c);
-            step(z + 1, pre[1], ap[1], std::false_type{}, edge_c);
+            step(z + 1, pre[NB - 1], ap[NB - 1], std::false_type{}, edge_c);
         }
         if (z < z1) step(z, pre[0], ap[0], std::true_type{}, edge_c);
     };
@@ -1408,10 +1410,11 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     // partials buffer of X plane s.  One barrier per step: the ring entry of
     // plane s is staged; the partials of plane s - 1 are complete; the gather
     // of plane s - 2, which read the buffer that plane s fills, is done.
-    auto step = [&](int s, int es, Raw* buf, u32x2& dbuf, auto full_c) {
+    auto step = [&](int s, int es, Raw* buf, u32x2& dbuf, auto full_c, auto keep_c) {
         // FULL: a step of the chunk's interior (z0 + 2 <= s <= z1 - 2), which
-        // has every part below
-        constexpr bool FULL = decltype(full_c)::value;
+        // has every part below; KEEP: s is even, so ev already holds the
+        // coarse plane (s + 1) / 2 (loaded by step s - 1)
+        constexpr bool FULL = decltype(full_c)::value, KEEP = decltype(keep_c)::value;
         store_plane(buf, s + 1);
         {
             const double d64 = __builtin_bit_cast(double, dbuf), d32 = (double)__uint_as_float(dbuf[0]);
@@ -1426,7 +1429,8 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
         }
         __syncthreads();
         if (FULL || (s >= 0 && s < D && s <= z1)) partials(ev, es);  // uniform
-        if (FULL || (s + 1 <= z1 && s + 1 < D)) fetch_e(ev, s + 1);
+        if constexpr (!KEEP)
+            if (FULL || (s + 1 <= z1 && s + 1 < D)) fetch_e(ev, s + 1);
         const int sg = s - 1;                        // X plane gathered
         if (FULL || (sg >= 0 && sg < D && sg >= z0 - 1)) {  // uniform; every thread gathers,
             const float* const lg = lp + (es ^ 1) * kLPS;  // those without an output read the padding
@@ -1462,19 +1466,22 @@ __global__ __launch_bounds__(256, 4) void k_outf(ConvArgs a, int zc)
     // would leave fewer loads behind the waits of the path that skips it, and
     // every wait would drain the prefetch); the steps at the chunk's ends
     // check which parts apply.
-    step(z0 - 1, (z0 - 1) & 1, preA, dA, std::false_type{});
-    step(z0, z0 & 1, preB, dB, std::false_type{});
-    step(z0 + 1, (z0 + 1) & 1, preA, dA, std::false_type{});
+    // (chunks start on even planes, so the interior pairs start on even s)
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    step(z0 - 1, (z0 - 1) & 1, preA, dA, F_{}, F_{});
+    step(z0, z0 & 1, preB, dB, F_{}, F_{});
+    step(z0 + 1, (z0 + 1) & 1, preA, dA, F_{}, F_{});
     int s = z0 + 2;
     for (; s + 1 <= z1 - 2; s += 2) {
-        step(s, s & 1, preB, dB, std::true_type{});
-        step(s + 1, (s + 1) & 1, preA, dA, std::true_type{});
+        step(s, s & 1, preB, dB, T_{}, T_{});
+        step(s + 1, (s + 1) & 1, preA, dA, T_{}, F_{});
     }
     // the steps left up to z1 + 1: 3 or 4, or 1 or 2 for a chunk of 1 or 2 planes
-    if (s <= z1 + 1) step(s, s & 1, preB, dB, std::false_type{});
-    if (s + 1 <= z1 + 1) step(s + 1, (s + 1) & 1, preA, dA, std::false_type{});
-    if (s + 2 <= z1 + 1) step(s + 2, (s + 2) & 1, preB, dB, std::false_type{});
-    if (s + 3 <= z1 + 1) step(s + 3, (s + 3) & 1, preA, dA, std::false_type{});
+    if (s <= z1 + 1) step(s, s & 1, preB, dB, F_{}, F_{});
+    if (s + 1 <= z1 + 1) step(s + 1, (s + 1) & 1, preA, dA, F_{}, F_{});
+    if (s + 2 <= z1 + 1) step(s + 2, (s + 2) & 1, preB, dB, F_{}, F_{});
+    if (s + 3 <= z1 + 1) step(s + 3, (s + 3) & 1, preA, dA, F_{}, F_{});
     if (a.dot) {
         __shared__ double red[4];
 #pragma unroll
@@ -1493,6 +1500,7 @@ static dim3 outf_grid(int D, int H, int W, int* zc)
     const int tiles = ((W + kFW - 1) / kFW) * ((H + kFH - 1) / kFH);
     int c = D;
     while (c > 16 && (long long)tiles * ((D + c - 1) / c) < 8192) c = (c + 1) / 2;
+    if ((c & 1) && c < D) ++c;  // chunks start on even planes
     *zc = c;
     return dim3((unsigned)((W + kFW - 1) / kFW), (unsigned)((H + kFH - 1) / kFH), (unsigned)((D + c - 1) / c));
 }
