@@ -64,7 +64,7 @@ def chains():
                            dict(kind=L.NONE, precision=32)],
         "nn_fcg_fp64": [dict(kind=L.FCG, tol=TOL, maxit=20000, true_finish=FINISH),
                         dict(kind=L.NN, stall_window=50, stall_ratio=0.9)],
-        "ir_nnfcg": [dict(kind=L.IR, tol=TOL, maxit=100, first_tol=2e-3),
+        "ir_nnfcg": [dict(kind=L.IR, tol=TOL, maxit=100, first_tol=2e-3, adapt=0.5),
                      dict(kind=L.FCG, precision=32, tol=1e-4, maxit=20000),
                      dict(kind=L.NN, precision=32, stall_window=50, stall_ratio=0.9)],
         # the reference's default GPU method (solverPoissonMPI_alpaka/include/inputParam.hpp:36,

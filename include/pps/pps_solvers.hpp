@@ -513,16 +513,20 @@ public:
 // FP64 outer refinement r = b - A u, u += d around an FP32 inner solver type
 // (BaseCG or FCG with T_data = float); no reference counterpart.
 // firstInnerTolMicro > 0: the inner tolerance of the first outer step, in
-// units of 1e-6 (pps_stage::ir_first_tol)
-template <int DIM, int tolerance, int maxOuter, typename T_Inner, int firstInnerTolMicro = 0>
+// units of 1e-6 (pps_stage::ir_first_tol); adaptPercent > 0: an outer step
+// asks the inner solver for no more reduction than the outer tolerance still
+// needs, times adaptPercent / 100 (pps_stage::ir_adapt)
+template <int DIM, int tolerance, int maxOuter, typename T_Inner, int firstInnerTolMicro = 0, int adaptPercent = 0>
 class MixedPrecisionIR
-    : public SolverFacade<MixedPrecisionIR<DIM, tolerance, maxOuter, T_Inner, firstInnerTolMicro>, DIM, double> {
+    : public SolverFacade<MixedPrecisionIR<DIM, tolerance, maxOuter, T_Inner, firstInnerTolMicro, adaptPercent>, DIM,
+                          double> {
 public:
     using SolverFacade<MixedPrecisionIR, DIM, double>::SolverFacade;
     static void stages(std::vector<pps_stage>& v)
     {
         pps_stage s = detail::stage(PPS_IR, 64, tolerance * tollScalingFactor, maxOuter);
         s.ir_first_tol = firstInnerTolMicro * 1e-6;
+        s.ir_adapt = adaptPercent * 0.01;
         v.push_back(s);
         T_Inner::stages(v);
     }

@@ -101,6 +101,11 @@ typedef struct pps_stage {
                             is replaced by it and the loop resumes (CG, FCG keep their search direction,
                             BiCGSTAB restarts with r0 = r), at most n times.  The recursive residual drifts from
                             the true one as iterations accumulate; this makes the true residual reach tol. */
+    double ir_adapt;     /* PPS_IR: 0 keeps the inner tolerance of every outer step.  a in (0, 1): an outer step
+                            asks the inner stage for no more reduction than the outer tolerance still needs,
+                            times a: max(inner tolerance, a * tol / e), e the outer residual before the step (the
+                            last step of a solve typically needs less than the inner tolerance).  No reference
+                            counterpart (the reference has no refinement) */
 } pps_stage;
 
 typedef struct pps_result {

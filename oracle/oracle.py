@@ -49,6 +49,7 @@ class Stage(ctypes.Structure):
         ("user", ctypes.c_void_p),
         ("ir_first_tol", ctypes.c_double),
         ("precision", ctypes.c_int),
+        ("ir_adapt", ctypes.c_double),
     ]
 
 
@@ -205,6 +206,7 @@ class OracleGrid:
             st[i].cheb_local = c.get("cheb_local", 0)
             st[i].ir_first_tol = c.get("first_tol", 0.0)
             st[i].precision = c.get("precision", 64)
+            st[i].ir_adapt = c.get("adapt", 0.0)
             if "user_fn" in c:
                 fn = USER_FN(c["user_fn"])
                 keep.append(fn)

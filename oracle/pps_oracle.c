@@ -856,7 +856,15 @@ static void og_ir(og_run* R, int s, double* x, double* b)
     while (outer < maxouter && err >= tol) {
         memset(SI->fin, 0, sizeof(float) * g->ntot);
         OG_FOR_SOLVER(g, c) SI->fin[c] = (float)S->r[c];
-        SI->tol_override = outer == 0 ? R->chain[s].ir_first_tol : 0.0;
+        {
+            /* the inner tolerance: the first step's own (ir_first_tol), else the
+             * inner stage's; with ir_adapt, no less than the reduction the outer
+             * tolerance still needs, times ir_adapt */
+            const double base = (outer == 0 && R->chain[s].ir_first_tol > 0.0) ? R->chain[s].ir_first_tol
+                                                                                : R->chain[s + 1].tol;
+            const double adapt = R->chain[s].ir_adapt;
+            SI->tol_override = (adapt > 0.0 && err > 0.0) ? fmax(base, adapt * tol / err) : base;
+        }
         og_cg_f(R, s + 1, SI->fout, SI->fin, R->chain[s + 1].kind == OG_FCG);
         inner_total += SI->iters;
         OG_FOR_SOLVER(g, c) x[c] += (double)SI->fout[c];

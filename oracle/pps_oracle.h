@@ -74,6 +74,8 @@ typedef struct og_stage {
     double ir_first_tol;                   /* OG_IR: inner tolerance of the first outer step (0: the inner's) */
     int precision;                         /* OG_CHEB under an FP64 caller: 32 runs the alpaka
                                               T_data_chebyshev = float form, else FP64          */
+    double ir_adapt;                       /* OG_IR: > 0: an outer step's inner tolerance is no less
+                                              than ir_adapt * tol / (outer residual)            */
 } og_stage;
 
 typedef struct og_result {

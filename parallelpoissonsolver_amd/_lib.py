@@ -55,6 +55,7 @@ class CStage(ctypes.Structure):
         ("stall_ratio", ctypes.c_double),
         ("ir_first_tol", ctypes.c_double),
         ("true_finish", ctypes.c_int),
+        ("ir_adapt", ctypes.c_double),
     ]
 
 
@@ -253,6 +254,7 @@ class Context:
             st[i].stall_ratio = c.get("stall_ratio", 0.0)
             st[i].ir_first_tol = c.get("first_tol", 0.0)
             st[i].true_finish = c.get("true_finish", 0)
+            st[i].ir_adapt = c.get("adapt", 0.0)
         return st
 
     def precondition(self, chain, out, inp):

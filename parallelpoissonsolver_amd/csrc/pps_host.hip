@@ -1135,7 +1135,7 @@ struct IRStage : KrylovStage {
         C.ghost(x, true, S, nullptr);
         const Geo g64 = C.gsolver(64);
         const double itol = pre->cfg.tol, itol0 = cfg.ir_first_tol > 0.0 ? cfg.ir_first_tol : itol;
-        launch_ir_residual(g64, C.stream, C.rb, x.d(), b.d(), r32.f(), S, SI, itol, itol0);
+        launch_ir_residual(g64, C.stream, C.rb, x.d(), b.d(), r32.f(), S, SI, itol, itol0, cfg.ir_adapt);
         Scal h;
         rc = scal_read(C, S, h);
         if (rc) return rc;
@@ -1146,7 +1146,7 @@ struct IRStage : KrylovStage {
             launch_ir_update(g64, C.stream, x.d(), d.f(), SI);
             C.ghost(x, true, S, nullptr);
             scal_op(C, S, SC_OUTER_INC);
-            launch_ir_residual(g64, C.stream, C.rb, x.d(), b.d(), r32.f(), S, SI, itol, itol0);
+            launch_ir_residual(g64, C.stream, C.rb, x.d(), b.d(), r32.f(), S, SI, itol, itol0, cfg.ir_adapt);
             outer_steps++;
             rc = scal_read(C, S, h);
             if (rc) return rc;

@@ -1955,6 +1955,7 @@ struct OpIrRes : Gate {
     Scal* So;
     Scal* SI;
     double itol, itol0;  // inner tolerance: first outer step, later ones
+    double adapt;        // > 0: no less than adapt * tol / ||r||
     __device__ void load() {}
     struct Ox { double b; };
     __device__ double in(long long c) const { return x[c]; }
@@ -1977,12 +1978,13 @@ struct OpIrRes : Gate {
         if (o == 0) So->v[S_ERR0] = e;
         So->done = e < So->v[S_TOL] ? 1 : 0;
         SI->v[S_INNER_NORM] = sqrt(t[1]);
-        SI->v[S_TOL] = o == 0 ? itol0 : itol;
+        const double base = o == 0 ? itol0 : itol;
+        SI->v[S_TOL] = (adapt > 0.0 && e > 0.0) ? fmax(base, adapt * So->v[S_TOL] / e) : base;
     }
 };
 
 void launch_ir_residual(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, float* r32, Scal* So,
-                        Scal* SI, double itol, double itol0)
+                        Scal* SI, double itol, double itol0, double adapt)
 {
     OpIrRes op;
     op.gate = nullptr;
@@ -1993,6 +1995,7 @@ void launch_ir_residual(const Geo& g, hipStream_t s, RedBuf rb, const double* x,
     op.SI = SI;
     op.itol = itol;
     op.itol0 = itol0;
+    op.adapt = adapt;
     run_stencil<double, 2>(g, s, op, rb);
 }
 

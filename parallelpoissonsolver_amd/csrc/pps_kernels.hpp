@@ -251,9 +251,10 @@ void launch_fcg_update(const Geo& g, hipStream_t s, RedBuf rb, T* r, const T* Ap
 // SI: S_INNER_NORM = ||r32||
 // r32 = b - A x (FP64, narrowed), the outer residual norm and convergence
 // test; sets the inner stage's tolerance to itol0 before the first inner
-// solve and to itol before the later ones
+// solve and to itol before the later ones, or (adapt > 0) to no less than
+// adapt * tol / ||r||, the reduction the outer tolerance still needs
 void launch_ir_residual(const Geo& g, hipStream_t s, RedBuf rb, const double* x, const double* b, float* r32, Scal* So,
-                        Scal* SI, double itol, double itol0);
+                        Scal* SI, double itol, double itol0, double adapt);
 // x += ||r32|| * d on the solver region (the inner solve's un-normalisation)
 void launch_ir_update(const Geo& g, hipStream_t s, double* x, const float* d, const Scal* SI);
 // inner start: r = r32 / ||r32||, x = 0, p = 0, err0, rz (pointwise jac) -> SI

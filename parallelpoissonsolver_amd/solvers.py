@@ -175,7 +175,8 @@ class SolverSpec:
         p = self.params
         st = {"kind": self.kind, "precision": p.get("precision", 64), "tol": p.get("tol", 1e-10),
               "maxit": p.get("maxit", 1)}
-        for k in ("cheb_eps", "cheb_rmin", "cheb_rmax", "cheb_local", "stall_window", "stall_ratio", "first_tol"):
+        for k in ("cheb_eps", "cheb_rmin", "cheb_rmax", "cheb_local", "stall_window", "stall_ratio", "first_tol",
+                  "adapt"):
             if k in p:
                 st[k] = p[k]
         return st
@@ -227,12 +228,15 @@ def FCG(DIM, T_data, tolerance, maxIteration, isMainLoop, communicationON, T_Pre
                                  "maxit": int(maxIteration), "isMainLoop": isMainLoop}, T_Preconditioner)
 
 
-def MixedPrecisionIR(tolerance, maxOuter, inner, first_inner_tol=0.0):
+def MixedPrecisionIR(tolerance, maxOuter, inner, first_inner_tol=0.0, adapt=0.0):
     """FP64 outer refinement r = b - A u, u += d around an FP32 inner solver
     spec (BaseCG or FCG with T_data float32).  first_inner_tol > 0: the inner
-    tolerance of the first outer step.  No reference counterpart."""
+    tolerance of the first outer step.  adapt in (0, 1): an outer step asks the
+    inner solver for no more reduction than the outer tolerance still needs,
+    times adapt (pps_stage::ir_adapt).  No reference counterpart."""
     return SolverSpec(_lib.IR, {"precision": 64, "tol": tolerance * tollScalingFactor, "maxit": int(maxOuter),
-                                "isMainLoop": True, "first_tol": float(first_inner_tol)}, inner)
+                                "isMainLoop": True, "first_tol": float(first_inner_tol), "adapt": float(adapt)},
+                      inner)
 
 
 def LearnedPreconditioner(T_data=np.float32, stall_window=0, stall_ratio=0.0):

@@ -378,6 +378,32 @@ def test_ir_first_step_tolerance_matches_oracle():
     assert res["error_compute_operator"] < 1e-10 and ro["error_compute_operator"] < 1e-10
 
 
+def test_ir_adaptive_inner_tolerance_matches_oracle():
+    """ir_adapt (an outer step asks the inner solve for no more than
+    adapt * tol / e) on the GPU follows the oracle: same outer step count,
+    inner counts within 2 %, the same residual before the last step to 1 %,
+    and the FP64 true residual at the outer tolerance."""
+    n = 34
+    G = O.OracleGrid((n,) * 3, (1 / (n - 1),) * 3, (0, 0, 0), (0,) * 6)
+    b = G.rhs("gauss", 3)
+    x = G.zeros()
+    C = ctx_for(G, (0,) * 6)
+    C.upload(L.FIELD_X, x)
+    C.upload(L.FIELD_B, b)
+    res = C.solve([dict(kind=L.IR, tol=1e-10, maxit=50, adapt=0.5),
+                   dict(kind=L.CG, precision=32, tol=1e-4, maxit=5000), dict(kind=L.NONE, precision=32)])
+    xo, bo = x.copy(), b.copy()
+    ro = G.solve([dict(kind=O.IR, tol=1e-10, maxit=50, adapt=0.5), dict(kind=O.CG, tol=1e-4, maxit=5000),
+                  dict(kind=O.NONE)], xo, bo)
+    print(f"[ir adapt] GPU {res['iterations']} inner ({res['outer_iterations']} outer), history {list(res['history'])}; "
+          f"oracle {ro['iterations']} ({ro['outer_iterations']}), {list(ro['history'])}", flush=True)
+    assert res["outer_iterations"] == ro["outer_iterations"]
+    assert abs(res["iterations"] - ro["iterations"]) <= max(2, round(0.02 * ro["iterations"]))
+    k = ro["outer_iterations"] - 1
+    assert abs(res["history"][k] - ro["history"][k]) <= 0.01 * ro["history"][k]
+    assert res["error_compute_operator"] < 1e-10 and ro["error_compute_operator"] < 1e-10
+
+
 ALP_CHEB = dict(cheb_eps=0.0, cheb_rmin=100.0, cheb_rmax=1 - 1e-4)  # solverSetup.hpp (alpaka):59-62
 
 
