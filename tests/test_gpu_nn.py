@@ -54,7 +54,7 @@ def test_forward_matches_oracle(net, shape):
         assert np.max(np.abs(a - acts[i])) <= 0.02 * np.max(np.abs(acts[i])) + 1e-6, i
 
 
-@pytest.mark.parametrize("dims", [(24, 24, 24), (31, 45, 20)])
+@pytest.mark.parametrize("dims", [(24, 24, 24), (31, 45, 20), (33, 17, 21)])
 def test_inference_path_matches_training_forward(net, dims):
     """The solver's path (pooled-only lift, output layer with the lift fused in,
     strided FP64 field, input scaled to unit RMS) against the dense batch
@@ -69,7 +69,8 @@ def test_inference_path_matches_training_forward(net, dims):
     FP32 summation order.  Against the whole oracle forward, whose activations
     differ from the GPU's by one-ulp BF16 flips that can cascade through the
     ReLUs of the coarse levels, the forward test's 1e-2 applies.
-    (31, 45, 20) has partial tiles in x and y and odd coarse extents."""
+    (31, 45, 20) has partial tiles in x and y and odd coarse extents; (33, 17, 21)
+    an odd depth, so z-chunks of odd length and a last plane without a pair."""
     nx, ny, nz = dims
     print(f"[nn] fused inference vs dense forward {dims}", flush=True)
     p = _params(31)
@@ -146,16 +147,17 @@ def test_fp32_fields_and_fused_gamma(net, dims):
     C.close()
 
 
+@pytest.mark.parametrize("dims", [(31, 45, 20), (33, 17, 21)])
 @pytest.mark.parametrize("prec", [64, 32])
-def test_fused_residual_update(net, prec):
+def test_fused_residual_update(net, prec, dims):
     """FCG's residual update folded into the network's first layer:
     r_cur = r_prev - alpha * Ap_prev is written by the lift (on the solver
     region only) and the output equals the plain apply on that r_cur, bit for
     bit (both stage the same values; the scale comes from the same ||r_cur||^2
     up to summation order, which moves it by ~1e-16).  r_cur against numpy in
     the field's precision: 1 ulp (the kernel may fuse the multiply-add)."""
-    nx, ny, nz = 31, 45, 20
-    print(f"[nn] fused residual update, prec {prec}", flush=True)
+    nx, ny, nz = dims
+    print(f"[nn] fused residual update, prec {prec}, {dims}", flush=True)
     net.set_params(_params(35))
     C = L.Context((nx + 2, ny + 2, nz + 2), (1.0 / (nx + 1), 1.0 / (ny + 1), 1.0 / (nz + 1)))
     net.attach(C)
