@@ -199,7 +199,8 @@ enum {
     PPS_K_CG_STENCIL_F32 = 5,
     PPS_K_CG_UPDATE_F32 = 6,
     PPS_K_FCG_STENCIL = 7,     /* FCG p-stencil as the learned FCG runs it: p = z - beta p_old,
-                                  Ap, p.Ap, p.r, r.Ap, Ap.Ap, r.r, x += alpha_prev p_old */
+                                  Ap, p.Ap, p.r, r.Ap, Ap.Ap, r.r (x's updates are deferred to a
+                                  pass per 16 iterations) */
     PPS_K_FCG_STENCIL_F32 = 8,
     PPS_K_COPY_FLAT = 9,       /* plain 1-D copy of a whole field allocation (16-byte
                                   non-temporal lanes): the measured HBM ceiling */

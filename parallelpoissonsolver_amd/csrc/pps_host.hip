@@ -813,6 +813,19 @@ struct FCGStage : KrylovStage {
         const bool fused_r = pre && pre->forms_update();
         Field rr[2] = {r, r};
         if (fused_r && (rc = C.get_work(k + ".r2", 64, rr[1]))) return rc;
+        // and its directions go round a ring, reaching x in one pass per
+        // kFcgRing iterations (iteration q's in slot q mod kFcgRing)
+        Field ring[kFcgRing];
+        double* ringp[kFcgRing] = {};
+        if (fused_r) {
+            ring[0] = p0;
+            ring[1] = p1;
+            for (int j = 2; j < kFcgRing && !rc; ++j) rc = C.get_work(k + ".p" + std::to_string(j), 64, ring[j]);
+            if (rc) return rc;
+            for (int j = 0; j < kFcgRing; ++j) ringp[j] = ring[j].d();
+            launch_fcg_ring_reset(C.stream, S);
+        }
+        int q = 0;
         int cur = 0;  // p[cur] = p_new, p[cur^1] = p_old
         auto body = [&]() -> int {
             if (fused_r) {
@@ -823,9 +836,12 @@ struct FCGStage : KrylovStage {
                 if (e) return e;
                 if (nb > 0) launch_fcg_gamma_finish<double>(C.stream, C.rb, nb, S);
                 launch_fcg_gamma<double>(g, C.stream, C.rb, z.d(), Ap[cur ^ 1].d(), S, nb > 0);
-                launch_fcg_p_fused<double>(g, C.stream, C.rb, z.d(), p[cur ^ 1].d(), p[cur].d(), Ap[cur].d(),
-                                           rr[cur ^ 1].d(), x.d(), S, main);
+                const int qs = q % kFcgRing, qo = (q + kFcgRing - 1) % kFcgRing;
+                launch_fcg_p_fused<double>(g, C.stream, C.rb, z.d(), ring[qo].d(), ring[qs].d(), Ap[cur].d(),
+                                           rr[cur ^ 1].d(), x.d(), S, main, qs);
+                if (qs == kFcgRing - 1) launch_fcg_flush<double>(g, C.stream, x.d(), ringp, S);
                 cur ^= 1;
+                ++q;
                 return PPS_OK;
             }
             // the learned preconditioner sums z . Ap_old in its output kernel
@@ -846,7 +862,9 @@ struct FCGStage : KrylovStage {
             rc = scal_read(C, S, h);
             if (rc) return rc;
             // iteration k wrote p[k & 1]; its x update waits for the next stencil pass
-            if (h.iter > 0) launch_fcg_final_x<double>(g, C.stream, x.d(), p[(h.iter - 1) & 1].d(), S);
+            // (or, with the ring, for its flush)
+            if (fused_r) launch_fcg_flush<double>(g, C.stream, x.d(), ringp, S);
+            else if (h.iter > 0) launch_fcg_final_x<double>(g, C.stream, x.d(), p[(h.iter - 1) & 1].d(), S);
             if (!main || repl >= cfg.true_finish) break;
             // true-residual finish: r = b - A x of the complete x, written where
             // iteration K reads its residual (rr[K & 1] when the preconditioner
@@ -855,6 +873,7 @@ struct FCGStage : KrylovStage {
             if (rc) return rc;
             if (h.done) break;
             cur = h.iter & 1;
+            q = h.iter;  // queued iterations past the stop advanced q without running
         }
         C.fallback_switches += h.fallback;
         count_pre(h.iter);  // one application per iteration that ran
@@ -1031,7 +1050,21 @@ struct IRStage : KrylovStage {
         // written SI[S_INNER_NORM], and ir_inner_init resets iter/done
         const Geo g = C.gsolver(32);
         Field p[2] = {p0, p1}, Ap[2] = {A0, A1};
+        // with the learned preconditioner the directions go round a ring and
+        // reach d in one pass per kFcgRing iterations (iteration q's in slot
+        // q mod kFcgRing): the p-stencil pass neither reads nor writes d
+        Field ring[kFcgRing];
+        float* ringp[kFcgRing] = {};
+        if (fused_r) {
+            ring[0] = p0;
+            ring[1] = p1;
+            for (int j = 2; j < kFcgRing && !rc; ++j) rc = C.get_work(k + ".p" + std::to_string(j), 32, ring[j]);
+            if (rc) return rc;
+            for (int j = 0; j < kFcgRing; ++j) ringp[j] = ring[j].f();
+        }
         launch_ir_inner_init(g, C.stream, C.rb, r32.f(), d.f(), p0.f(), SI, pw);
+        if (fused_r) launch_fcg_ring_reset(C.stream, SI);
+        int q = 0;
         // pin p1 and the old Ap to zero as well: iteration 0 multiplies them by beta = 0
         launch_fill<float>(g, C.stream, p1.f(), 0.0f);
         launch_fill<float>(g, C.stream, A0.f(), 0.0f);
@@ -1063,9 +1096,12 @@ struct IRStage : KrylovStage {
                 if (e) return e;
                 if (nb > 0) launch_fcg_gamma_finish<float>(C.stream, C.rb, nb, SI);
                 launch_fcg_gamma<float>(g, C.stream, C.rb, z.f(), Ap[cur ^ 1].f(), SI, nb > 0);
-                launch_fcg_p_fused<float>(g, C.stream, C.rb, z.f(), p[cur ^ 1].f(), p[cur].f(), Ap[cur].f(),
-                                          rr[cur ^ 1].f(), d.f(), SI, false);
+                const int qs = q % kFcgRing, qo = (q + kFcgRing - 1) % kFcgRing;
+                launch_fcg_p_fused<float>(g, C.stream, C.rb, z.f(), ring[qo].f(), ring[qs].f(), Ap[cur].f(),
+                                          rr[cur ^ 1].f(), d.f(), SI, false, qs);
+                if (qs == kFcgRing - 1) launch_fcg_flush<float>(g, C.stream, d.f(), ringp, SI);
                 cur ^= 1;
+                ++q;
             } else if (flexible) {
                 int nb = 0;
                 int e = applyM_dot(&SI->done, Ap[cur ^ 1], &nb);
@@ -1093,7 +1129,8 @@ struct IRStage : KrylovStage {
         rc = scal_read(C, SI, h);
         if (rc) return rc;
         if (!flexible && h.iter > 0) launch_cg_final_x<float>(g, C.stream, d.f(), p[h.iter & 1].f(), SI);
-        if (flexible && h.iter > 0) launch_fcg_final_x<float>(g, C.stream, d.f(), p[(h.iter - 1) & 1].f(), SI);
+        if (fused_r) launch_fcg_flush<float>(g, C.stream, d.f(), ringp, SI);  // the ring's last steps
+        else if (flexible && h.iter > 0) launch_fcg_final_x<float>(g, C.stream, d.f(), p[(h.iter - 1) & 1].f(), SI);
         inner_total += h.iter;
         if (flexible) C.fallback_switches += h.fallback;
         in->prec_iters += h.iter;
@@ -1889,11 +1926,11 @@ int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes)
         case PPS_K_CHEB_SWEEP: launch_cheb_sweep<double, double>(g, C->stream, a.d(), b.d(), c.d(), e.d(), 0.1, 0.2, 0.3, 1.0, nullptr); break;
         case PPS_K_FCG_STENCIL:
             launch_fcg_p_fused<double>(g, C->stream, C->rb, a.d(), (it & 1) ? c.d() : b.d(), (it & 1) ? b.d() : c.d(),
-                                       d.d(), f.d(), e.d(), S, false);
+                                       d.d(), f.d(), e.d(), S, false, 0);
             break;
         case PPS_K_FCG_STENCIL_F32:
             launch_fcg_p_fused<float>(g, C->stream, C->rb, a.f(), (it & 1) ? c.f() : b.f(), (it & 1) ? b.f() : c.f(),
-                                      d.f(), f.f(), e.f(), S, false);
+                                      d.f(), f.f(), e.f(), S, false, 0);
             break;
         case PPS_K_COPY_FLAT: launch_copy_flat(C->stream, a.d(), e.d(), C->L64.total); break;
         case PPS_K_CHEB_PAIR: {
@@ -1912,7 +1949,7 @@ int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes)
     case PPS_K_RESIDUAL: per = 3; break;                                // read x, b ; write r
     case PPS_K_COPY: per = 2; break;
     case PPS_K_CHEB_SWEEP: per = 4; break;                              // read Y, B, Z ; write W
-    case PPS_K_FCG_STENCIL: case PPS_K_FCG_STENCIL_F32: per = 7; break; // read z, p_old, r, x ; write p, Ap, x
+    case PPS_K_FCG_STENCIL: case PPS_K_FCG_STENCIL_F32: per = 5; break; // read z, p_old, r ; write p, Ap
     case PPS_K_COPY_FLAT: per = 2.0 * (double)C->L64.total / n; break;  // the whole allocation, read and written
     case PPS_K_CHEB_PAIR: per = 5; break;                               // read Y, Z, B ; write W1, W2 (two sweeps)
     default: return C->fail(PPS_ERR_ARG, "unknown kernel id %d", kernel);

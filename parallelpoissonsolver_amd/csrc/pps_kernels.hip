@@ -1786,11 +1786,12 @@ struct OpFcgP : Gate {
     bool history;  // FUSED: record the residual history
     T beta, alpha_prev;
     bool upd;  // apply the previous step to x
+    int slot;  // >= 0: deferred x updates (x untouched; alpha to S->ahist[slot])
     __device__ void load()
     {
         beta = (T)S->v[S_BETA];
         alpha_prev = (T)S->v[S_ALPHA];
-        upd = !S->restart && alpha_prev != (T)0;
+        upd = slot < 0 && !S->restart && alpha_prev != (T)0;
     }
     // beta = 0 takes z as it is, and pold is read only when it is used, so a
     // restart never multiplies a non-finite pold
@@ -1834,6 +1835,7 @@ struct OpFcgP : Gate {
             // a zero scale
             if constexpr (FUSED) rr_next = fmax(t[4] - 2.0 * a * t[2] + a * a * t[3], 1e-8 * t[4]);
         }
+        if (slot >= 0) S->ahist[slot] = S->v[S_ALPHA];
         if constexpr (FUSED) update_finish(S, rr_next, history);
     }
 };
@@ -1852,12 +1854,13 @@ void launch_fcg_p(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* p
     op.x = x;
     op.S = S;
     op.history = false;
+    op.slot = -1;
     run_stencil<T, 2>(g, s, op, rb);
 }
 
 template <typename T>
 void launch_fcg_p_fused(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* pold, T* p, T* Ap, const T* r,
-                        T* x, Scal* S, bool history)
+                        T* x, Scal* S, bool history, int ring_slot)
 {
     OpFcgP<T, true> op;
     op.gate = &S->done;
@@ -1869,7 +1872,69 @@ void launch_fcg_p_fused(const Geo& g, hipStream_t s, RedBuf rb, const T* z, cons
     op.x = x;
     op.S = S;
     op.history = history;
+    op.slot = ring_slot;
     run_stencil<T, 5>(g, s, op, rb);
+}
+
+// deferred x updates: x += ahist[j] * ring[j] over the slots in order, the
+// slots with alpha 0 (unused, or a discarded step whose direction may hold
+// non-finite values) neither read nor added
+template <typename T>
+struct OpFcgFlush : Gate {
+    T* __restrict__ x;
+    const T* ring[kFcgRing];
+    const Scal* S;
+    T a[kFcgRing];
+    unsigned used;
+    struct R { T x, p[kFcgRing]; };
+    __device__ void load()
+    {
+        used = 0u;
+#pragma unroll
+        for (int j = 0; j < kFcgRing; ++j) {
+            a[j] = (T)S->ahist[j];
+            if (S->ahist[j] != 0.0) used |= 1u << j;
+        }
+    }
+    template <bool NT> __device__ R fetch(long long c) const
+    {
+        R v;
+        v.x = ld<NT>(x + c);
+#pragma unroll
+        for (int j = 0; j < kFcgRing; ++j) v.p[j] = ((used >> j) & 1u) ? ld<NT>(ring[j] + c) : (T)0;
+        return v;
+    }
+    template <bool NT> __device__ void apply(long long c, const R& v, int, int, int, double*) const
+    {
+        T acc = v.x;
+#pragma unroll
+        for (int j = 0; j < kFcgRing; ++j)
+            if ((used >> j) & 1u) acc = acc + a[j] * v.p[j];
+        put<NT>(x + c, acc);
+    }
+    __device__ void finish(const double*) {}
+};
+
+__global__ void k_fcg_ring_reset(Scal* S)
+{
+    if (threadIdx.x < kFcgRing) S->ahist[threadIdx.x] = 0.0;
+}
+
+void launch_fcg_ring_reset(hipStream_t s, Scal* S)
+{
+    hipLaunchKernelGGL(k_fcg_ring_reset, dim3(1), dim3(64), 0, s, S);
+}
+
+template <typename T>
+void launch_fcg_flush(const Geo& g, hipStream_t s, T* x, T* const ring[kFcgRing], Scal* S)
+{
+    OpFcgFlush<T> op;
+    op.gate = nullptr;
+    op.x = x;
+    for (int j = 0; j < kFcgRing; ++j) op.ring[j] = ring[j];
+    op.S = S;
+    run_vector<0>(g, s, op, kNoRed);
+    launch_fcg_ring_reset(s, S);
 }
 
 // the last step of an FCG solve: x += alpha p unless it was discarded
@@ -2101,8 +2166,9 @@ void launch_ir_inner_init(const Geo& g, hipStream_t s, RedBuf rb, float* r, floa
     template void launch_fcg_gamma_finish<T>(hipStream_t, RedBuf, int, Scal*);                                  \
     template void launch_fcg_p<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, T*, T*, const T*, T*, Scal*); \
     template void launch_fcg_p_fused<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, T*, T*, const T*, T*,    \
-                                        Scal*, bool);                                                              \
+                                        Scal*, bool, int);                                                         \
     template void launch_fcg_final_x<T>(const Geo&, hipStream_t, T*, const T*, const Scal*);                      \
+    template void launch_fcg_flush<T>(const Geo&, hipStream_t, T*, T* const[kFcgRing], Scal*);                    \
     template void launch_fcg_update<T>(const Geo&, hipStream_t, RedBuf, T*, const T*, Scal*, bool);
 
 PPS_INST_T(double)

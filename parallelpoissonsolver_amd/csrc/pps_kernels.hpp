@@ -58,8 +58,13 @@ enum ScalIdx {
     S_COUNT
 };
 
+// FCG with deferred x updates: the steps of a ring of kFcgRing directions are
+// added to x in one pass (launch_fcg_flush)
+constexpr int kFcgRing = 16;
+
 struct alignas(256) Scal {
     double v[S_COUNT];
+    double ahist[kFcgRing];  // FCG ring: alpha of the step whose direction is in slot j (0: none, or discarded)
     int iter;        // completed iterations
     int done;        // 0 running, 1 converged or capped, 2 early exit before the loop
     int maxit;
@@ -238,11 +243,21 @@ void launch_fcg_p(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* p
 // launch_fcg_p when the preconditioner applies the residual update itself
 // (r of the next iteration = r - alpha Ap, formed while it reads r): also ends
 // the iteration (||r||^2 of the updated residual, convergence, stall monitor)
+// ring_slot >= 0: x is not touched; the step's alpha goes to
+// S->ahist[ring_slot] (0 when the step is discarded) for launch_fcg_flush
 template <typename T>
 void launch_fcg_p_fused(const Geo& g, hipStream_t s, RedBuf rb, const T* z, const T* pold, T* p, T* Ap, const T* r,
-                        T* x, Scal* S, bool history);
+                        T* x, Scal* S, bool history, int ring_slot = -1);
 template <typename T>
 void launch_fcg_final_x(const Geo& g, hipStream_t s, T* x, const T* p, const Scal* S);
+// x += sum over the ring's slots j = 0 .. kFcgRing-1 (in that order) of
+// S->ahist[j] * ring[j], skipping the slots whose alpha is 0; then clears the
+// alphas.  Adding the steps in the order they were taken gives x bit for bit
+// as the per-iteration updates do.
+template <typename T>
+void launch_fcg_flush(const Geo& g, hipStream_t s, T* x, T* const ring[kFcgRing], Scal* S);
+// clears S->ahist (a solve's start)
+void launch_fcg_ring_reset(hipStream_t s, Scal* S);
 template <typename T>
 void launch_fcg_update(const Geo& g, hipStream_t s, RedBuf rb, T* r, const T* Ap, Scal* S, bool history);
 
