@@ -191,8 +191,10 @@ int pps_rhs_spectrum(void* ctx, uint64_t seed, int kmax, int accumulate);
 /* ---- measurement ------------------------------------------------------- */
 /* kernel ids for pps_time_kernel */
 enum {
-    PPS_K_CG_STENCIL = 0,   /* fused p = z + beta p, Ap = A p, p.Ap       */
-    PPS_K_CG_UPDATE = 1,    /* x += alpha p, r -= alpha Ap, r.r, r.z      */
+    PPS_K_CG_STENCIL = 0,   /* fused p = z + beta p, Ap = A p, p.Ap (x's updates deferred to a
+                               pass per 16 iterations, as CG runs it where the device holds
+                               the directions' ring) */
+    PPS_K_CG_UPDATE = 1,    /* r -= alpha Ap, r.r, r.z                    */
     PPS_K_RESIDUAL = 2,     /* r = b - A x, r.r                           */
     PPS_K_COPY = 3,         /* plain copy of the solver region (reference) */
     PPS_K_CHEB_SWEEP = 4,   /* Chebyshev sweep                            */

@@ -182,6 +182,30 @@ struct Ctx {
         out = f;
         return PPS_OK;
     }
+    // the ring of search directions of a Krylov stage with deferred x updates
+    // (`key`, slot 0 = first, slot kFcgRing-1 = last, the others work fields):
+    // kFcgRing slots, or 0 when the device cannot hold the other kFcgRing-2
+    // fields with 8 more to spare (the stage then updates x every iteration)
+    int ring(const std::string& key, int prec, Field first, Field last, Field* out, int* slots)
+    {
+        *slots = 0;
+        bool have = true;
+        for (int j = 1; j + 1 < kFcgRing; ++j)
+            have = have && work.count(key + ".ring" + std::to_string(j) + (prec == 32 ? "/32" : "/64")) > 0;
+        if (!have) {
+            size_t fr = 0, tot = 0;
+            const size_t bytes = (size_t)lay(prec).total * (prec == 32 ? 4 : 8);
+            if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr < (size_t)(kFcgRing - 2 + 8) * bytes) return PPS_OK;
+        }
+        out[0] = first;
+        out[kFcgRing - 1] = last;
+        for (int j = 1; j + 1 < kFcgRing; ++j) {
+            int rc = get_work(key + ".ring" + std::to_string(j), prec, out[j]);
+            if (rc) return rc;
+        }
+        *slots = kFcgRing;
+        return PPS_OK;
+    }
     void zero(Field f)
     {
         if (f.prec == 32) launch_fill<float>(gbox(32), stream, f.f(), 0.0f);
@@ -624,11 +648,33 @@ struct CGStage : KrylovStage {
             launch_cg_rz<double>(g, C.stream, C.rb, r.d(), z.d(), S);
         }
         Field p[2] = {p0, p1};
+        // where the device holds a ring of directions, x's updates are deferred
+        // to one pass per nring iterations (iteration q's direction in slot q
+        // mod nring; slot 0 = p1 and the last slot = p0 are zero, so the first
+        // iteration reads zeros as its old direction)
+        Field ring[kFcgRing];
+        double* ringp[kFcgRing] = {};
+        int nring = 0;
+        rc = C.ring(k, 64, p1, p0, ring, &nring);
+        if (rc) return rc;
+        for (int j = 0; j < nring; ++j) ringp[j] = ring[j].d();
+        if (nring) launch_fcg_ring_reset(C.stream, S);
+        int q = 0;
         int cur = 0;
         auto body = [&]() -> int {
             // iteration k reads p[k&1], writes p[(k+1)&1], and applies
-            // iteration k-1's x update on the way
-            launch_cg_A2<double>(g, C.stream, C.rb, pw ? r.d() : z.d(), p[cur].d(), p[cur ^ 1].d(), Ap.d(), x.d(), S, pw);
+            // iteration k-1's x update on the way (with the ring: reads slot
+            // k-1, writes slot k, and leaves x to the flush)
+            if (nring) {
+                const int qs = q % nring, qo = (q + nring - 1) % nring;
+                launch_cg_A2<double>(g, C.stream, C.rb, pw ? r.d() : z.d(), ring[qo].d(), ring[qs].d(), Ap.d(), x.d(),
+                                     S, pw, qs);
+                if (qs == nring - 1) launch_fcg_flush<double>(g, C.stream, x.d(), ringp, S);
+                ++q;
+            } else {
+                launch_cg_A2<double>(g, C.stream, C.rb, pw ? r.d() : z.d(), p[cur].d(), p[cur ^ 1].d(), Ap.d(), x.d(), S,
+                                     pw);
+            }
             cur ^= 1;
             launch_cg_B<double>(g, C.stream, C.rb, r.d(), Ap.d(), S, pw, main);
             if (!pw) {
@@ -651,7 +697,8 @@ struct CGStage : KrylovStage {
             if (rc) return rc;
             if (!pw) pre_calls += std::max(0, h.iter - it0 - 1);
             it0 = h.iter;
-            if (h.iter > 0) launch_cg_final_x<double>(g, C.stream, x.d(), p[h.iter & 1].d(), S);
+            if (nring) launch_fcg_flush<double>(g, C.stream, x.d(), ringp, S);
+            else if (h.iter > 0) launch_cg_final_x<double>(g, C.stream, x.d(), p[h.iter & 1].d(), S);
             if (!main || repl >= cfg.true_finish) break;
             // true-residual finish: r = b - A x of the complete x replaces the
             // recursive residual; the search direction p[K&1] and beta stay
@@ -665,6 +712,7 @@ struct CGStage : KrylovStage {
                 ++pre_calls;
             }
             cur = h.iter & 1;
+            q = h.iter;  // queued iterations past the stop advanced q without running
         }
         count_pre(pre_calls);
         epilogue(main, x, b, r);
@@ -813,17 +861,17 @@ struct FCGStage : KrylovStage {
         const bool fused_r = pre && pre->forms_update();
         Field rr[2] = {r, r};
         if (fused_r && (rc = C.get_work(k + ".r2", 64, rr[1]))) return rc;
-        // and its directions go round a ring, reaching x in one pass per
-        // kFcgRing iterations (iteration q's in slot q mod kFcgRing)
+        // and its directions go round a ring where the device holds one,
+        // reaching x in one pass per nring iterations (iteration q's in slot
+        // q mod nring)
         Field ring[kFcgRing];
         double* ringp[kFcgRing] = {};
+        int nring = 0;
         if (fused_r) {
-            ring[0] = p0;
-            ring[1] = p1;
-            for (int j = 2; j < kFcgRing && !rc; ++j) rc = C.get_work(k + ".p" + std::to_string(j), 64, ring[j]);
+            rc = C.ring(k, 64, p0, p1, ring, &nring);
             if (rc) return rc;
-            for (int j = 0; j < kFcgRing; ++j) ringp[j] = ring[j].d();
-            launch_fcg_ring_reset(C.stream, S);
+            for (int j = 0; j < nring; ++j) ringp[j] = ring[j].d();
+            if (nring) launch_fcg_ring_reset(C.stream, S);
         }
         int q = 0;
         int cur = 0;  // p[cur] = p_new, p[cur^1] = p_old
@@ -836,12 +884,17 @@ struct FCGStage : KrylovStage {
                 if (e) return e;
                 if (nb > 0) launch_fcg_gamma_finish<double>(C.stream, C.rb, nb, S);
                 launch_fcg_gamma<double>(g, C.stream, C.rb, z.d(), Ap[cur ^ 1].d(), S, nb > 0);
-                const int qs = q % kFcgRing, qo = (q + kFcgRing - 1) % kFcgRing;
-                launch_fcg_p_fused<double>(g, C.stream, C.rb, z.d(), ring[qo].d(), ring[qs].d(), Ap[cur].d(),
-                                           rr[cur ^ 1].d(), x.d(), S, main, qs);
-                if (qs == kFcgRing - 1) launch_fcg_flush<double>(g, C.stream, x.d(), ringp, S);
+                if (nring) {
+                    const int qs = q % nring, qo = (q + nring - 1) % nring;
+                    launch_fcg_p_fused<double>(g, C.stream, C.rb, z.d(), ring[qo].d(), ring[qs].d(), Ap[cur].d(),
+                                               rr[cur ^ 1].d(), x.d(), S, main, qs);
+                    if (qs == nring - 1) launch_fcg_flush<double>(g, C.stream, x.d(), ringp, S);
+                    ++q;
+                } else {
+                    launch_fcg_p_fused<double>(g, C.stream, C.rb, z.d(), p[cur ^ 1].d(), p[cur].d(), Ap[cur].d(),
+                                               rr[cur ^ 1].d(), x.d(), S, main);
+                }
                 cur ^= 1;
-                ++q;
                 return PPS_OK;
             }
             // the learned preconditioner sums z . Ap_old in its output kernel
@@ -863,7 +916,7 @@ struct FCGStage : KrylovStage {
             if (rc) return rc;
             // iteration k wrote p[k & 1]; its x update waits for the next stencil pass
             // (or, with the ring, for its flush)
-            if (fused_r) launch_fcg_flush<double>(g, C.stream, x.d(), ringp, S);
+            if (nring) launch_fcg_flush<double>(g, C.stream, x.d(), ringp, S);
             else if (h.iter > 0) launch_fcg_final_x<double>(g, C.stream, x.d(), p[(h.iter - 1) & 1].d(), S);
             if (!main || repl >= cfg.true_finish) break;
             // true-residual finish: r = b - A x of the complete x, written where
@@ -1053,17 +1106,19 @@ struct IRStage : KrylovStage {
         // with the learned preconditioner the directions go round a ring and
         // reach d in one pass per kFcgRing iterations (iteration q's in slot
         // q mod kFcgRing): the p-stencil pass neither reads nor writes d
+        // (as the plain inner CG's, where the device holds the ring; slot 0 =
+        // p1 and the last slot = p0, both zero, so CG's first iteration reads
+        // zeros as its old direction)
         Field ring[kFcgRing];
         float* ringp[kFcgRing] = {};
-        if (fused_r) {
-            ring[0] = p0;
-            ring[1] = p1;
-            for (int j = 2; j < kFcgRing && !rc; ++j) rc = C.get_work(k + ".p" + std::to_string(j), 32, ring[j]);
+        int nring = 0;
+        if (fused_r || !flexible) {
+            rc = C.ring(k, 32, p1, p0, ring, &nring);
             if (rc) return rc;
-            for (int j = 0; j < kFcgRing; ++j) ringp[j] = ring[j].f();
+            for (int j = 0; j < nring; ++j) ringp[j] = ring[j].f();
         }
         launch_ir_inner_init(g, C.stream, C.rb, r32.f(), d.f(), p0.f(), SI, pw);
-        if (fused_r) launch_fcg_ring_reset(C.stream, SI);
+        if (nring) launch_fcg_ring_reset(C.stream, SI);
         int q = 0;
         // pin p1 and the old Ap to zero as well: iteration 0 multiplies them by beta = 0
         launch_fill<float>(g, C.stream, p1.f(), 0.0f);
@@ -1096,12 +1151,17 @@ struct IRStage : KrylovStage {
                 if (e) return e;
                 if (nb > 0) launch_fcg_gamma_finish<float>(C.stream, C.rb, nb, SI);
                 launch_fcg_gamma<float>(g, C.stream, C.rb, z.f(), Ap[cur ^ 1].f(), SI, nb > 0);
-                const int qs = q % kFcgRing, qo = (q + kFcgRing - 1) % kFcgRing;
-                launch_fcg_p_fused<float>(g, C.stream, C.rb, z.f(), ring[qo].f(), ring[qs].f(), Ap[cur].f(),
-                                          rr[cur ^ 1].f(), d.f(), SI, false, qs);
-                if (qs == kFcgRing - 1) launch_fcg_flush<float>(g, C.stream, d.f(), ringp, SI);
+                if (nring) {
+                    const int qs = q % nring, qo = (q + nring - 1) % nring;
+                    launch_fcg_p_fused<float>(g, C.stream, C.rb, z.f(), ring[qo].f(), ring[qs].f(), Ap[cur].f(),
+                                              rr[cur ^ 1].f(), d.f(), SI, false, qs);
+                    if (qs == nring - 1) launch_fcg_flush<float>(g, C.stream, d.f(), ringp, SI);
+                    ++q;
+                } else {
+                    launch_fcg_p_fused<float>(g, C.stream, C.rb, z.f(), p[cur ^ 1].f(), p[cur].f(), Ap[cur].f(),
+                                              rr[cur ^ 1].f(), d.f(), SI, false);
+                }
                 cur ^= 1;
-                ++q;
             } else if (flexible) {
                 int nb = 0;
                 int e = applyM_dot(&SI->done, Ap[cur ^ 1], &nb);
@@ -1112,8 +1172,16 @@ struct IRStage : KrylovStage {
                 launch_fcg_update<float>(g, C.stream, C.rb, r32.f(), Ap[cur].f(), SI, false);
                 cur ^= 1;
             } else {
-                launch_cg_A2<float>(g, C.stream, C.rb, pw ? r32.f() : z.f(), p[cur].f(), p[cur ^ 1].f(), Ap[0].f(), d.f(),
-                                    SI, pw);
+                if (nring) {
+                    const int qs = q % nring, qo = (q + nring - 1) % nring;
+                    launch_cg_A2<float>(g, C.stream, C.rb, pw ? r32.f() : z.f(), ring[qo].f(), ring[qs].f(), Ap[0].f(),
+                                        d.f(), SI, pw, qs);
+                    if (qs == nring - 1) launch_fcg_flush<float>(g, C.stream, d.f(), ringp, SI);
+                    ++q;
+                } else {
+                    launch_cg_A2<float>(g, C.stream, C.rb, pw ? r32.f() : z.f(), p[cur].f(), p[cur ^ 1].f(), Ap[0].f(),
+                                        d.f(), SI, pw);
+                }
                 cur ^= 1;
                 launch_cg_B<float>(g, C.stream, C.rb, r32.f(), Ap[0].f(), SI, pw, false);
                 if (!pw) {
@@ -1128,8 +1196,8 @@ struct IRStage : KrylovStage {
         Scal h;
         rc = scal_read(C, SI, h);
         if (rc) return rc;
-        if (!flexible && h.iter > 0) launch_cg_final_x<float>(g, C.stream, d.f(), p[h.iter & 1].f(), SI);
-        if (fused_r) launch_fcg_flush<float>(g, C.stream, d.f(), ringp, SI);  // the ring's last steps
+        if (nring) launch_fcg_flush<float>(g, C.stream, d.f(), ringp, SI);  // the ring's last steps
+        else if (!flexible && h.iter > 0) launch_cg_final_x<float>(g, C.stream, d.f(), p[h.iter & 1].f(), SI);
         else if (flexible && h.iter > 0) launch_fcg_final_x<float>(g, C.stream, d.f(), p[(h.iter - 1) & 1].f(), SI);
         inner_total += h.iter;
         if (flexible) C.fallback_switches += h.fallback;
@@ -1913,11 +1981,11 @@ int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes)
         switch (kernel) {
         case PPS_K_CG_STENCIL:
             launch_cg_A2<double>(g, C->stream, C->rb, a.d(), (it & 1) ? c.d() : b.d(), (it & 1) ? b.d() : c.d(), d.d(),
-                                 e.d(), S, true);
+                                 e.d(), S, true, 0);
             break;
         case PPS_K_CG_STENCIL_F32:
             launch_cg_A2<float>(g, C->stream, C->rb, a.f(), (it & 1) ? c.f() : b.f(), (it & 1) ? b.f() : c.f(), d.f(),
-                                e.f(), S, true);
+                                e.f(), S, true, 0);
             break;
         case PPS_K_CG_UPDATE: launch_cg_B<double>(g, C->stream, C->rb, a.d(), c.d(), S, true, false); break;
         case PPS_K_CG_UPDATE_F32: launch_cg_B<float>(g, C->stream, C->rb, a.f(), c.f(), S, true, false); break;
@@ -1944,7 +2012,7 @@ int pps_time_kernel(void* ctx, int kernel, int reps, double* ms, double* bytes)
     };
     double per = 0;
     switch (kernel) {
-    case PPS_K_CG_STENCIL: case PPS_K_CG_STENCIL_F32: per = 6; break;   // read r, p, x ; write p, Ap, x
+    case PPS_K_CG_STENCIL: case PPS_K_CG_STENCIL_F32: per = 4; break;   // read r, p ; write p, Ap
     case PPS_K_CG_UPDATE: case PPS_K_CG_UPDATE_F32: per = 3; break;     // read r, Ap ; write r
     case PPS_K_RESIDUAL: per = 3; break;                                // read x, b ; write r
     case PPS_K_COPY: per = 2; break;

@@ -1013,6 +1013,7 @@ struct OpCgA : Gate {
     T* __restrict__ x;
     Scal* S;
     bool pointwise;
+    int slot;  // >= 0: deferred x updates (x untouched; alpha to S->ahist[slot])
     T a, beta, alpha_prev;
     __device__ void load()
     {
@@ -1024,18 +1025,19 @@ struct OpCgA : Gate {
     struct Ox { T x; };
     __device__ Raw in_raw(long long c) const { return {src[c], pin[c]}; }
     __device__ T in_value(const Raw& r) const { return a * r.src + beta * r.pold; }
-    template <bool NT> __device__ Ox out_raw(long long c) const { return {ld<NT>(x + c)}; }
+    template <bool NT> __device__ Ox out_raw(long long c) const { return {slot < 0 ? ld<NT>(x + c) : T(0)}; }
     template <bool NT> __device__ void out(long long c, T vc, T Av, const Raw& r, const Ox& o, double* acc) const
     {
         put<NT>(pout + c, vc);
         put<NT>(Ap + c, Av);
-        put<NT>(x + c, o.x + alpha_prev * r.pold);
+        if (slot < 0) put<NT>(x + c, o.x + alpha_prev * r.pold);
         acc[0] += (double)vc * (double)Av;
     }
     __device__ void finish(const double* t)
     {
         S->v[S_PAP] = t[0];
         S->v[S_ALPHA] = S->v[S_RZ] / t[0];
+        if (slot >= 0) S->ahist[slot] = S->v[S_ALPHA];
     }
 };
 
@@ -1043,7 +1045,7 @@ struct OpCgA : Gate {
 // p_new on their halo from p_old, so p_old must stay intact during the launch.
 template <typename T>
 void launch_cg_A2(const Geo& g, hipStream_t s, RedBuf rb, const T* src, const T* pin, T* pout, T* Ap, T* x, Scal* S,
-                  bool pointwise)
+                  bool pointwise, int ring_slot)
 {
     OpCgA<T> op;
     op.gate = &S->done;
@@ -1054,6 +1056,7 @@ void launch_cg_A2(const Geo& g, hipStream_t s, RedBuf rb, const T* src, const T*
     op.x = x;
     op.S = S;
     op.pointwise = pointwise;
+    op.slot = ring_slot;
     run_stencil<T, 1>(g, s, op, rb);
 }
 
@@ -2150,7 +2153,8 @@ void launch_ir_inner_init(const Geo& g, hipStream_t s, RedBuf rb, float* r, floa
     template void launch_scale_box<T>(const Geo&, hipStream_t, T*, const Scal*, int);                          \
     template void launch_fill<T>(const Geo&, hipStream_t, T*, T);                                              \
     template void launch_ghost<T>(hipStream_t, const Layout&, const Faces&, T*, int, const Scal*, const int*);  \
-    template void launch_cg_A2<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, T*, T*, T*, Scal*, bool); \
+    template void launch_cg_A2<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, T*, T*, T*, Scal*, bool,  \
+                                  int);                                                                            \
     template void launch_cg_final_x<T>(const Geo&, hipStream_t, T*, const T*, const Scal*);                      \
     template void launch_cg_B<T>(const Geo&, hipStream_t, RedBuf, T*, const T*, Scal*, bool, bool);            \
     template void launch_cg_rz<T>(const Geo&, hipStream_t, RedBuf, const T*, const T*, Scal*);                 \

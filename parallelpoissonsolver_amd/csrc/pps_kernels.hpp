@@ -172,7 +172,7 @@ void launch_ghost(hipStream_t s, const Layout& L, const Faces& F, T* field, int 
 // p_new from p_old).
 template <typename T>
 void launch_cg_A2(const Geo& g, hipStream_t s, RedBuf rb, const T* src, const T* pin, T* pout, T* Ap, T* x, Scal* S,
-                  bool pointwise);
+                  bool pointwise, int ring_slot = -1);  // ring_slot >= 0: as launch_fcg_p_fused's
 // r -= alpha Ap, rr (and r.(jac r) when pointwise) -> err, beta, iter, done
 template <typename T>
 void launch_cg_B(const Geo& g, hipStream_t s, RedBuf rb, T* r, const T* Ap, Scal* S, bool pointwise, bool history);
