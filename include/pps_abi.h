@@ -227,6 +227,13 @@ int pps_set_tiling(void* ctx, int kz_stencil, int nontemporal);
  * (kernelsAlpakaChebyshev.hpp:191-273). */
 int pps_set_cheb_pairs(void* ctx, int pairs, int kz);
 
+/* CG and FCG with the learned preconditioner keep their search directions in
+ * a ring of 16 fields and add the steps to x in one pass per 16 iterations
+ * (x bit for bit as with a per-iteration update, fewer bytes per iteration),
+ * where the device holds the ring.  on = 0 updates x every iteration instead;
+ * 1 (the default) uses the ring.  No counterpart in the reference. */
+int pps_set_deferred_x(void* ctx, int on);
+
 /* ---- learned preconditioner (new: the reference has no counterpart) ------
  * A 3-level convolutional network (16 channels, 3^3 kernels, ReLU, BF16
  * weights/activations, FP32 accumulation) used as the PPS_NN stage below a

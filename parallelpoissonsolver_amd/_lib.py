@@ -131,6 +131,7 @@ SIGNATURES = {
     "pps_time_kernel": ([_VP, ctypes.c_int, ctypes.c_int, _DP, _DP], ctypes.c_int),
     "pps_set_tiling": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "pps_set_cheb_pairs": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "pps_set_deferred_x": ([_VP, ctypes.c_int], ctypes.c_int),
     "pps_nn_create": ([ctypes.c_uint64, ctypes.POINTER(_VP)], ctypes.c_int),
     "pps_nn_destroy": ([_VP], None),
     "pps_nn_last_error": ([_VP], ctypes.c_char_p),
@@ -336,6 +337,11 @@ class Context:
         self._chk(lib().pps_nn_time_layer(self.h, layer, prec, reps, ctypes.byref(ms), ctypes.byref(nb),
                                           ctypes.byref(fl)))
         return ms.value, nb.value, fl.value
+
+    def set_deferred_x(self, on=True):
+        """CG and learned-FCG x updates deferred over a ring of 16 directions (True, the default, where the
+        device holds it) or applied every iteration (False)."""
+        self._chk(lib().pps_set_deferred_x(self.h, int(bool(on))))
 
     def set_cheb_pairs(self, pairs=1, kz=0):
         """Chebyshev sweeps two per pass (1, the default) or one per pass (0); kz: z-chunk of the fused kernel."""

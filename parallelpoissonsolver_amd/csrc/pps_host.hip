@@ -118,6 +118,7 @@ struct Ctx {
     int nt_mode = 0;  // 0: automatic (nt_auto), 1: always, -1: never
     int nt_auto = 0;  // the fields' working set exceeds the last-level cache
     int cheb_pairs = 1;  // Chebyshev sweeps run two per pass (k_cheb_pair)
+    int deferred_x = 1;  // Krylov stages keep a ring of directions where the device holds it
     int cheb_kz = 0;     // z-chunk of the fused Chebyshev pair (0: chosen per launch)
     std::string err;
     std::shared_ptr<NNPrecond> nn;  // learned preconditioner of PPS_NN stages (pps_nn_attach)
@@ -189,6 +190,7 @@ struct Ctx {
     int ring(const std::string& key, int prec, Field first, Field last, Field* out, int* slots)
     {
         *slots = 0;
+        if (!deferred_x) return PPS_OK;
         bool have = true;
         for (int j = 1; j + 1 < kFcgRing; ++j)
             have = have && work.count(key + ".ring" + std::to_string(j) + (prec == 32 ? "/32" : "/64")) > 0;
@@ -1928,6 +1930,15 @@ int pps_set_tiling(void* ctx, int kz_stencil, int nontemporal)
     if (kz_stencil < 0) C->kz_st = 0;
     C->nt_mode = nontemporal;
     C->nt = nontemporal == 0 ? C->nt_auto : (nontemporal > 0 ? 1 : 0);
+    return PPS_OK;
+}
+
+int pps_set_deferred_x(void* ctx, int on)
+{
+    Ctx* C = as_ctx(ctx);
+    if (!C) return PPS_ERR_ARG;
+    if (on < 0 || on > 1) return C->fail(PPS_ERR_ARG, "on must be 0 or 1");
+    C->deferred_x = on;
     return PPS_OK;
 }
 

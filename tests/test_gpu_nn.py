@@ -351,3 +351,37 @@ def test_mixed_precision_learned_fcg(trained):
     assert r["error_compute_operator"] < 1e-10
     assert r["fallback_switches"] == 0
     C.close()
+
+
+def test_deferred_x_is_bit_identical(trained):
+    """The directions' ring (x's updates deferred to one pass per 16
+    iterations, adding the steps in the order they were taken) gives x bit for
+    bit as the per-iteration updates, with the same iteration counts: the FP64
+    learned FCG and plain FP64 CG with the true-residual finish (whose restart
+    resumes the ring where the device's iteration count says), and the FP32
+    learned FCG and FP32 CG under refinement; every solve runs longer than one
+    ring."""
+    net, _ = trained
+    C = _ctx(48)
+    net.attach(C)
+    chains = {
+        "fcg64": [dict(kind=L.FCG, tol=1e-10, maxit=5000, true_finish=3), dict(kind=L.NN)],
+        "ir_fcg32": [dict(kind=L.IR, tol=1e-10, maxit=50), dict(kind=L.FCG, precision=32, tol=1e-5, maxit=5000),
+                     dict(kind=L.NN, precision=32)],
+        "cg64": [dict(kind=L.CG, tol=1e-10, maxit=5000, true_finish=3), dict(kind=L.NONE)],
+        "ir_cg32": [dict(kind=L.IR, tol=1e-10, maxit=50), dict(kind=L.CG, precision=32, tol=1e-5, maxit=5000),
+                    dict(kind=L.NONE, precision=32)],
+    }
+    for name, chain in chains.items():
+        got = {}
+        for on in (True, False):
+            C.set_deferred_x(on)
+            r = _solve(C, chain)
+            got[on] = (C.download(L.FIELD_X), r["iterations"], r["error_compute_operator"])
+        print(f"[nn] deferred x, {name}: {got[True][1]} iterations, true residual {got[True][2]:.3e} "
+              f"(per-iteration updates: {got[False][1]}, {got[False][2]:.3e})", flush=True)
+        assert got[True][1] == got[False][1] > 16
+        assert np.array_equal(got[True][0], got[False][0])
+        assert got[True][2] < 1e-10
+    C.set_deferred_x(True)
+    C.close()
