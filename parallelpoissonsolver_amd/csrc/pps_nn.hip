@@ -1080,7 +1080,7 @@ __global__ __launch_bounds__(kWThreads) __attribute__((amdgpu_waves_per_eu(4, 4)
                 for (int c = 0; c < 4; ++c) asm volatile("" : "+v"(ps[s][ip][c]));
             __builtin_amdgcn_sched_barrier(0);
         }
-        if ((z & 1) || z == D - 1) {  // uniform
+        if (a.pout.p && ((z & 1) || z == D - 1)) {  // uniform
             const __amdgpu_buffer_rsrc_t rp = buf_rsrc(a.pout.p + vox(a.pout, b, 0, 0, z >> 1), kInRange);
 #pragma unroll
             for (int s = 0; s < 2; ++s)
